@@ -1,0 +1,169 @@
+"""SGEMM throughput on one MI355X through the libhbsgemm C-ABI.
+
+A step is one hbs_sgemm call over the whole batch: NITER = 10 distinct
+row-major N x N float32 problems (N = 8192), all in one launch, as the
+reference's kernel computes its NITER slices (apps/sgemm/kernel.cpp:293-297).
+Inputs are synthetic uniform[-1,1) from a fixed seed, generated on the device
+and resident in HBM before the timed region.
+
+Prints one JSON line: whole-job TFLOP/s, the roofline of the dominant kernel
+(timed with HIP events on the stream it is launched on), the FP64-verified
+error of the result, and the reference's CPU golden (host_mm restated in the
+oracle, single thread) timed on a bounded row sample on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_TFLOPS = {"fp32": 157.3, "bf16": 2516.6, "fp8": 5033.2}
+METRIC = "SGEMM TFLOP/s N=8192: FP32 vs BF16x3 at FP32-level error; % MFMA peak"
+
+
+def parse():
+    p = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--variant", choices=["fp32", "bf16x3"], default="fp32")
+    p.add_argument("--n", type=int, default=8192)
+    p.add_argument("--niter", type=int, default=10)
+    p.add_argument("--seed", type=int, default=20241)
+    p.add_argument("--cpu-seconds", type=float, default=15.0,
+                   help="target duration of the CPU golden sample")
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    p.add_argument("--no-verify", action="store_true", help="skip the FP64 error check")
+    return p.parse_args()
+
+
+def cpu_baseline(N, target_s):
+    """The reference's golden host_mm (apps/sgemm/main.cpp:15-26) restated in
+    the oracle, single-threaded as in the reference, on a bounded row sample of
+    one N x N slice: TFLOP/s = 2 * rows * N^2 / t."""
+    import numpy as np
+
+    import oracle
+    A = oracle.gen("uniform", 1, N * N).reshape(N, N)
+    B = oracle.gen("uniform", 2, N * N).reshape(N, N)
+    C = np.zeros((N, N), np.float32)
+    lib = oracle.lib()
+    # calibrate on one row, then size the sample to about target_s seconds
+    t0 = time.perf_counter()
+    lib.orc_host_mm_rows(C, A, B, N, 0, 1)
+    t1 = time.perf_counter() - t0
+    rows = max(1, min(N - 1, int(target_s / max(t1, 1e-6))))
+    t0 = time.perf_counter()
+    lib.orc_host_mm_rows(C, A, B, N, 1, 1 + rows)
+    dt = time.perf_counter() - t0
+    return {
+        "value": 2.0 * rows * N * N / dt / 1e12,
+        "unit": "TFLOP/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"host_mm rows 1..{rows} of one {N}x{N} uniform slice "
+                  f"({rows}/{N} of a slice, {dt:.1f} s single thread, -O2 -ffp-contract=off)",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(this build runs the path on a single MI355X)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    import hb_hammerbench_amd as hb
+
+    torch.cuda.set_device(0)
+    N, NITER = args.n, args.niter
+    n_total = N * N * NITER
+    a = torch.empty(n_total, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    c = torch.empty_like(a)
+    kind = hb.GEN_UNIFORM
+    hb.fill_random(a, n_total, kind, args.seed)
+    hb.fill_random(b, n_total, kind, args.seed + 1)
+    ws = None
+    if args.variant == "bf16x3":
+        ws = torch.empty(max(1, hb.bf16x3_workspace_size(N, NITER)), dtype=torch.uint8,
+                         device="cuda")
+    torch.cuda.synchronize()
+
+    def step():
+        if args.variant == "fp32":
+            hb.sgemm(a, b, c, N, NITER)
+        else:
+            hb.sgemm_bf16x3(a, b, c, N, NITER, workspace=ws)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    stream = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s, e in ev:
+        s.record(stream)
+        step()
+        e.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    per_launch_ms = [s.elapsed_time(e) for s, e in ev]
+    avg_ms = sum(per_launch_ms) / len(per_launch_ms)
+
+    flops = 2.0 * N * N * N * NITER
+    value = flops * args.steps / wall / 1e12
+    achieved = flops / (avg_ms / 1e3) / 1e12
+    peak = PEAK_TFLOPS["fp32"] if args.variant == "fp32" else PEAK_TFLOPS["bf16"] / 3.0
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.variant == "fp32" else "bf16x3->f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{'plain FP32 MFMA' if args.variant == 'fp32' else 'BF16x3 split'} "
+                        f"SGEMM, square N={N}, NITER={NITER} slices per launch, "
+                        f"uniform[-1,1) seed {args.seed}",
+            "N": N, "niter": NITER, "variant": args.variant,
+            "flops_per_step": flops,
+        },
+        "roofline": {
+            "bound": "mfma",
+            "achieved": round(achieved, 3),
+            "peak": round(peak, 1),
+            "unit": "TFLOP/s",
+            "frac": round(achieved / peak, 4),
+            "traffic": None,
+            "kernel_ms_avg": round(avg_ms, 4),
+        },
+    }
+    if not args.no_verify:
+        it = NITER - 1
+        sl = slice(it * N * N, (it + 1) * N * N)
+        st = hb.verify_f64(a[sl], b[sl], c[sl], N, N, N)
+        line["accuracy"] = {
+            "slice": it, "e_cw": st["e_cw"], "e_nw": st["e_nw"], "e_F": st["e_F"],
+            "gate_e_cw": 1e-5 * N ** 0.5, "oracle": "FP64 on device (hbs_verify_f64)",
+        }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(N, args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,162 @@
+"""hb_hammerbench_amd -- MI355X (gfx950) implementation of HammerBench's apps/sgemm path.
+
+The product is the C-ABI library ``libhbsgemm.so`` (declared in include/hbsgemm.h)
+and the C++ launcher under ``apps/sgemm``.  This module is a thin ctypes binding
+used by the tests and bench.py; torch supplies device memory and streams only.
+
+Reference call shape (apps/sgemm/main.cpp:30-149, kernel.cpp:279-333):
+``kernel(mat1, mat2, result, pod_id)`` computes ``result_i = mat1_i . mat2_i`` for
+NITER row-major N x N slices stored back to back.  :func:`sgemm` is that call.
+Errors surface as :class:`HbsError` carrying the library status code, the
+counterpart of ``BSG_CUDA_CALL`` returning early on a non-success code.
+"""
+import ctypes
+import os
+
+__all__ = [
+    "HbsError", "lib", "library_path", "sgemm", "sgemm_strided", "sgemm_bf16x3",
+    "bf16x3_workspace_size", "fill_pattern", "fill_random", "verify_f64",
+    "GEN_UNIFORM", "GEN_LOGUNIFORM", "GEN_NORMAL_OUTLIER",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_NAME = "libhbsgemm.so"
+
+GEN_UNIFORM, GEN_LOGUNIFORM, GEN_NORMAL_OUTLIER = 0, 1, 2
+
+_lib = None
+
+
+class HbsError(RuntimeError):
+    """A non-success status code from libhbsgemm."""
+
+    def __init__(self, code, where):
+        self.code = code
+        msg = lib().hbs_strerror(code).decode()
+        super().__init__(f"{where}: status {code}: {msg}")
+
+
+class ErrStats(ctypes.Structure):
+    _fields_ = [("e_cw", ctypes.c_double), ("e_nw", ctypes.c_double),
+                ("e_F", ctypes.c_double), ("max_abs_err", ctypes.c_double),
+                ("max_abs_ref", ctypes.c_double), ("sum_sq_err", ctypes.c_double),
+                ("sum_sq_ref", ctypes.c_double), ("nonfinite", ctypes.c_int64)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+def library_path():
+    return os.path.join(_HERE, _LIB_NAME)
+
+
+_vp, _i, _i64, _u64, _sz = (ctypes.c_void_p, ctypes.c_int, ctypes.c_int64,
+                            ctypes.c_uint64, ctypes.c_size_t)
+
+# name -> (restype, argtypes); every symbol include/hbsgemm.h declares.
+SIGNATURES = {
+    "hbs_strerror": (ctypes.c_char_p, [_i]),
+    "hbs_version": (_i, []),
+    "hbs_sgemm": (_i, [_vp, _vp, _vp, _i, _i, _vp]),
+    "hbs_sgemm_strided": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i64, _i64, _i64, _i, _vp]),
+    "hbs_sgemm_bf16x3_workspace_size": (_sz, [_i, _i]),
+    "hbs_sgemm_bf16x3": (_i, [_vp, _vp, _vp, _i, _i, _vp, _sz, _vp]),
+    "hbs_fill_pattern": (_i, [_vp, _vp, _i64, _i64, _vp]),
+    "hbs_fill_random": (_i, [_vp, _i64, _i, _u64, _u64, _vp]),
+    "hbs_verify_f64": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _i,
+                            ctypes.POINTER(ErrStats), _vp]),
+}
+
+
+def lib():
+    """Load the in-tree libhbsgemm.so; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        path = library_path()
+        if not os.path.exists(path):
+            raise ImportError(f"{path} is missing: run __graft_entry__.build() "
+                              "(make -C hb_hammerbench_amd/csrc)")
+        L = ctypes.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(code, where):
+    if code != 0:
+        raise HbsError(code, where)
+
+
+def _stream(stream):
+    if stream is None:
+        import torch
+        return torch.cuda.current_stream().cuda_stream
+    return getattr(stream, "cuda_stream", stream)
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    return t.data_ptr() if hasattr(t, "data_ptr") else t
+
+
+def sgemm(mat1, mat2, result, N, niter=1, stream=None):
+    """result_i = mat1_i . mat2_i for i < niter N x N row-major slices (kernel.cpp:279-333)."""
+    _check(lib().hbs_sgemm(_ptr(mat1), _ptr(mat2), _ptr(result), N, niter, _stream(stream)),
+           "hbs_sgemm")
+
+
+def sgemm_strided(A, B, C, M, N, K, lda, ldb, ldc, strideA=0, strideB=0, strideC=0, batch=1,
+                  stream=None):
+    _check(lib().hbs_sgemm_strided(_ptr(A), _ptr(B), _ptr(C), M, N, K, lda, ldb, ldc,
+                                   strideA, strideB, strideC, batch, _stream(stream)),
+           "hbs_sgemm_strided")
+
+
+def bf16x3_workspace_size(N, niter=1):
+    return int(lib().hbs_sgemm_bf16x3_workspace_size(N, niter))
+
+
+def sgemm_bf16x3(mat1, mat2, result, N, niter=1, workspace=None, stream=None):
+    """BF16x3 split-precision SGEMM with the hbs_sgemm call shape.
+
+    ``workspace`` is a device buffer of at least bf16x3_workspace_size(N, niter)
+    bytes; when None one is allocated with torch (outside any timed region).
+    """
+    ws_bytes = bf16x3_workspace_size(N, niter)
+    if workspace is None:
+        import torch
+        workspace = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device="cuda")
+    nbytes = workspace.numel() * workspace.element_size() if hasattr(workspace, "numel") else ws_bytes
+    _check(lib().hbs_sgemm_bf16x3(_ptr(mat1), _ptr(mat2), _ptr(result), N, niter,
+                                  _ptr(workspace), nbytes, _stream(stream)),
+           "hbs_sgemm_bf16x3")
+
+
+def fill_pattern(mat1, mat2, n, first=0, stream=None):
+    """mat1[i] = i % 7, mat2[i] = i % 3 (apps/sgemm/main.cpp:46-52)."""
+    _check(lib().hbs_fill_pattern(_ptr(mat1), _ptr(mat2), n, first, _stream(stream)),
+           "hbs_fill_pattern")
+
+
+def fill_random(dst, n, kind, seed, first=0, stream=None):
+    _check(lib().hbs_fill_random(_ptr(dst), n, kind, seed, first, _stream(stream)),
+           "hbs_fill_random")
+
+
+def verify_f64(A, B, C, M, N, K, lda=None, ldb=None, ldc=None, row_begin=0, row_end=None,
+               stream=None):
+    """FP64 error statistics of C against A.B computed on the device (synchronous)."""
+    out = ErrStats()
+    _check(lib().hbs_verify_f64(_ptr(A), _ptr(B), _ptr(C), M, N, K,
+                                K if lda is None else lda, N if ldb is None else ldb,
+                                N if ldc is None else ldc, row_begin,
+                                M if row_end is None else row_end, ctypes.byref(out),
+                                _stream(stream)),
+           "hbs_verify_f64")
+    return out.as_dict()

@@ -1,0 +1,56 @@
+// hbs_internal.h -- shared host/device helpers for libhbsgemm (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/hbsgemm.h"
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+#define HBS_HIP_TRY(expr)                                   \
+    do {                                                    \
+        hipError_t _e = (expr);                             \
+        if (_e != hipSuccess) return (int)_e;               \
+    } while (0)
+
+namespace hbs {
+
+// Checks that the current device is a gfx950 GPU (cached per device).
+int check_device();
+
+// Returns the launch error of the last kernel launch as a status code.
+inline int launch_status() {
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? HBS_SUCCESS : (int)e;
+}
+
+inline bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
+
+// Workgroup id remap so that each XCD (workgroups are dealt to the 8 XCDs
+// round-robin by id) receives one contiguous range of logical tile ids, which
+// keeps the operand panels a group of neighbouring tiles shares in that XCD's L2.
+__device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
+    constexpr int kXcd = 8;
+    const int per = nblocks / kXcd;
+    const int full = per * kXcd;
+    if (bid >= full) return bid;  // tail that does not divide evenly
+    return (bid % kXcd) * per + bid / kXcd;
+}
+
+// Grouped raster: logical tile id -> (tile row, tile col) walking `group`
+// tile-rows column by column, so concurrently resident tiles share A panels
+// (same rows) and B panels (same cols).
+__device__ __forceinline__ void tile_coords(int t, int tiles_m, int tiles_n, int group, int &tm,
+                                            int &tn) {
+    const int per_group = group * tiles_n;
+    const int g = t / per_group;
+    const int first_m = g * group;
+    const int gm = min(tiles_m - first_m, group);
+    const int r = t - g * per_group;
+    tm = first_m + r % gm;
+    tn = r / gm;
+}
+
+}  // namespace hbs

@@ -1,0 +1,113 @@
+/*
+ * hbsgemm.h -- C-ABI of libhbsgemm.so, the MI355X (gfx950) implementation of
+ * HammerBench's apps/sgemm path.
+ *
+ * Boundary it replaces (reference, apps/sgemm):
+ *   device entry  extern "C" int kernel(float *mat1, float *mat2, float *result,
+ *                                       int pod_id)            kernel.cpp:279-280
+ *   host runtime  hb_mc_device_malloc / transfer_data_to_device /
+ *                 kernel_enqueue / pods_kernels_execute        main.cpp:61-143
+ * On one GPU the runtime calls become HIP allocation, copies and stream order,
+ * owned by the caller; this library only launches work on a caller's stream.
+ *
+ * Conventions (all entry points):
+ *  - matrices are row-major float32 in device memory owned by the caller;
+ *  - every call is asynchronous on `stream` (a hipStream_t, NULL = default
+ *    stream) unless stated otherwise; the caller synchronises, as
+ *    hb_mc_device_pods_kernels_execute blocks in main.cpp:98;
+ *  - the return value is HBS_SUCCESS (0), a negative HBS_ERR_* code for an
+ *    invalid argument (nothing is launched), or a positive HIP runtime error
+ *    code -- the counterpart of HB_MC_SUCCESS / BSG_CUDA_CALL (main.cpp:61-148);
+ *  - no entry point allocates device memory on the GEMM path; scratch space
+ *    is passed in as a workspace whose size the *_workspace_size call returns;
+ *  - calls on distinct streams are independent; calls sharing a workspace
+ *    must be ordered by the caller.
+ */
+#ifndef HBSGEMM_H
+#define HBSGEMM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    HBS_SUCCESS = 0,
+    HBS_ERR_ARG = -1,         /* size or argument out of range */
+    HBS_ERR_NULL = -2,        /* null pointer */
+    HBS_ERR_ALIGN = -3,       /* pointer or leading dimension misaligned */
+    HBS_ERR_WORKSPACE = -4,   /* workspace missing or too small */
+    HBS_ERR_DEVICE = -5,      /* current device is not a gfx950 GPU */
+    HBS_ERR_UNSUPPORTED = -6  /* valid request this build does not handle */
+};
+
+/* Human-readable text for any status code (never NULL). */
+const char *hbs_strerror(int code);
+/* Library version, major * 10000 + minor * 100 + patch. */
+int hbs_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Plain FP32 SGEMM on FP32 matrix cores.                                     */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces kernel(mat1, mat2, result, pod_id), apps/sgemm/kernel.cpp:279-333:
+ * C_i = A_i . B_i for i < niter, each an N x N row-major slice stored back to
+ * back with stride N*N (kernel.cpp:293-297), all in one launch.  N and NITER
+ * are compile-time macros in the reference (template.mk:30-31,54-55); here
+ * they are arguments.  Any N >= 1 is accepted (the reference needs N % 16 == 0,
+ * kernel.cpp:9-10).  Requires niter >= 1 and niter * N * N < 2^62. */
+int hbs_sgemm(const float *A, const float *B, float *C, int N, int niter, void *stream);
+
+/* Rectangular, strided generalisation (apps/legacy/gemm/sgemm_data_parallel
+ * main.cpp:42-74 uses M x N x P tensors with row strides):
+ * C_b[M x N] = A_b[M x K] . B_b[K x N] for b < batch, row-major with leading
+ * dimensions lda >= K, ldb >= N, ldc >= N and element strides between batch
+ * entries.  M, N, K >= 1. */
+int hbs_sgemm_strided(const float *A, const float *B, float *C, int M, int N, int K,
+                      int lda, int ldb, int ldc, int64_t strideA, int64_t strideB,
+                      int64_t strideC, int batch, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Input generation on the device.                                            */
+/* ------------------------------------------------------------------------ */
+
+/* apps/sgemm/main.cpp:46-52: mat1[i] = i % 7, mat2[i] = i % 3 for global index
+ * i = first .. first + n - 1 (written to mat1[0..n), mat2[0..n)). */
+int hbs_fill_pattern(float *mat1, float *mat2, int64_t n, int64_t first, void *stream);
+
+/* Seeded counter-based inputs: element j is a function of (seed, first + j).
+ * kind HBS_GEN_UNIFORM: uniform[-1,1); HBS_GEN_LOGUNIFORM: |x| = 10^(-3+6u)
+ * with random sign; HBS_GEN_NORMAL_OUTLIER: normal(0,1) with 1% of entries
+ * scaled by 1e3. */
+enum { HBS_GEN_UNIFORM = 0, HBS_GEN_LOGUNIFORM = 1, HBS_GEN_NORMAL_OUTLIER = 2 };
+int hbs_fill_random(float *dst, int64_t n, int kind, uint64_t seed, uint64_t first,
+                    void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* FP64 verification on the GPU (the validation of main.cpp:102-135 with an   */
+/* FP64 reference instead of the float host golden).  Synchronous.           */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    double e_cw;        /* max_ij |C - R| / (|A||B|)_ij                 */
+    double e_nw;        /* max_ij |C - R| / max_ij |R|                  */
+    double e_F;         /* ||C - R||_F / ||R||_F                        */
+    double max_abs_err; /* max_ij |C - R|                               */
+    double max_abs_ref; /* max_ij |R|                                   */
+    double sum_sq_err;  /* sum_ij (C - R)^2                             */
+    double sum_sq_ref;  /* sum_ij R^2                                   */
+    int64_t nonfinite;  /* entries of C that are Inf or NaN             */
+} hbs_errstats;
+
+/* R = A.B and (|A||B|) computed in FP64 on the device for rows
+ * [row_begin, row_end) of C (row_end <= M; pass 0, M for all), compared with C.
+ * Blocks the calling thread until the statistics are in *out. */
+int hbs_verify_f64(const float *A, const float *B, const float *C, int M, int N, int K,
+                   int lda, int ldb, int ldc, int row_begin, int row_end,
+                   hbs_errstats *out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,91 @@
+"""C-ABI contract checks that need no GPU: the library loads, exports every
+symbol include/hbsgemm.h declares, and rejects invalid arguments with the
+documented status codes before touching a device (the counterpart of
+BSG_CUDA_CALL returning early, apps/sgemm/main.cpp:61-143)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import hb_hammerbench_amd as hb
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "hbsgemm.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(hbs_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_expected_entry_points():
+    syms = declared_symbols()
+    for s in ["hbs_sgemm", "hbs_sgemm_strided", "hbs_fill_pattern", "hbs_fill_random",
+              "hbs_verify_f64", "hbs_strerror", "hbs_version"]:
+        assert s in syms
+
+
+def test_library_exports_every_declared_symbol():
+    L = ctypes.CDLL(hb.library_path())
+    missing = [s for s in declared_symbols() if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_python_binding_covers_every_declared_symbol():
+    assert set(declared_symbols()) <= set(hb.SIGNATURES)
+
+
+def test_strerror_and_version():
+    L = hb.lib()
+    assert L.hbs_version() >= 100
+    for code in [0, -1, -2, -3, -4, -5, -6]:
+        assert L.hbs_strerror(code)
+    assert L.hbs_strerror(-99) == b"unknown status code"
+
+
+P = ctypes.c_void_p(16)  # non-null dummy pointer; never dereferenced on these paths
+
+
+@pytest.mark.parametrize("N,niter", [(0, 1), (-3, 1), (16, 0), (16, -1)])
+def test_sgemm_rejects_bad_sizes(N, niter):
+    assert hb.lib().hbs_sgemm(P, P, P, N, niter, None) == -1
+
+
+def test_sgemm_rejects_null_pointers():
+    L = hb.lib()
+    assert L.hbs_sgemm(None, P, P, 16, 1, None) == -2
+    assert L.hbs_sgemm(P, None, P, 16, 1, None) == -2
+    assert L.hbs_sgemm(P, P, None, 16, 1, None) == -2
+
+
+def test_sgemm_rejects_overflowing_batch():
+    assert hb.lib().hbs_sgemm(P, P, P, 1 << 30, 1 << 30, None) == -1
+
+
+def test_strided_rejects_bad_leading_dimensions():
+    L = hb.lib()
+    assert L.hbs_sgemm_strided(P, P, P, 8, 8, 8, 7, 8, 8, 0, 0, 0, 1, None) == -1
+    assert L.hbs_sgemm_strided(P, P, P, 8, 8, 8, 8, 7, 8, 0, 0, 0, 1, None) == -1
+    assert L.hbs_sgemm_strided(P, P, P, 8, 8, 8, 8, 8, 7, 0, 0, 0, 1, None) == -1
+    assert L.hbs_sgemm_strided(P, P, P, 8, 8, 8, 8, 8, 8, -1, 0, 0, 1, None) == -1
+    assert L.hbs_sgemm_strided(P, P, P, 8, 8, 8, 8, 8, 8, 0, 0, 0, 0, None) == -1
+
+
+def test_fill_and_verify_reject_bad_arguments():
+    L = hb.lib()
+    assert L.hbs_fill_pattern(P, P, -1, 0, None) == -1
+    assert L.hbs_fill_pattern(P, P, 0, 0, None) == 0  # empty fill is a no-op
+    assert L.hbs_fill_random(P, 10, 7, 1, 0, None) == -1
+    assert L.hbs_fill_random(None, 10, 0, 1, 0, None) == -2
+    out = hb.ErrStats()
+    assert L.hbs_verify_f64(P, P, P, 8, 8, 8, 8, 8, 8, 0, 9, ctypes.byref(out), None) == -1
+    assert L.hbs_verify_f64(P, P, P, 8, 8, 8, 8, 8, 8, 4, 4, ctypes.byref(out), None) == -1
+    assert L.hbs_verify_f64(P, P, P, 8, 8, 8, 8, 8, 8, 0, 8, None, None) == -2
+
+
+def test_python_wrapper_raises_hbs_error():
+    with pytest.raises(hb.HbsError) as ei:
+        hb.sgemm(P.value, P.value, P.value, 0, 1, stream=0)
+    assert ei.value.code == -1

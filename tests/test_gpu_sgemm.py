@@ -1,0 +1,209 @@
+"""GPU parity of the plain FP32-MFMA SGEMM against the oracle, through the C-ABI.
+
+Bars: bit-exact on the reference's integer pattern (apps/sgemm/main.cpp:46-52,
+every partial sum is an integer < 2^24); on random inputs every element within
+the deterministic FP32 summation bound |C - R| <= gamma_{K+2} (|A||B|) of the
+FP64 oracle R (gamma_n = n u / (1 - n u), u = 2^-24), which any order of FP32
+additions satisfies, plus the reference's own SSE < 0.01 check against the
+host_mm golden (main.cpp:120-134).
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+torch = pytest.importorskip("torch")
+import hb_hammerbench_amd as hb  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+U = 2.0 ** -24
+
+
+def gamma(n):
+    return n * U / (1 - n * U)
+
+
+def dev_empty(n):
+    return torch.empty(int(n), dtype=torch.float32, device="cuda")
+
+
+def dev_pattern(N, niter):
+    a = dev_empty(niter * N * N); b = dev_empty(niter * N * N)
+    hb.fill_pattern(a, b, niter * N * N)
+    return a, b
+
+
+def dev_uniform(n, seed, first=0):
+    x = dev_empty(n)
+    hb.fill_random(x, n, hb.GEN_UNIFORM, seed, first)
+    return x
+
+
+def pattern_closed_form_gpu(N, it):
+    """Closed form of the main.cpp:46-52 product as a rank-21 product of small
+    integer matrices (exact in float64): C = P . Q with P[y][r] = cnt(r) *
+    ((o + N y + r) mod 7) and Q[r][x] = ((o + x) + (N mod 3) r) mod 3."""
+    o = it * N * N
+    r = torch.arange(21, dtype=torch.int64, device="cuda")
+    cnt = N // 21 + (r < N % 21).to(torch.int64)
+    y = torch.arange(N, dtype=torch.int64, device="cuda")
+    P = cnt[None, :] * ((o + N * y[:, None] + r[None, :]) % 7)
+    Q = ((o + y[None, :]) + (N % 3) * r[:, None]) % 3
+    return (P.double() @ Q.double()).float()
+
+
+# --- generators ---------------------------------------------------------------------
+
+def test_fill_pattern_matches_reference_init():
+    n = 2 * 512 * 512
+    a, b = dev_pattern(512, 2)
+    ea, eb = oracle.pattern_init(n)
+    assert np.array_equal(a.cpu().numpy(), ea) and np.array_equal(b.cpu().numpy(), eb)
+    a2 = dev_empty(1000); b2 = dev_empty(1000)
+    hb.fill_pattern(a2, b2, 1000, first=123457)
+    ea, eb = oracle.pattern_init(1000, first=123457)
+    assert np.array_equal(a2.cpu().numpy(), ea) and np.array_equal(b2.cpu().numpy(), eb)
+
+
+def test_fill_random_matches_oracle_generators():
+    n = 1 << 18
+    x = dev_empty(n)
+    hb.fill_random(x, n, hb.GEN_UNIFORM, 2024, 77)
+    assert np.array_equal(x.cpu().numpy(), oracle.gen("uniform", 2024, n, 77))
+    for kind, name in [(hb.GEN_LOGUNIFORM, "loguniform"), (hb.GEN_NORMAL_OUTLIER, "normal_outlier")]:
+        hb.fill_random(x, n, kind, 99, 5)
+        g = x.cpu().numpy(); o = oracle.gen(name, 99, n, 5)
+        # device and host libm may differ by an ulp in double; after rounding
+        # to float that leaves at most a 1-ulp difference on rare elements
+        ulps = np.abs(g.view(np.int32).astype(np.int64) - o.view(np.int32).astype(np.int64))
+        assert ulps.max() <= 1 and (ulps > 0).mean() < 1e-3, (name, ulps.max(), (ulps > 0).mean())
+
+
+# --- the reference configuration: N_512__NITER_2 ----------------------------------------
+
+def test_sgemm_reference_pattern_N512_NITER2_bit_exact():
+    N, NITER = 512, 2
+    a, b = dev_pattern(N, NITER)
+    c = dev_empty(NITER * N * N)
+    hb.sgemm(a, b, c, N, NITER)
+    torch.cuda.synchronize()
+    got = c.cpu().numpy().reshape(NITER, N, N)
+    ha, hbm = oracle.pattern_init(NITER * N * N)
+    for it in range(NITER):
+        sl = slice(it * N * N, (it + 1) * N * N)
+        golden = oracle.host_mm(ha[sl].reshape(N, N), hbm[sl].reshape(N, N))  # main.cpp:54-56
+        assert np.array_equal(golden, oracle.pattern_slice(N, it))
+        sse, ndiff = oracle.sse(golden, got[it])                             # main.cpp:120-134
+        assert sse == 0.0 and ndiff == 0
+        assert np.array_equal(got[it], golden)
+
+
+def test_sgemm_uniform_N512_NITER2_within_fp32_bound():
+    N, NITER = 512, 2
+    a = dev_uniform(NITER * N * N, 1); b = dev_uniform(NITER * N * N, 2)
+    c = dev_empty(NITER * N * N)
+    hb.sgemm(a, b, c, N, NITER)
+    torch.cuda.synchronize()
+    A = a.cpu().numpy().reshape(NITER, N, N); B = b.cpu().numpy().reshape(NITER, N, N)
+    C = c.cpu().numpy().reshape(NITER, N, N)
+    for it in range(NITER):
+        R, S = oracle.gemm_f64(A[it], B[it])
+        err = np.abs(C[it].astype(np.float64) - R)
+        assert np.all(err <= gamma(N + 2) * S)
+        e = oracle.errors(C[it], R, S)
+        assert e["e_cw"] <= 1e-5 * np.sqrt(N)
+        golden = oracle.host_mm(A[it], B[it])
+        sse, _ = oracle.sse(golden, C[it])
+        assert sse < 0.01
+
+
+@pytest.mark.parametrize("N,niter", [(1, 1), (7, 2), (16, 1), (100, 3), (129, 1), (300, 2), (640, 1)])
+def test_sgemm_ragged_sizes(N, niter):
+    a, b = dev_pattern(N, niter)
+    c = dev_empty(niter * N * N)
+    hb.sgemm(a, b, c, N, niter)
+    torch.cuda.synchronize()
+    got = c.cpu().numpy().reshape(niter, N, N)
+    for it in range(niter):
+        assert np.array_equal(got[it], oracle.pattern_slice(N, it)), (N, it)
+    a = dev_uniform(niter * N * N, 3); b = dev_uniform(niter * N * N, 4)
+    hb.sgemm(a, b, c, N, niter)
+    torch.cuda.synchronize()
+    A = a.cpu().numpy().reshape(niter, N, N); B = b.cpu().numpy().reshape(niter, N, N)
+    C = c.cpu().numpy().reshape(niter, N, N)
+    for it in range(niter):
+        R, S = oracle.gemm_f64(A[it], B[it])
+        assert np.all(np.abs(C[it] - R) <= gamma(N + 2) * S), (N, it)
+
+
+@pytest.mark.parametrize("M,N,K,pad", [(200, 136, 72, (5, 3, 1)), (256, 256, 256, (4, 4, 4)),
+                                       (64, 512, 1000, (0, 0, 0)), (513, 65, 33, (2, 0, 7))])
+def test_sgemm_strided_rectangular(M, N, K, pad):
+    batch = 3
+    lda, ldb, ldc = K + pad[0], N + pad[1], N + pad[2]
+    sA, sB, sC = M * lda + 8, K * ldb + 4, M * ldc + 12
+    a = dev_uniform(batch * sA, 5); b = dev_uniform(batch * sB, 6)
+    c = torch.full((batch * sC,), -7.0, dtype=torch.float32, device="cuda")
+    hb.sgemm_strided(a, b, c, M, N, K, lda, ldb, ldc, sA, sB, sC, batch)
+    torch.cuda.synchronize()
+    Ah, Bh, Ch = a.cpu().numpy(), b.cpu().numpy(), c.cpu().numpy()
+    written = np.zeros(batch * sC, bool)
+    for i in range(batch):
+        A = np.lib.stride_tricks.as_strided(Ah[i * sA:], (M, K), (lda * 4, 4)).copy()
+        B = np.lib.stride_tricks.as_strided(Bh[i * sB:], (K, N), (ldb * 4, 4)).copy()
+        C = np.lib.stride_tricks.as_strided(Ch[i * sC:], (M, N), (ldc * 4, 4)).copy()
+        R, S = oracle.gemm_f64(A, B)
+        assert np.all(np.abs(C - R) <= gamma(K + 2) * S)
+        for r in range(M):
+            written[i * sC + r * ldc: i * sC + r * ldc + N] = True
+    assert np.all(Ch[~written] == -7.0)  # padding and gaps untouched
+
+
+def test_verify_f64_matches_oracle_metrics():
+    M, N, K = 256, 192, 320
+    a = dev_uniform(M * K, 7); b = dev_uniform(K * N, 8)
+    c = dev_empty(M * N)
+    hb.sgemm_strided(a, b, c, M, N, K, K, N, N)
+    c[5] += 0.125  # a planted error must be seen by both
+    st = hb.verify_f64(a, b, c, M, N, K)
+    A = a.cpu().numpy().reshape(M, K); B = b.cpu().numpy().reshape(K, N)
+    C = c.cpu().numpy().reshape(M, N)
+    R, S = oracle.gemm_f64(A, B)
+    e = oracle.errors(C, R, S)
+    for k in ["e_cw", "e_nw", "e_F", "max_abs_err", "max_abs_ref"]:
+        assert st[k] == pytest.approx(e[k], rel=1e-9), k
+    assert st["max_abs_err"] >= 0.12 and st["nonfinite"] == 0
+    part = hb.verify_f64(a, b, c, M, N, K, row_begin=64, row_end=128)
+    e2 = oracle.errors(C[64:128], R[64:128], S[64:128])
+    assert part["e_cw"] == pytest.approx(e2["e_cw"], rel=1e-9)
+
+
+# --- full size: N = 8192, NITER = 10 -------------------------------------------------
+
+def test_sgemm_full_size_pattern_bit_exact_and_uniform_sampled():
+    N, NITER = 8192, 10
+    a, b = dev_pattern(N, NITER)
+    c = dev_empty(NITER * N * N)
+    hb.sgemm(a, b, c, N, NITER)
+    torch.cuda.synchronize()
+    C = c.view(NITER, N, N)
+    for it in range(NITER):
+        expect = pattern_closed_form_gpu(N, it)
+        assert torch.equal(C[it], expect), it
+    for (it, y, x) in [(0, 0, 0), (9, 8191, 8191), (4, 1234, 5678)]:
+        assert C[it, y, x].item() == oracle.pattern_value(N, it, y, x)
+    del a, b
+    # uniform inputs: FP64 oracle on a row sample of the last slice
+    a = dev_uniform(NITER * N * N, 11); b = dev_uniform(NITER * N * N, 12)
+    hb.sgemm(a, b, c, N, NITER)
+    torch.cuda.synchronize()
+    it, rows = NITER - 1, slice(4096, 4096 + 32)
+    A = a.view(NITER, N, N)[it, rows].cpu().numpy()
+    B = b.view(NITER, N, N)[it].cpu().numpy()
+    R, S = oracle.gemm_f64(A, B)
+    Cs = C[it, rows].cpu().numpy()
+    assert np.all(np.abs(Cs - R) <= gamma(N + 2) * S)
+    st = hb.verify_f64(a.view(NITER, N, N)[it], b.view(NITER, N, N)[it], C[it], N, N, N,
+                       row_begin=4096, row_end=4096 + 32)
+    assert st["e_cw"] == pytest.approx(oracle.errors(Cs, R, S)["e_cw"], rel=1e-9)
+    assert st["e_cw"] <= 1e-5 * np.sqrt(N)
