@@ -37,7 +37,28 @@ def parse():
                    help="target duration of the CPU golden sample")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--no-verify", action="store_true", help="skip the FP64 error check")
+    p.add_argument("--mode", choices=["gemm", "copy"], default="gemm",
+                   help="copy: a plain device-to-device copy of --copy-mib MiB, the "
+                        "reference kernel that HBM byte counters are calibrated on")
+    p.add_argument("--copy-mib", type=int, default=1024)
     return p.parse_args()
+
+
+def copy_calibration(mib, reps):
+    """Plain copy of known size: FETCH_SIZE / WRITE_SIZE of these dispatches
+    divided by mib * 1024 KiB gives the counters' scale factor."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    print(json.dumps({"mode": "copy", "bytes_read": 4 * n, "bytes_written": 4 * n,
+                      "reps": reps, "GB_s": 8 * n / dt / 1e9}), flush=True)
 
 
 def cpu_baseline(N, target_s):
@@ -80,6 +101,9 @@ def main():
     import hb_hammerbench_amd as hb
 
     torch.cuda.set_device(0)
+    if args.mode == "copy":
+        copy_calibration(args.copy_mib, max(1, args.steps))
+        return
     N, NITER = args.n, args.niter
     n_total = N * N * NITER
     a = torch.empty(n_total, dtype=torch.float32, device="cuda")

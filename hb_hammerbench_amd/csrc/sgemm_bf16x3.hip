@@ -1,0 +1,303 @@
+// sgemm_bf16x3.hip -- split-precision SGEMM on gfx950 BF16 matrix cores.
+//
+// No reference counterpart: a precision variant of apps/sgemm's C = A.B
+// (kernel.cpp:279-333 call shape) with FP32-level error.  Each float is split
+// x = hi + lo, hi = bf16_rne(x), lo = bf16_rne(x - hi) (x - hi is exact), and
+//   C = Alo.Bhi + Ahi.Blo + Ahi.Bhi
+// is accumulated in FP32 by v_mfma_f32_32x32x16_bf16; the lo.lo term
+// (relative size 2^-16) is dropped.
+//
+// Two launches per call:
+//  1. pack: A -> Ap[Mp][Kp/8][hi8|lo8] and B -> Bp[Np][Kp/8][hi8|lo8] (B is
+//     transposed so both operands are K-contiguous); rows and K are zero-padded
+//     to the tile multiples, so the GEMM needs no load bounds checks;
+//  2. GEMM: 256x256 output tile per workgroup, BK = 32, global -> registers ->
+//     LDS double buffer of the packed rows, 8 waves each owning 128x64.
+#include "hbs_internal.h"
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short ushort8 __attribute__((ext_vector_type(8)));
+
+namespace {
+
+constexpr int kBM = 256, kBN = 256, kBK = 32;
+constexpr int kWM = 2, kWN = 4;
+constexpr int kThreads = kWM * kWN * 64;
+constexpr int kWTM = kBM / kWM, kWTN = kBN / kWN;
+constexpr int kFM = kWTM / 32, kFN = kWTN / 32;
+constexpr int kRowElems = kBK * 2;        // packed bf16 per tile row (hi + lo)
+constexpr int kLdsRow = kRowElems + 8;    // + 16 B pad
+constexpr int kStage = (kBM + kBN) * kLdsRow;  // bf16 elements per stage
+constexpr int kChunksPerRow = kRowElems / 8;   // 16-byte chunks per tile row
+constexpr int kLoads = kBM * kChunksPerRow / kThreads;  // per operand per thread
+
+__device__ __forceinline__ uint16_t bf16_rne(float x) {
+    uint32_t u = __float_as_uint(x);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (uint16_t)((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ void split(float x, uint16_t &hi, uint16_t &lo) {
+    hi = bf16_rne(x);
+    lo = bf16_rne(x - __uint_as_float((uint32_t)hi << 16));
+}
+
+struct PackArgs {
+    const float *X;
+    uint16_t *P;
+    int rows, cols;      // logical matrix (rows x cols for A; K x N source for B)
+    int ld;              // leading dimension of X
+    int rows_p, cols_p;  // padded packed extents (rows_p x cols_p, cols_p % 8 == 0)
+    int64_t sX, sP;      // slice strides (elements)
+};
+
+// A [M][K] -> Ap[Mp][Kp/8][hi8 lo8]; one thread per 8-element group.
+__global__ __launch_bounds__(256) void pack_rows_kernel(PackArgs a, int64_t groups_per_slice,
+                                                        int64_t total) {
+    const int gpr = a.cols_p / 8;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t s = g / groups_per_slice;
+        const int64_t r = g - s * groups_per_slice;
+        const int row = (int)(r / gpr), kg = (int)(r % gpr);
+        const float *x = a.X + s * a.sX + (int64_t)row * a.ld + kg * 8;
+        ushort8 hi, lo;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const float v = (row < a.rows && kg * 8 + j < a.cols) ? x[j] : 0.0f;
+            uint16_t h, l;
+            split(v, h, l);
+            hi[j] = h;
+            lo[j] = l;
+        }
+        ushort8 *dst = (ushort8 *)(a.P + s * a.sP + ((int64_t)row * gpr + kg) * 16);
+        dst[0] = hi;
+        dst[1] = lo;
+    }
+}
+
+// B [K][N] -> Bp[Np][Kp/8][hi8 lo8] (transposed) through a 64x64 LDS tile.
+__global__ __launch_bounds__(256) void pack_cols_kernel(PackArgs a, int tiles_k, int tiles_n) {
+    __shared__ float t[64][65];
+    const int per_slice = tiles_k * tiles_n;
+    const int s = blockIdx.x / per_slice;
+    const int r = blockIdx.x % per_slice;
+    const int tk = r / tiles_n, tn = r % tiles_n;
+    const int k0 = tk * 64, n0 = tn * 64;
+    const float *X = a.X + (int64_t)s * a.sX;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < 64 * 64; e += 256) {
+        const int kk = e / 64, nn = e % 64;
+        const int gk = k0 + kk, gn = n0 + nn;
+        t[kk][nn] = (gk < a.rows && gn < a.cols) ? X[(int64_t)gk * a.ld + gn] : 0.0f;
+    }
+    __syncthreads();
+    const int gpr = a.rows_p / 8;  // k groups per packed row (rows_p = Kp here)
+    uint16_t *P = a.P + (int64_t)s * a.sP;
+    for (int e = tid; e < 64 * 8; e += 256) {
+        const int nn = e / 8, kg = e % 8;
+        const int gn = n0 + nn, gkg = k0 / 8 + kg;
+        if (gn >= a.cols_p || gkg >= gpr) continue;
+        ushort8 hi, lo;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            uint16_t h, l;
+            split(t[kg * 8 + j][nn], h, l);
+            hi[j] = h;
+            lo[j] = l;
+        }
+        ushort8 *dst = (ushort8 *)(P + ((int64_t)gn * gpr + gkg) * 16);
+        dst[0] = hi;
+        dst[1] = lo;
+    }
+}
+
+struct GemmArgs {
+    const uint16_t *Ap;
+    const uint16_t *Bp;
+    float *C;
+    int M, N, Kp, ldc;
+    int64_t sAp, sBp, sC;
+    int tiles_m, tiles_n, group_m;
+};
+
+template <bool EDGE>
+__global__ __launch_bounds__(kThreads) void gemm_bf16x3_kernel(GemmArgs p) {
+    __shared__ __attribute__((aligned(16))) uint16_t smem[2 * kStage];
+
+    const int tiles = p.tiles_m * p.tiles_n;
+    const int bid = hbs::xcd_remap(blockIdx.x, gridDim.x);
+    const int bs = bid / tiles;
+    int tm, tn;
+    hbs::tile_coords(bid - bs * tiles, p.tiles_m, p.tiles_n, p.group_m, tm, tn);
+    const int m0 = tm * kBM, n0 = tn * kBN;
+    const int64_t row_elems = (int64_t)p.Kp * 2;
+    const uint16_t *__restrict__ Ag = p.Ap + bs * p.sAp + (int64_t)m0 * row_elems;
+    const uint16_t *__restrict__ Bg = p.Bp + bs * p.sBp + (int64_t)n0 * row_elems;
+    float *__restrict__ Cg = p.C + bs * p.sC;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm = wave % kWM, wn = wave / kWM;
+    const int l32 = lane & 31, h = lane >> 5;
+
+    floatx16 acc[kFM][kFN];
+#pragma unroll
+    for (int r = 0; r < kFM; ++r)
+#pragma unroll
+        for (int c = 0; c < kFN; ++c) acc[r][c] = floatx16{0};
+
+    floatx4 ra[kLoads], rb[kLoads];
+
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < kLoads; ++i) {
+            const int f = tid + i * kThreads;
+            const int row = f / kChunksPerRow, c = f % kChunksPerRow;
+            const int64_t off = (int64_t)row * row_elems + (int64_t)k0 * 2 + c * 8;
+            ra[i] = *(const floatx4 *)(Ag + off);
+            rb[i] = *(const floatx4 *)(Bg + off);
+        }
+    };
+    auto sstore = [&](int buf) {
+        uint16_t *as = smem + buf * kStage;
+        uint16_t *bsm = as + kBM * kLdsRow;
+#pragma unroll
+        for (int i = 0; i < kLoads; ++i) {
+            const int f = tid + i * kThreads;
+            const int row = f / kChunksPerRow, c = f % kChunksPerRow;
+            *(floatx4 *)(as + row * kLdsRow + c * 8) = ra[i];
+            *(floatx4 *)(bsm + row * kLdsRow + c * 8) = rb[i];
+        }
+    };
+    auto compute = [&](int buf) {
+        const uint16_t *as = smem + buf * kStage + (wm * kWTM + l32) * kLdsRow;
+        const uint16_t *bsm = smem + buf * kStage + kBM * kLdsRow + (wn * kWTN + l32) * kLdsRow;
+#pragma unroll
+        for (int st = 0; st < kBK / 16; ++st) {
+            const int off = (2 * st + h) * 16;  // this lane half's 8-k group
+            bf16x8 ahi[kFM], alo[kFM], bhi[kFN], blo[kFN];
+#pragma unroll
+            for (int r = 0; r < kFM; ++r) {
+                ahi[r] = *(const bf16x8 *)(as + r * 32 * kLdsRow + off);
+                alo[r] = *(const bf16x8 *)(as + r * 32 * kLdsRow + off + 8);
+            }
+#pragma unroll
+            for (int c = 0; c < kFN; ++c) {
+                bhi[c] = *(const bf16x8 *)(bsm + c * 32 * kLdsRow + off);
+                blo[c] = *(const bf16x8 *)(bsm + c * 32 * kLdsRow + off + 8);
+            }
+#pragma unroll
+            for (int r = 0; r < kFM; ++r)
+#pragma unroll
+                for (int c = 0; c < kFN; ++c) {
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[r], bhi[c], acc[r][c], 0, 0, 0);
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[r], blo[c], acc[r][c], 0, 0, 0);
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[r], bhi[c], acc[r][c], 0, 0, 0);
+                }
+        }
+    };
+
+    const int KT = p.Kp / kBK;
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    for (int kt = 0; kt < KT; ++kt) {
+        const int cur = kt & 1;
+        if (kt + 1 < KT) gload((kt + 1) * kBK);
+        compute(cur);
+        if (kt + 1 < KT) sstore(cur ^ 1);
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int r = 0; r < kFM; ++r) {
+#pragma unroll
+        for (int c = 0; c < kFN; ++c) {
+            const int col = n0 + wn * kWTN + c * 32 + l32;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int row = m0 + wm * kWTM + r * 32 + 8 * (q >> 2) + 4 * h + (q & 3);
+                if (!EDGE || (row < p.M && col < p.N)) Cg[(int64_t)row * p.ldc + col] = acc[r][c][q];
+            }
+        }
+    }
+}
+
+inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+struct Layout {
+    int Mp, Np, Kp;
+    int64_t sAp, sBp;  // elements per slice
+    size_t bytes_a, bytes_total;
+};
+
+Layout layout(int M, int N, int K, int batch) {
+    Layout L;
+    L.Mp = (int)round_up(M, kBM);
+    L.Np = (int)round_up(N, kBN);
+    L.Kp = (int)round_up(K, kBK);
+    L.sAp = (int64_t)L.Mp * L.Kp * 2;
+    L.sBp = (int64_t)L.Np * L.Kp * 2;
+    L.bytes_a = (size_t)round_up(L.sAp * batch * 2, 256);
+    L.bytes_total = L.bytes_a + (size_t)L.sBp * batch * 2;
+    return L;
+}
+
+int run_bf16x3(const float *A, const float *B, float *C, int M, int N, int K, int lda, int ldb,
+               int ldc, int64_t sA, int64_t sB, int64_t sC, int batch, void *ws, size_t ws_bytes,
+               hipStream_t stream) {
+    const Layout L = layout(M, N, K, batch);
+    if (!ws || ws_bytes < L.bytes_total) return HBS_ERR_WORKSPACE;
+    if (!hbs::aligned16(ws)) return HBS_ERR_ALIGN;
+    uint16_t *Ap = (uint16_t *)ws;
+    uint16_t *Bp = (uint16_t *)((char *)ws + L.bytes_a);
+
+    PackArgs pa{A, Ap, M, K, lda, L.Mp, L.Kp, sA, L.sAp};
+    const int64_t gps = (int64_t)L.Mp * (L.Kp / 8);
+    const int64_t total = gps * batch;
+    int64_t blocks = (total + 255) / 256;
+    if (blocks > 262144) blocks = 262144;
+    pack_rows_kernel<<<(unsigned)blocks, 256, 0, stream>>>(pa, gps, total);
+    int st = hbs::launch_status();
+    if (st) return st;
+
+    PackArgs pb{B, Bp, K, N, ldb, L.Kp, L.Np, sB, L.sBp};
+    const int tk = L.Kp / 64 + (L.Kp % 64 ? 1 : 0), tnb = L.Np / 64;
+    const int64_t pblocks = (int64_t)tk * tnb * batch;
+    if (pblocks > 0x7fffffffLL) return HBS_ERR_ARG;
+    pack_cols_kernel<<<(unsigned)pblocks, 256, 0, stream>>>(pb, tk, tnb);
+    st = hbs::launch_status();
+    if (st) return st;
+
+    GemmArgs g{Ap, Bp, C, M, N, L.Kp, ldc, L.sAp, L.sBp, sC, L.Mp / kBM, L.Np / kBN, 8};
+    const int64_t gblocks = (int64_t)g.tiles_m * g.tiles_n * batch;
+    if (gblocks > 0x7fffffffLL) return HBS_ERR_ARG;
+    if (M % kBM == 0 && N % kBN == 0)
+        gemm_bf16x3_kernel<false><<<(unsigned)gblocks, kThreads, 0, stream>>>(g);
+    else
+        gemm_bf16x3_kernel<true><<<(unsigned)gblocks, kThreads, 0, stream>>>(g);
+    return hbs::launch_status();
+}
+
+}  // namespace
+
+extern "C" size_t hbs_sgemm_bf16x3_workspace_size(int N, int niter) {
+    if (N < 1 || niter < 1) return 0;
+    return layout(N, N, N, niter).bytes_total;
+}
+
+extern "C" int hbs_sgemm_bf16x3(const float *A, const float *B, float *C, int N, int niter,
+                                void *workspace, size_t workspace_bytes, void *stream) {
+    if (N < 1 || niter < 1) return HBS_ERR_ARG;
+    const int64_t nn = (int64_t)N * N;
+    if (nn > ((int64_t)1 << 62) / niter) return HBS_ERR_ARG;
+    if (!A || !B || !C) return HBS_ERR_NULL;
+    if (!workspace || workspace_bytes < layout(N, N, N, niter).bytes_total) return HBS_ERR_WORKSPACE;
+    if (!hbs::aligned16(workspace)) return HBS_ERR_ALIGN;
+    const int st = hbs::check_device();
+    if (st) return st;
+    return run_bf16x3(A, B, C, N, N, N, N, N, N, nn, nn, nn, niter, workspace, workspace_bytes,
+                      (hipStream_t)stream);
+}

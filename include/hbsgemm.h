@@ -70,6 +70,23 @@ int hbs_sgemm_strided(const float *A, const float *B, float *C, int M, int N, in
                       int64_t strideC, int batch, void *stream);
 
 /* ------------------------------------------------------------------------ */
+/* BF16x3 split-precision SGEMM (no reference counterpart): x = hi + lo with  */
+/* hi = bf16(x), lo = bf16(x - hi); C = Ahi.Bhi + Ahi.Blo + Alo.Bhi on BF16   */
+/* matrix cores with FP32 accumulation.  Same call shape as hbs_sgemm.        */
+/* ------------------------------------------------------------------------ */
+
+/* Bytes of workspace hbs_sgemm_bf16x3 needs for (N, niter): the packed hi/lo
+ * planes of every slice (about 4 * N * N * 2 bytes per slice, rows and K
+ * padded to the tile multiples).  0 for invalid sizes. */
+size_t hbs_sgemm_bf16x3_workspace_size(int N, int niter);
+/* C_i = A_i . B_i for i < niter N x N slices, as hbs_sgemm, on BF16 matrix
+ * cores.  `workspace` (16-byte aligned, >= hbs_sgemm_bf16x3_workspace_size)
+ * is overwritten; HBS_ERR_WORKSPACE if it is missing or too small. */
+int hbs_sgemm_bf16x3(const float *A, const float *B, float *C, int N, int niter,
+                     void *workspace, size_t workspace_bytes, void *stream);
+
+
+/* ------------------------------------------------------------------------ */
 /* Input generation on the device.                                            */
 /* ------------------------------------------------------------------------ */
 

@@ -89,3 +89,17 @@ def test_python_wrapper_raises_hbs_error():
     with pytest.raises(hb.HbsError) as ei:
         hb.sgemm(P.value, P.value, P.value, 0, 1, stream=0)
     assert ei.value.code == -1
+
+
+def test_bf16x3_workspace_contract():
+    L = hb.lib()
+    assert L.hbs_sgemm_bf16x3_workspace_size(0, 1) == 0
+    need = L.hbs_sgemm_bf16x3_workspace_size(8192, 10)
+    assert need >= 10 * 4 * 8192 * 8192 * 2  # hi + lo of A and B per slice
+    assert L.hbs_sgemm_bf16x3_workspace_size(100, 1) > 0
+    assert L.hbs_sgemm_bf16x3(P, P, P, 64, 1, None, 1 << 30, None) == -4
+    small = L.hbs_sgemm_bf16x3_workspace_size(64, 1) - 1
+    assert L.hbs_sgemm_bf16x3(P, P, P, 64, 1, P, small, None) == -4
+    assert L.hbs_sgemm_bf16x3(P, P, P, 0, 1, P, 1 << 30, None) == -1
+    assert L.hbs_sgemm_bf16x3(None, P, P, 64, 1, P, 1 << 30, None) == -2
+    assert L.hbs_sgemm_bf16x3(P, P, P, 64, 1, ctypes.c_void_p(24), 1 << 30, None) == -3

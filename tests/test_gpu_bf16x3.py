@@ -1,0 +1,103 @@
+"""GPU parity of the BF16x3 split-precision SGEMM, through the C-ABI.
+
+No reference function exists for this kernel; it is pinned by
+ * bit-exactness on the reference integer pattern (apps/sgemm/main.cpp:46-52):
+   values 0..6 are exact in bf16, so lo = 0 and every product is exact;
+ * a rigorous per-element bound against the FP64 oracle: the split drops at
+   most 3.1 * 2^-16 |a||b| per product (lo.lo and the rounding of lo), and the
+   3K products are summed in FP32: |C - R| <= (3.1 * 2^-16 + gamma_{3K+2}) S;
+ * the FP32-level accuracy claim at N = 8192 on log-uniform inputs: max
+   componentwise error no worse than 2x the plain FP32 kernel's on the same
+   inputs.
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+torch = pytest.importorskip("torch")
+import hb_hammerbench_amd as hb  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+U = 2.0 ** -24
+
+
+def gamma(n):
+    return n * U / (1 - n * U)
+
+
+def dev_empty(n):
+    return torch.empty(int(n), dtype=torch.float32, device="cuda")
+
+
+def dev_fill(n, kind, seed):
+    x = dev_empty(n)
+    hb.fill_random(x, n, kind, seed)
+    return x
+
+
+def test_bf16x3_reference_pattern_N512_NITER2_bit_exact():
+    N, NITER = 512, 2
+    a = dev_empty(NITER * N * N); b = dev_empty(NITER * N * N); c = dev_empty(NITER * N * N)
+    hb.fill_pattern(a, b, NITER * N * N)
+    hb.sgemm_bf16x3(a, b, c, N, NITER)
+    torch.cuda.synchronize()
+    got = c.cpu().numpy().reshape(NITER, N, N)
+    for it in range(NITER):
+        sse, nd = oracle.sse(oracle.pattern_slice(N, it), got[it])
+        assert sse == 0.0 and nd == 0
+
+
+@pytest.mark.parametrize("N,niter,kind", [(512, 2, hb.GEN_UNIFORM), (256, 1, hb.GEN_LOGUNIFORM),
+                                          (1, 1, hb.GEN_UNIFORM), (77, 2, hb.GEN_UNIFORM),
+                                          (300, 1, hb.GEN_LOGUNIFORM), (520, 1, hb.GEN_UNIFORM)])
+def test_bf16x3_within_split_bound(N, niter, kind):
+    a = dev_fill(niter * N * N, kind, 21); b = dev_fill(niter * N * N, kind, 22)
+    c = dev_empty(niter * N * N)
+    hb.sgemm_bf16x3(a, b, c, N, niter)
+    torch.cuda.synchronize()
+    A = a.cpu().numpy().reshape(niter, N, N); B = b.cpu().numpy().reshape(niter, N, N)
+    C = c.cpu().numpy().reshape(niter, N, N)
+    for it in range(niter):
+        R, S = oracle.gemm_f64(A[it], B[it])
+        bound = (3.1 * 2.0 ** -16 + gamma(3 * N + 2)) * S
+        assert np.all(np.abs(C[it] - R) <= bound), (N, it)
+        e = oracle.errors(C[it], R, S)
+        assert e["e_cw"] <= 1e-5 * np.sqrt(N)
+
+
+def test_bf16x3_pattern_ragged_sizes():
+    for N in (3, 100, 257):
+        a = dev_empty(N * N); b = dev_empty(N * N); c = dev_empty(N * N)
+        hb.fill_pattern(a, b, N * N)
+        hb.sgemm_bf16x3(a, b, c, N, 1)
+        torch.cuda.synchronize()
+        assert np.array_equal(c.cpu().numpy().reshape(N, N), oracle.pattern_slice(N, 0)), N
+
+
+def test_bf16x3_reuses_workspace_across_sizes():
+    big = hb.bf16x3_workspace_size(512, 1)
+    ws = torch.empty(big, dtype=torch.uint8, device="cuda")
+    for N in (512, 128):
+        a = dev_fill(N * N, hb.GEN_UNIFORM, 1); b = dev_fill(N * N, hb.GEN_UNIFORM, 2)
+        c1 = dev_empty(N * N); c2 = dev_empty(N * N)
+        hb.sgemm_bf16x3(a, b, c1, N, 1, workspace=ws)
+        hb.sgemm_bf16x3(a, b, c2, N, 1)
+        torch.cuda.synchronize()
+        assert torch.equal(c1, c2)  # deterministic, independent of workspace history
+
+
+def test_bf16x3_fp32_level_error_at_N8192_loguniform():
+    """BF16x3 max componentwise error <= 2x the FP32 kernel's, same inputs."""
+    N = 8192
+    a = dev_fill(N * N, hb.GEN_LOGUNIFORM, 31); b = dev_fill(N * N, hb.GEN_LOGUNIFORM, 32)
+    c32 = dev_empty(N * N); c3 = dev_empty(N * N)
+    hb.sgemm(a, b, c32, N, 1)
+    hb.sgemm_bf16x3(a, b, c3, N, 1)
+    torch.cuda.synchronize()
+    e32 = hb.verify_f64(a, b, c32, N, N, N)
+    e3 = hb.verify_f64(a, b, c3, N, N, N)
+    print(f"\nN=8192 log-uniform: e_cw fp32={e32['e_cw']:.3e} bf16x3={e3['e_cw']:.3e} "
+          f"e_F fp32={e32['e_F']:.3e} bf16x3={e3['e_F']:.3e}", flush=True)
+    assert e32["nonfinite"] == 0 and e3["nonfinite"] == 0
+    assert e3["e_cw"] <= 2.0 * e32["e_cw"]
