@@ -1,0 +1,234 @@
+// main.cpp -- host launcher for the sgemm path on one MI355X.
+//
+// Keeps the flow of the reference host program apps/sgemm/main.cpp:30-149:
+//   print N / NITER (:38-39); allocate NITER*N*N floats for mat1, mat2 and the
+//   golden (:46-48); initialise mat1[i] = i % 7, mat2[i] = i % 3 (:49-52);
+//   compute the golden with host_mm per slice (:54-56, :15-26); allocate device
+//   buffers and copy the inputs (:74-83); launch the kernel over all NITER
+//   slices (:86-98); copy the result back (:105-117); validate with the float
+//   SSE rule, printing mismatching elements and failing when SSE >= 0.01
+//   (:120-134); return success / failure (:144-148).
+// The HammerBlade runtime calls become HIP calls and one libhbsgemm entry point.
+//
+// Additions, all optional: --precision bf16x3 selects the split kernel;
+// --input uniform|loguniform draws seeded inputs on the device (the pattern is
+// the default); --golden f64 replaces the O(N^3) host golden by the device FP64
+// check (needed at N = 8192, where host_mm takes tens of minutes per slice);
+// --reps times repeated launches with HIP events.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+
+#include <string>
+#include <vector>
+
+#include "hbsgemm.h"
+
+#define HIP_CALL(expr)                                                              \
+    do {                                                                            \
+        hipError_t _e = (expr);                                                     \
+        if (_e != hipSuccess) {                                                     \
+            fprintf(stderr, "%s:%d: %s: %s\n", __FILE__, __LINE__, #expr,           \
+                    hipGetErrorString(_e));                                         \
+            return 2;                                                               \
+        }                                                                           \
+    } while (0)
+
+#define HBS_CALL(expr)                                                              \
+    do {                                                                            \
+        int _rc = (expr);                                                           \
+        if (_rc != HBS_SUCCESS) {                                                   \
+            fprintf(stderr, "%s:%d: %s: %s\n", __FILE__, __LINE__, #expr,           \
+                    hbs_strerror(_rc));                                             \
+            return 2;                                                               \
+        }                                                                           \
+    } while (0)
+
+// apps/sgemm/main.cpp:15-26 -- golden: y -> x -> z, float accumulator.
+static void host_mm(float *result, const float *mat1, const float *mat2, int N) {
+    for (int y = 0; y < N; y++) {
+        for (int x = 0; x < N; x++) {
+            float sum = 0.0f;
+            for (int z = 0; z < N; z++) sum += mat1[(int64_t)N * y + z] * mat2[(int64_t)N * z + x];
+            result[(int64_t)N * y + x] = sum;
+        }
+    }
+}
+
+struct Options {
+    int N = 512;
+    int NITER = 2;
+    std::string precision = "fp32";
+    std::string input = "pattern";
+    std::string golden = "auto";
+    uint64_t seed = 1;
+    int reps = 0;
+    int max_print = 16;
+};
+
+static void usage(const char *prog) {
+    fprintf(stderr,
+            "usage: %s [--N n] [--NITER k] [--precision fp32|bf16x3]\n"
+            "          [--input pattern|uniform|loguniform] [--seed s]\n"
+            "          [--golden host|f64|auto] [--reps r] [--max-print m]\n",
+            prog);
+}
+
+static bool parse(int argc, char **argv, Options &o) {
+    for (int i = 1; i < argc; ++i) {
+        std::string a = argv[i];
+        auto next = [&](void) -> const char * { return i + 1 < argc ? argv[++i] : nullptr; };
+        const char *v = nullptr;
+        if (a == "--N" && (v = next())) o.N = atoi(v);
+        else if (a == "--NITER" && (v = next())) o.NITER = atoi(v);
+        else if (a == "--precision" && (v = next())) o.precision = v;
+        else if (a == "--input" && (v = next())) o.input = v;
+        else if (a == "--golden" && (v = next())) o.golden = v;
+        else if (a == "--seed" && (v = next())) o.seed = strtoull(v, nullptr, 10);
+        else if (a == "--reps" && (v = next())) o.reps = atoi(v);
+        else if (a == "--max-print" && (v = next())) o.max_print = atoi(v);
+        else return false;
+    }
+    if (o.N < 1 || o.NITER < 1 || o.reps < 0) return false;
+    if (o.precision != "fp32" && o.precision != "bf16x3") return false;
+    if (o.input != "pattern" && o.input != "uniform" && o.input != "loguniform") return false;
+    if (o.golden == "auto") o.golden = (o.N <= 1024) ? "host" : "f64";
+    return o.golden == "host" || o.golden == "f64";
+}
+
+int main(int argc, char **argv) {
+    Options o;
+    if (!parse(argc, argv, o)) {
+        usage(argv[0]);
+        return 2;
+    }
+    const int N = o.N, NITER = o.NITER;
+    printf("N=%d\n", N);
+    printf("NITER=%d\n", NITER);
+    printf("precision=%s input=%s golden=%s\n", o.precision.c_str(), o.input.c_str(),
+           o.golden.c_str());
+    const int64_t nn = (int64_t)N * N, total = nn * NITER;
+    const size_t bytes = (size_t)total * sizeof(float);
+
+    int dev = 0;
+    HIP_CALL(hipSetDevice(dev));
+    float *d_mat1 = nullptr, *d_mat2 = nullptr, *d_result = nullptr;
+    HIP_CALL(hipMalloc(&d_mat1, bytes));
+    HIP_CALL(hipMalloc(&d_mat2, bytes));
+    HIP_CALL(hipMalloc(&d_result, bytes));
+    void *d_ws = nullptr;
+    size_t ws_bytes = 0;
+    if (o.precision == "bf16x3") {
+        ws_bytes = hbs_sgemm_bf16x3_workspace_size(N, NITER);
+        HIP_CALL(hipMalloc(&d_ws, ws_bytes));
+    }
+
+    // Inputs (main.cpp:46-52).  The pattern is built on the host exactly as the
+    // reference does; seeded inputs are drawn on the device and copied back.
+    const bool need_host = (o.golden == "host");
+    std::vector<float> mat1, mat2;
+    if (o.input == "pattern") {
+        mat1.resize(total);
+        mat2.resize(total);
+        for (int64_t i = 0; i < total; i++) {
+            mat1[i] = (float)(i % 7);
+            mat2[i] = (float)(i % 3);
+        }
+        HIP_CALL(hipMemcpy(d_mat1, mat1.data(), bytes, hipMemcpyHostToDevice));
+        HIP_CALL(hipMemcpy(d_mat2, mat2.data(), bytes, hipMemcpyHostToDevice));
+    } else {
+        const int kind = (o.input == "uniform") ? HBS_GEN_UNIFORM : HBS_GEN_LOGUNIFORM;
+        HBS_CALL(hbs_fill_random(d_mat1, total, kind, o.seed, 0, nullptr));
+        HBS_CALL(hbs_fill_random(d_mat2, total, kind, o.seed + 1, 0, nullptr));
+        HIP_CALL(hipDeviceSynchronize());
+        if (need_host) {
+            mat1.resize(total);
+            mat2.resize(total);
+            HIP_CALL(hipMemcpy(mat1.data(), d_mat1, bytes, hipMemcpyDeviceToHost));
+            HIP_CALL(hipMemcpy(mat2.data(), d_mat2, bytes, hipMemcpyDeviceToHost));
+        }
+    }
+
+    // Golden (main.cpp:54-56).
+    std::vector<float> mat_result;
+    if (need_host) {
+        mat_result.resize(total);
+        for (int i = 0; i < NITER; i++)
+            host_mm(&mat_result[nn * i], &mat1[nn * i], &mat2[nn * i], N);
+    }
+
+    // Launch all NITER slices (main.cpp:86-98).
+    auto launch = [&]() -> int {
+        if (o.precision == "fp32") return hbs_sgemm(d_mat1, d_mat2, d_result, N, NITER, nullptr);
+        return hbs_sgemm_bf16x3(d_mat1, d_mat2, d_result, N, NITER, d_ws, ws_bytes, nullptr);
+    };
+    printf("Launching sgemm (%s) over %d slices\n", o.precision.c_str(), NITER);
+    HBS_CALL(launch());
+    HIP_CALL(hipDeviceSynchronize());
+
+    if (o.reps > 0) {
+        hipEvent_t e0, e1;
+        HIP_CALL(hipEventCreate(&e0));
+        HIP_CALL(hipEventCreate(&e1));
+        HIP_CALL(hipEventRecord(e0, nullptr));
+        for (int r = 0; r < o.reps; ++r) HBS_CALL(launch());
+        HIP_CALL(hipEventRecord(e1, nullptr));
+        HIP_CALL(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        HIP_CALL(hipEventElapsedTime(&ms, e0, e1));
+        const double per = ms / o.reps;
+        const double tflops = 2.0 * (double)N * N * N * NITER / (per * 1e-3) / 1e12;
+        printf("time per launch %.3f ms, %.2f TFLOP/s\n", per, tflops);
+        HIP_CALL(hipEventDestroy(e0));
+        HIP_CALL(hipEventDestroy(e1));
+    }
+
+    // Readback and validation (main.cpp:102-135).
+    bool fail = false;
+    if (need_host) {
+        std::vector<float> actual_result(total, 0.0f);
+        HIP_CALL(hipMemcpy(actual_result.data(), d_result, bytes, hipMemcpyDeviceToHost));
+        float sse = 0.0f;
+        int64_t printed = 0, ndiff = 0;
+        for (int64_t i = 0; i < total; i++) {
+            const float actual = actual_result[i];
+            const float expected = mat_result[i];
+            const float diff = expected - actual;
+            sse += diff * diff;
+            if (diff != 0.0f) {
+                ndiff++;
+                if (printed++ < o.max_print) printf("[%lld] actual=%f, expected=%f\n",
+                                                    (long long)i, actual, expected);
+            }
+        }
+        printf("elements differing: %lld, SSE= %f\n", (long long)ndiff, sse);
+        if (sse >= .01f) {
+            printf("Matrix Mismatch. SSE= %f\n", sse);
+            fail = true;
+        }
+    } else {
+        // FP64 check of every slice on the device: componentwise error gate
+        // e_cw <= 1e-5 * sqrt(N), plus bit-exactness for the pattern.
+        const double gate = 1e-5 * sqrt((double)N);
+        for (int i = 0; i < NITER; i++) {
+            hbs_errstats st;
+            HBS_CALL(hbs_verify_f64(d_mat1 + nn * i, d_mat2 + nn * i, d_result + nn * i, N, N, N,
+                                    N, N, N, 0, N, &st, nullptr));
+            const bool bad = st.nonfinite != 0 || st.e_cw > gate ||
+                             (o.input == "pattern" && st.max_abs_err != 0.0);
+            printf("slice %d: e_cw=%.3e e_nw=%.3e e_F=%.3e max|err|=%.3e%s\n", i, st.e_cw, st.e_nw,
+                   st.e_F, st.max_abs_err, bad ? "  MISMATCH" : "");
+            fail = fail || bad;
+        }
+    }
+
+    HIP_CALL(hipFree(d_mat1));
+    HIP_CALL(hipFree(d_mat2));
+    HIP_CALL(hipFree(d_result));
+    if (d_ws) HIP_CALL(hipFree(d_ws));
+    printf("HBS REGRESSION TEST %s\n", fail ? "FAILED" : "PASSED");
+    return fail ? 1 : 0;
+}
