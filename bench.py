@@ -29,7 +29,10 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--variant", choices=["fp32", "bf16x3"], default="fp32")
+    p.add_argument("--variant", choices=["fp32", "bf16x3", "grouped-fp32", "grouped-bf16x3"],
+                   default="fp32")
+    p.add_argument("--problems", type=int, default=512, help="grouped: number of problems")
+    p.add_argument("--tuning", default=None, help="grouped: autotuner tuning file")
     p.add_argument("--n", type=int, default=8192)
     p.add_argument("--niter", type=int, default=10)
     p.add_argument("--seed", type=int, default=20241)
@@ -90,6 +93,93 @@ def cpu_baseline(N, target_s):
     }
 
 
+def setup_square(args, hb, torch):
+    """NITER row-major N x N slices in one launch (apps/sgemm/kernel.cpp:293-297)."""
+    N, NITER = args.n, args.niter
+    n_total = N * N * NITER
+    a = torch.empty(n_total, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    c = torch.empty_like(a)
+    hb.fill_random(a, n_total, hb.GEN_UNIFORM, args.seed)
+    hb.fill_random(b, n_total, hb.GEN_UNIFORM, args.seed + 1)
+    ws = None
+    if args.variant == "bf16x3":
+        ws = torch.empty(max(1, hb.bf16x3_workspace_size(N, NITER)), dtype=torch.uint8,
+                         device="cuda")
+    torch.cuda.synchronize()
+
+    def step():
+        if args.variant == "fp32":
+            hb.sgemm(a, b, c, N, NITER)
+        else:
+            hb.sgemm_bf16x3(a, b, c, N, NITER, workspace=ws)
+
+    def verify():
+        it = NITER - 1
+        sl = slice(it * N * N, (it + 1) * N * N)
+        st = hb.verify_f64(a[sl], b[sl], c[sl], N, N, N)
+        return {"slice": it, "e_cw": st["e_cw"], "e_nw": st["e_nw"], "e_F": st["e_F"],
+                "gate_e_cw": 1e-5 * N ** 0.5, "oracle": "FP64 on device (hbs_verify_f64)"}
+
+    kind = "plain FP32 MFMA" if args.variant == "fp32" else "BF16x3 split"
+    workload = (f"{kind} SGEMM, square N={N}, NITER={NITER} slices per launch, "
+                f"uniform[-1,1) seed {args.seed}")
+    return step, 2.0 * N * N * N * NITER, workload, {"verify": verify,
+                                                     "config": {"N": N, "niter": NITER}}
+
+
+def setup_grouped(args, hb, torch):
+    """`--problems` independent GEMMs, M, N, K each uniform over {64, 128, ..., 2048}
+    (numpy default_rng(seed)), uniform[-1,1) inputs, one persistent launch replayed
+    from a hipGraph."""
+    import numpy as np
+    rng = np.random.default_rng(args.seed)
+    shapes = [tuple(int(x) for x in rng.integers(1, 33, 3) * 64) for _ in range(args.problems)]
+    ta = sum(M * K for M, N, K in shapes)
+    tb = sum(K * N for M, N, K in shapes)
+    tc = sum(M * N for M, N, K in shapes)
+    a = torch.empty(ta, dtype=torch.float32, device="cuda")
+    b = torch.empty(tb, dtype=torch.float32, device="cuda")
+    c = torch.empty(tc, dtype=torch.float32, device="cuda")
+    hb.fill_random(a, ta, hb.GEN_UNIFORM, args.seed)
+    hb.fill_random(b, tb, hb.GEN_UNIFORM, args.seed + 1)
+    probs, views = [], []
+    oa = ob = oc = 0
+    for M, N, K in shapes:
+        A = a[oa:oa + M * K]; B = b[ob:ob + K * N]; C = c[oc:oc + M * N]
+        probs.append((A, B, C, M, N, K))
+        views.append((A, B, C, M, N, K))
+        oa += M * K; ob += K * N; oc += M * N
+    prec = hb.PREC_FP32 if args.variant == "grouped-fp32" else hb.PREC_BF16X3
+    plan = hb.GroupedPlan(probs, prec, tuning_file=args.tuning)
+    graph = plan.graph()
+    torch.cuda.synchronize()
+    flops = float(sum(2 * M * N * K for M, N, K in shapes))
+
+    def step():
+        graph.launch()
+
+    def verify():
+        worst = {"e_cw": 0.0, "e_F": 0.0}
+        big = sorted(range(len(views)), key=lambda i: -shapes[i][2])[:8]
+        for i in big:
+            A, B, C, M, N, K = views[i]
+            st = hb.verify_f64(A, B, C, M, N, K)
+            worst["e_cw"] = max(worst["e_cw"], st["e_cw"])
+            worst["e_F"] = max(worst["e_F"], st["e_F"])
+        worst["checked"] = "8 problems with the largest K, FP64 on device"
+        return worst
+
+    info = plan.info()
+    workload = (f"grouped {'FP32 MFMA' if prec == hb.PREC_FP32 else 'BF16x3'} GEMM, "
+                f"{args.problems} problems, M,N,K uniform over {{64..2048}} step 64 "
+                f"(numpy default_rng({args.seed})), one persistent hipGraph launch")
+    cfg = {"problems": args.problems, "plan": info,
+           "bytes_algorithmic": float(sum(4 * (M * K + K * N + M * N) for M, N, K in shapes)),
+           "tuning": args.tuning}
+    return step, flops, workload, {"verify": verify, "config": cfg, "keep": (plan, graph)}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -104,25 +194,11 @@ def main():
     if args.mode == "copy":
         copy_calibration(args.copy_mib, max(1, args.steps))
         return
-    N, NITER = args.n, args.niter
-    n_total = N * N * NITER
-    a = torch.empty(n_total, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    c = torch.empty_like(a)
-    kind = hb.GEN_UNIFORM
-    hb.fill_random(a, n_total, kind, args.seed)
-    hb.fill_random(b, n_total, kind, args.seed + 1)
-    ws = None
-    if args.variant == "bf16x3":
-        ws = torch.empty(max(1, hb.bf16x3_workspace_size(N, NITER)), dtype=torch.uint8,
-                         device="cuda")
-    torch.cuda.synchronize()
-
-    def step():
-        if args.variant == "fp32":
-            hb.sgemm(a, b, c, N, NITER)
-        else:
-            hb.sgemm_bf16x3(a, b, c, N, NITER, workspace=ws)
+    if args.variant.startswith("grouped"):
+        setup = setup_grouped(args, hb, torch)
+    else:
+        setup = setup_square(args, hb, torch)
+    step, flops, workload, extra = setup
 
     for _ in range(args.warmup):
         step()
@@ -142,10 +218,9 @@ def main():
     per_launch_ms = [s.elapsed_time(e) for s, e in ev]
     avg_ms = sum(per_launch_ms) / len(per_launch_ms)
 
-    flops = 2.0 * N * N * N * NITER
     value = flops * args.steps / wall / 1e12
     achieved = flops / (avg_ms / 1e3) / 1e12
-    peak = PEAK_TFLOPS["fp32"] if args.variant == "fp32" else PEAK_TFLOPS["bf16"] / 3.0
+    peak = PEAK_TFLOPS["fp32"] if args.variant.endswith("fp32") else PEAK_TFLOPS["bf16"] / 3.0
     line = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -157,15 +232,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if args.variant == "fp32" else "bf16x3->f32",
+        "dtype": "f32" if args.variant.endswith("fp32") else "bf16x3->f32",
         "data": "synthetic",
-        "config": {
-            "workload": f"{'plain FP32 MFMA' if args.variant == 'fp32' else 'BF16x3 split'} "
-                        f"SGEMM, square N={N}, NITER={NITER} slices per launch, "
-                        f"uniform[-1,1) seed {args.seed}",
-            "N": N, "niter": NITER, "variant": args.variant,
-            "flops_per_step": flops,
-        },
+        "config": dict(workload=workload, variant=args.variant, flops_per_step=flops,
+                       **extra.get("config", {})),
         "roofline": {
             "bound": "mfma",
             "achieved": round(achieved, 3),
@@ -177,15 +247,9 @@ def main():
         },
     }
     if not args.no_verify:
-        it = NITER - 1
-        sl = slice(it * N * N, (it + 1) * N * N)
-        st = hb.verify_f64(a[sl], b[sl], c[sl], N, N, N)
-        line["accuracy"] = {
-            "slice": it, "e_cw": st["e_cw"], "e_nw": st["e_nw"], "e_F": st["e_F"],
-            "gate_e_cw": 1e-5 * N ** 0.5, "oracle": "FP64 on device (hbs_verify_f64)",
-        }
+        line["accuracy"] = extra["verify"]()
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(N, args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(args.n, args.cpu_seconds)
     print(json.dumps(line), flush=True)
 
 

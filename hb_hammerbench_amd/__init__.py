@@ -16,13 +16,16 @@ import os
 __all__ = [
     "HbsError", "lib", "library_path", "sgemm", "sgemm_strided", "sgemm_bf16x3",
     "bf16x3_workspace_size", "fill_pattern", "fill_random", "verify_f64",
-    "GEN_UNIFORM", "GEN_LOGUNIFORM", "GEN_NORMAL_OUTLIER",
+    "GEN_UNIFORM", "GEN_LOGUNIFORM", "GEN_NORMAL_OUTLIER", "PREC_FP32", "PREC_BF16X3",
+    "GroupedPlan", "Graph",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_NAME = "libhbsgemm.so"
 
 GEN_UNIFORM, GEN_LOGUNIFORM, GEN_NORMAL_OUTLIER = 0, 1, 2
+PREC_FP32, PREC_BF16X3 = 0, 1
+TILE_128x128, TILE_64x64, TILE_128x64, TILE_64x128 = 0, 1, 2, 3
 
 _lib = None
 
@@ -46,6 +49,18 @@ class ErrStats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class Problem(ctypes.Structure):
+    _fields_ = [("A", ctypes.c_void_p), ("B", ctypes.c_void_p), ("C", ctypes.c_void_p),
+                ("M", ctypes.c_int), ("N", ctypes.c_int), ("K", ctypes.c_int),
+                ("lda", ctypes.c_int), ("ldb", ctypes.c_int), ("ldc", ctypes.c_int)]
+
+
+class GroupedInfo(ctypes.Structure):
+    _fields_ = [("problems", ctypes.c_int), ("workgroups", ctypes.c_int),
+                ("tiles", ctypes.c_int64), ("work_items", ctypes.c_int64),
+                ("split_tiles", ctypes.c_int64), ("est_imbalance", ctypes.c_double)]
+
+
 def library_path():
     return os.path.join(_HERE, _LIB_NAME)
 
@@ -61,6 +76,16 @@ SIGNATURES = {
     "hbs_sgemm_strided": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i64, _i64, _i64, _i, _vp]),
     "hbs_sgemm_bf16x3_workspace_size": (_sz, [_i, _i]),
     "hbs_sgemm_bf16x3": (_i, [_vp, _vp, _vp, _i, _i, _vp, _sz, _vp]),
+    "hbs_grouped_plan_create": (_i, [ctypes.POINTER(Problem), _i, _i, ctypes.c_char_p,
+                                     ctypes.POINTER(_vp)]),
+    "hbs_grouped_plan_create_forced": (_i, [ctypes.POINTER(Problem), _i, _i, _i, _i,
+                                            ctypes.POINTER(_vp)]),
+    "hbs_grouped_plan_info": (_i, [_vp, ctypes.POINTER(GroupedInfo)]),
+    "hbs_grouped_run": (_i, [_vp, _vp]),
+    "hbs_grouped_plan_destroy": (_i, [_vp]),
+    "hbs_grouped_graph_create": (_i, [_vp, ctypes.POINTER(_vp)]),
+    "hbs_graph_launch": (_i, [_vp, _vp]),
+    "hbs_graph_destroy": (_i, [_vp]),
     "hbs_fill_pattern": (_i, [_vp, _vp, _i64, _i64, _vp]),
     "hbs_fill_random": (_i, [_vp, _i64, _i, _u64, _u64, _vp]),
     "hbs_verify_f64": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _i,
@@ -160,3 +185,77 @@ def verify_f64(A, B, C, M, N, K, lda=None, ldb=None, ldc=None, row_begin=0, row_
                                 _stream(stream)),
            "hbs_verify_f64")
     return out.as_dict()
+
+
+class GroupedPlan:
+    """A grouped persistent GEMM plan over (A, B, C, M, N, K[, lda, ldb, ldc]) problems.
+
+    Tensors are referenced, not copied: keep them alive while the plan is used.
+    """
+
+    def __init__(self, problems, precision=PREC_FP32, tuning_file=None, tile=None, split_k=1):
+        arr = (Problem * len(problems))()
+        self._keep = []
+        for i, pr in enumerate(problems):
+            A, B, C, M, N, K = pr[:6]
+            lda, ldb, ldc = (pr[6:9] if len(pr) >= 9 else (K, N, N))
+            self._keep += [A, B, C]
+            arr[i] = Problem(_ptr(A), _ptr(B), _ptr(C), M, N, K, lda, ldb, ldc)
+        h = ctypes.c_void_p()
+        if tile is None:
+            path = tuning_file.encode() if tuning_file else None
+            rc = lib().hbs_grouped_plan_create(arr, len(problems), precision, path,
+                                               ctypes.byref(h))
+        else:
+            rc = lib().hbs_grouped_plan_create_forced(arr, len(problems), precision, tile,
+                                                      split_k, ctypes.byref(h))
+        _check(rc, "hbs_grouped_plan_create")
+        self.handle = h
+        self.precision = precision
+
+    def info(self):
+        out = GroupedInfo()
+        _check(lib().hbs_grouped_plan_info(self.handle, ctypes.byref(out)), "hbs_grouped_plan_info")
+        return {name: getattr(out, name) for name, _ in GroupedInfo._fields_}
+
+    def run(self, stream=None):
+        _check(lib().hbs_grouped_run(self.handle, _stream(stream)), "hbs_grouped_run")
+
+    def graph(self):
+        return Graph(self)
+
+    def close(self):
+        if self.handle:
+            _check(lib().hbs_grouped_plan_destroy(self.handle), "hbs_grouped_plan_destroy")
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Graph:
+    """An executable hipGraph holding one run of a GroupedPlan."""
+
+    def __init__(self, plan):
+        h = ctypes.c_void_p()
+        _check(lib().hbs_grouped_graph_create(plan.handle, ctypes.byref(h)),
+               "hbs_grouped_graph_create")
+        self.handle = h
+        self.plan = plan
+
+    def launch(self, stream=None):
+        _check(lib().hbs_graph_launch(self.handle, _stream(stream)), "hbs_graph_launch")
+
+    def close(self):
+        if self.handle:
+            _check(lib().hbs_graph_destroy(self.handle), "hbs_graph_destroy")
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

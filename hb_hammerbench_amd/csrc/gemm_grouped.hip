@@ -1,0 +1,730 @@
+// gemm_grouped.hip -- grouped persistent GEMM: many independent FP32 problems
+// of uneven M/N/K in one launch, FP32-MFMA or BF16x3 arithmetic.
+//
+// No reference counterpart: the batching pattern built around apps/sgemm's
+// C = A.B.  The reference runs its NITER equal-size slices in one launch
+// (apps/sgemm/kernel.cpp:293-297); this generalises that to uneven problems.
+//
+// Design:
+//  * a plan (host side, built once) chooses a tile shape and split-K factor per
+//    problem -- from the autotuner's tuning file when one is given, else from a
+//    size heuristic -- expands every problem into work items (one output tile
+//    times one K range) and assigns them to a fixed number of persistent
+//    workgroups by longest-processing-time-first bin packing;
+//  * the kernel is one launch of (1 or 2 x CUs) workgroups, each walking its own
+//    item list, so the launch is capturable in a hipGraph and needs no host
+//    work or memset between replays;
+//  * split-K tiles write FP32 partials; the last of a tile's splits to finish
+//    (an agent-scope counter) adds them in split order -- deterministic -- and
+//    resets the counter to zero for the next run;
+//  * BF16x3 splits each float into bf16 hi/lo while staging A and B into LDS,
+//    so the group stays a single launch.
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <fstream>
+#include <map>
+#include <queue>
+#include <sstream>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "hbs_internal.h"
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short ushort4v __attribute__((ext_vector_type(4)));
+typedef unsigned short ushort8v __attribute__((ext_vector_type(8)));
+
+struct hbs_grouped_plan {
+    int device;
+    int precision;
+    int count;
+    int nblocks;
+    int64_t items;
+    int64_t tiles;
+    int64_t split_tiles;
+    double est_imbalance;
+    void *dmem;  // problems | items | block offsets | counters | partials
+    const void *d_probs;
+    const void *d_items;
+    const int *d_boff;
+    int *d_counters;
+    float *d_partials;
+};
+
+namespace {
+
+struct Prob {
+    const float *A;
+    const float *B;
+    float *C;
+    int M, N, K, lda, ldb, ldc;
+};
+
+struct Item {
+    int prob;
+    int tm, tn;
+    int k0, k1;
+    int cfg;       // tile shape id
+    int nsplit;    // splits of this tile
+    int split;     // this item's split index
+    int slot;      // counter index of the tile (split tiles only)
+    int64_t poff;  // float offset of the tile's partial slots (split tiles only)
+};
+
+// Tile shapes: id -> (BM, BN).  All use 256 threads as 2 x 2 waves.
+constexpr int kTileBM[4] = {128, 64, 128, 64};
+constexpr int kTileBN[4] = {128, 64, 64, 128};
+constexpr int kThreads = 256;
+
+// ---------------------------------------------------------------------------
+// shared helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint16_t bf16_rne(float x) {
+    uint32_t u = __float_as_uint(x);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (uint16_t)((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ void split_bf16(float x, uint16_t &hi, uint16_t &lo) {
+    hi = bf16_rne(x);
+    lo = bf16_rne(x - __uint_as_float((uint32_t)hi << 16));
+}
+
+// Epilogue for a 2x2-wave tile of FMxFN 32x32 accumulator blocks per wave.
+// Direct tiles store C; split tiles store a partial, and the last split to
+// arrive sums all partials in split order and stores C.
+template <int BM, int BN>
+__device__ __forceinline__ void finish_tile(floatx16 (&acc)[BM / 64][BN / 64], const Prob &p,
+                                            const Item &it, float *partials, int *counters,
+                                            int *s_flag) {
+    constexpr int FM = BM / 64, FN = BN / 64;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int l32 = lane & 31, h = lane >> 5;
+    const int m0 = it.tm * BM, n0 = it.tn * BN;
+
+    if (it.nsplit > 1) {
+        float *own = partials + it.poff + (int64_t)it.split * BM * BN + wave * (FM * FN * 16 * 64);
+#pragma unroll
+        for (int r = 0; r < FM; ++r)
+#pragma unroll
+            for (int c = 0; c < FN; ++c)
+#pragma unroll
+                for (int q = 0; q < 16; ++q) own[((r * FN + c) * 16 + q) * 64 + lane] = acc[r][c][q];
+        __threadfence();
+        __syncthreads();
+        if (tid == 0) {
+            const int old = __hip_atomic_fetch_add(&counters[it.slot], 1, __ATOMIC_ACQ_REL,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+            *s_flag = (old == it.nsplit - 1);
+        }
+        __syncthreads();
+        if (!*s_flag) return;
+        __threadfence();
+#pragma unroll
+        for (int r = 0; r < FM; ++r)
+#pragma unroll
+            for (int c = 0; c < FN; ++c)
+#pragma unroll
+                for (int q = 0; q < 16; ++q) acc[r][c][q] = 0.0f;
+        for (int s = 0; s < it.nsplit; ++s) {
+            const float *src = partials + it.poff + (int64_t)s * BM * BN + wave * (FM * FN * 16 * 64);
+#pragma unroll
+            for (int r = 0; r < FM; ++r)
+#pragma unroll
+                for (int c = 0; c < FN; ++c)
+#pragma unroll
+                    for (int q = 0; q < 16; ++q)
+                        acc[r][c][q] += __builtin_nontemporal_load(&src[((r * FN + c) * 16 + q) * 64 + lane]);
+        }
+        if (tid == 0)
+            __hip_atomic_store(&counters[it.slot], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+#pragma unroll
+    for (int r = 0; r < FM; ++r) {
+#pragma unroll
+        for (int c = 0; c < FN; ++c) {
+            const int col = n0 + wn * (BN / 2) + c * 32 + l32;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int row = m0 + wm * (BM / 2) + r * 32 + 8 * (q >> 2) + 4 * h + (q & 3);
+                if (row < p.M && col < p.N) p.C[(int64_t)row * p.ldc + col] = acc[r][c][q];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP32-MFMA tile: BK = 16, v_mfma_f32_32x32x2_f32, LDS double buffer.
+// ---------------------------------------------------------------------------
+template <int BM, int BN>
+struct F32T {
+    static constexpr int BK = 16, LDA = BK + 4, LDB = BN + 8;
+    static constexpr int STAGE = BM * LDA + BK * LDB;  // floats
+    static constexpr int AF4 = BM * BK / 4 / kThreads, BF4 = BK * BN / 4 / kThreads;
+    static constexpr int LDS_BYTES = 2 * STAGE * 4;
+};
+
+template <int BM, int BN>
+__device__ void f32_tile(float *smem, const Prob &p, const Item &it, float *partials,
+                         int *counters, int *s_flag) {
+    using T = F32T<BM, BN>;
+    constexpr int FM = BM / 64, FN = BN / 64, BK = T::BK;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int l32 = lane & 31, h = lane >> 5;
+    const int m0 = it.tm * BM, n0 = it.tn * BN;
+
+    floatx16 acc[FM][FN];
+#pragma unroll
+    for (int r = 0; r < FM; ++r)
+#pragma unroll
+        for (int c = 0; c < FN; ++c) acc[r][c] = floatx16{0};
+
+    floatx4 ra[T::AF4], rb[T::BF4];
+    auto gload = [&](int k) {
+#pragma unroll
+        for (int i = 0; i < T::AF4; ++i) {
+            const int f = tid + i * kThreads;
+            const int row = f / (BK / 4), gk = k + (f % (BK / 4)) * 4;
+            ra[i] = (m0 + row < p.M && gk < it.k1)
+                        ? *(const floatx4 *)(p.A + (int64_t)(m0 + row) * p.lda + gk)
+                        : floatx4{0, 0, 0, 0};
+        }
+#pragma unroll
+        for (int i = 0; i < T::BF4; ++i) {
+            const int f = tid + i * kThreads;
+            const int gk = k + f / (BN / 4), gn = n0 + (f % (BN / 4)) * 4;
+            rb[i] = (gk < it.k1 && gn < p.N) ? *(const floatx4 *)(p.B + (int64_t)gk * p.ldb + gn)
+                                              : floatx4{0, 0, 0, 0};
+        }
+    };
+    auto sstore = [&](int buf) {
+        float *as = smem + buf * T::STAGE;
+        float *bs = as + BM * T::LDA;
+#pragma unroll
+        for (int i = 0; i < T::AF4; ++i) {
+            const int f = tid + i * kThreads;
+            *(floatx4 *)(as + (f / (BK / 4)) * T::LDA + (f % (BK / 4)) * 4) = ra[i];
+        }
+#pragma unroll
+        for (int i = 0; i < T::BF4; ++i) {
+            const int f = tid + i * kThreads;
+            *(floatx4 *)(bs + (f / (BN / 4)) * T::LDB + (f % (BN / 4)) * 4) = rb[i];
+        }
+    };
+    auto compute = [&](int buf) {
+        const float *as = smem + buf * T::STAGE + (wm * (BM / 2) + l32) * T::LDA + 4 * h;
+        const float *bs = smem + buf * T::STAGE + BM * T::LDA + 4 * h * T::LDB + wn * (BN / 2) + l32;
+#pragma unroll
+        for (int kc = 0; kc < BK / 8; ++kc) {
+            floatx4 a[FM];
+            float b[FN][4];
+#pragma unroll
+            for (int r = 0; r < FM; ++r) a[r] = *(const floatx4 *)(as + r * 32 * T::LDA + kc * 8);
+#pragma unroll
+            for (int c = 0; c < FN; ++c)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) b[c][q] = bs[(kc * 8 + q) * T::LDB + c * 32];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int r = 0; r < FM; ++r)
+#pragma unroll
+                    for (int c = 0; c < FN; ++c)
+                        acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[r][q], b[c][q], acc[r][c], 0, 0, 0);
+        }
+    };
+
+    const int KT = (it.k1 - it.k0 + BK - 1) / BK;
+    gload(it.k0);
+    sstore(0);
+    __syncthreads();
+    for (int kt = 0; kt < KT; ++kt) {
+        const int cur = kt & 1;
+        if (kt + 1 < KT) gload(it.k0 + (kt + 1) * BK);
+        compute(cur);
+        if (kt + 1 < KT) sstore(cur ^ 1);
+        __syncthreads();
+    }
+    finish_tile<BM, BN>(acc, p, it, partials, counters, s_flag);
+}
+
+// ---------------------------------------------------------------------------
+// BF16x3 tile: BK = 32, floats split into packed hi/lo rows while staging,
+// three v_mfma_f32_32x32x16_bf16 products per k step.
+// ---------------------------------------------------------------------------
+template <int BM, int BN>
+struct B3T {
+    static constexpr int BK = 32, ROW = 2 * BK + 8;  // packed bf16 per LDS row (+16 B pad)
+    static constexpr int STAGE = (BM + BN) * ROW;     // bf16 elements
+    static constexpr int AT = BM * 4 / kThreads;      // A tasks (row, 8-k group) per thread
+    static constexpr int BT = (2 * BN + kThreads - 1) / kThreads;  // B tasks (4 k x 4 n)
+    static constexpr int LDS_BYTES = 2 * STAGE * 2;
+};
+
+template <int BM, int BN>
+__device__ void b3_tile(uint16_t *smem, const Prob &p, const Item &it, float *partials,
+                        int *counters, int *s_flag) {
+    using T = B3T<BM, BN>;
+    constexpr int FM = BM / 64, FN = BN / 64, BK = T::BK, ROW = T::ROW;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int l32 = lane & 31, h = lane >> 5;
+    const int m0 = it.tm * BM, n0 = it.tn * BN;
+
+    floatx16 acc[FM][FN];
+#pragma unroll
+    for (int r = 0; r < FM; ++r)
+#pragma unroll
+        for (int c = 0; c < FN; ++c) acc[r][c] = floatx16{0};
+
+    floatx4 ra[T::AT][2], rb[T::BT][4];
+    auto gload = [&](int k) {
+#pragma unroll
+        for (int i = 0; i < T::AT; ++i) {
+            const int f = tid + i * kThreads;
+            const int row = f / 4, gk = k + (f % 4) * 8;
+            const bool rv = m0 + row < p.M;
+            const float *src = p.A + (int64_t)(m0 + row) * p.lda + gk;
+            ra[i][0] = (rv && gk < it.k1) ? *(const floatx4 *)src : floatx4{0, 0, 0, 0};
+            ra[i][1] = (rv && gk + 4 < it.k1) ? *(const floatx4 *)(src + 4) : floatx4{0, 0, 0, 0};
+        }
+#pragma unroll
+        for (int i = 0; i < T::BT; ++i) {
+            const int f = tid + i * kThreads;
+            const int hg = f / (BN / 4), gn = n0 + (f % (BN / 4)) * 4;
+            const bool task = f < 2 * BN;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int gk = k + hg * 4 + j;
+                rb[i][j] = (task && gk < it.k1 && gn < p.N)
+                               ? *(const floatx4 *)(p.B + (int64_t)gk * p.ldb + gn)
+                               : floatx4{0, 0, 0, 0};
+            }
+        }
+    };
+    auto sstore = [&](int buf) {
+        uint16_t *as = smem + buf * T::STAGE;
+        uint16_t *bs = as + BM * ROW;
+#pragma unroll
+        for (int i = 0; i < T::AT; ++i) {
+            const int f = tid + i * kThreads;
+            const int row = f / 4, g = f % 4;
+            ushort8v hi, lo;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                uint16_t a, b;
+                split_bf16(ra[i][j / 4][j % 4], a, b);
+                hi[j] = a;
+                lo[j] = b;
+            }
+            *(ushort8v *)(as + row * ROW + g * 16) = hi;
+            *(ushort8v *)(as + row * ROW + g * 16 + 8) = lo;
+        }
+#pragma unroll
+        for (int i = 0; i < T::BT; ++i) {
+            const int f = tid + i * kThreads;
+            if (f >= 2 * BN) break;
+            const int hg = f / (BN / 4), c = f % (BN / 4);
+            const int off = (hg >> 1) * 16 + (hg & 1) * 4;
+#pragma unroll
+            for (int jn = 0; jn < 4; ++jn) {
+                ushort4v hi, lo;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    uint16_t a, b;
+                    split_bf16(rb[i][j][jn], a, b);
+                    hi[j] = a;
+                    lo[j] = b;
+                }
+                uint16_t *dst = bs + (c * 4 + jn) * ROW + off;
+                *(ushort4v *)dst = hi;
+                *(ushort4v *)(dst + 8) = lo;
+            }
+        }
+    };
+    auto compute = [&](int buf) {
+        const uint16_t *as = smem + buf * T::STAGE + (wm * (BM / 2) + l32) * ROW;
+        const uint16_t *bs = smem + buf * T::STAGE + BM * ROW + (wn * (BN / 2) + l32) * ROW;
+#pragma unroll
+        for (int st = 0; st < BK / 16; ++st) {
+            const int off = (2 * st + h) * 16;
+            bf16x8 ahi[FM], alo[FM], bhi[FN], blo[FN];
+#pragma unroll
+            for (int r = 0; r < FM; ++r) {
+                ahi[r] = *(const bf16x8 *)(as + r * 32 * ROW + off);
+                alo[r] = *(const bf16x8 *)(as + r * 32 * ROW + off + 8);
+            }
+#pragma unroll
+            for (int c = 0; c < FN; ++c) {
+                bhi[c] = *(const bf16x8 *)(bs + c * 32 * ROW + off);
+                blo[c] = *(const bf16x8 *)(bs + c * 32 * ROW + off + 8);
+            }
+#pragma unroll
+            for (int r = 0; r < FM; ++r)
+#pragma unroll
+                for (int c = 0; c < FN; ++c) {
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[r], bhi[c], acc[r][c], 0, 0, 0);
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[r], blo[c], acc[r][c], 0, 0, 0);
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[r], bhi[c], acc[r][c], 0, 0, 0);
+                }
+        }
+    };
+
+    const int KT = (it.k1 - it.k0 + BK - 1) / BK;
+    gload(it.k0);
+    sstore(0);
+    __syncthreads();
+    for (int kt = 0; kt < KT; ++kt) {
+        const int cur = kt & 1;
+        if (kt + 1 < KT) gload(it.k0 + (kt + 1) * BK);
+        compute(cur);
+        if (kt + 1 < KT) sstore(cur ^ 1);
+        __syncthreads();
+    }
+    finish_tile<BM, BN>(acc, p, it, partials, counters, s_flag);
+}
+
+constexpr int kLdsF32 = F32T<128, 128>::LDS_BYTES;
+constexpr int kLdsB3 = B3T<128, 128>::LDS_BYTES;
+static_assert(F32T<64, 128>::LDS_BYTES <= kLdsF32 && F32T<128, 64>::LDS_BYTES <= kLdsF32, "lds");
+
+template <int PREC>
+__global__ __launch_bounds__(kThreads, PREC == HBS_PREC_FP32 ? 2 : 1) void grouped_kernel(const Prob *__restrict__ probs,
+                                                              const Item *__restrict__ items,
+                                                              const int *__restrict__ boff,
+                                                              float *partials, int *counters) {
+    constexpr int LDS = PREC == HBS_PREC_FP32 ? kLdsF32 : kLdsB3;
+    __shared__ __attribute__((aligned(16))) char smem[LDS];
+    __shared__ int s_flag;
+    const int b = blockIdx.x;
+    for (int i = boff[b]; i < boff[b + 1]; ++i) {
+        const Item it = items[i];
+        const Prob p = probs[it.prob];
+        if constexpr (PREC == HBS_PREC_FP32) {
+            float *s = (float *)smem;
+            switch (it.cfg) {
+                case 0: f32_tile<128, 128>(s, p, it, partials, counters, &s_flag); break;
+                case 1: f32_tile<64, 64>(s, p, it, partials, counters, &s_flag); break;
+                case 2: f32_tile<128, 64>(s, p, it, partials, counters, &s_flag); break;
+                default: f32_tile<64, 128>(s, p, it, partials, counters, &s_flag); break;
+            }
+        } else {
+            uint16_t *s = (uint16_t *)smem;
+            switch (it.cfg) {
+                case 0: b3_tile<128, 128>(s, p, it, partials, counters, &s_flag); break;
+                default: b3_tile<64, 64>(s, p, it, partials, counters, &s_flag); break;
+            }
+        }
+        __syncthreads();  // s_flag and LDS are reused by the next item
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side: tuning table, plan construction
+// ---------------------------------------------------------------------------
+int size_class(int x) {
+    int c = 64;
+    while (c < x && c < 2048) c *= 2;
+    return c;
+}
+
+struct Choice {
+    int tile;
+    int split;
+    double ns;  // measured time of one full tile (0 = unknown)
+};
+
+using TuneKey = std::tuple<int, int, int, int>;  // prec, M class, N class, K class
+
+// Text format, one entry per line ('#' starts a comment):
+//   <fp32|bf16x3> <M class> <N class> <K class> <tile id> <split> <ns per tile>
+bool load_tuning(const char *path, std::map<TuneKey, Choice> &table) {
+    std::ifstream in(path);
+    if (!in) return false;
+    std::string line;
+    while (std::getline(in, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        std::istringstream ss(line);
+        std::string prec;
+        int mc, nc, kc, tile, split;
+        double ns = 0.0;
+        if (!(ss >> prec >> mc >> nc >> kc >> tile >> split)) continue;
+        ss >> ns;
+        const int pr = prec == "fp32" ? HBS_PREC_FP32 : prec == "bf16x3" ? HBS_PREC_BF16X3 : -1;
+        if (pr < 0 || tile < 0 || tile > 3 || split < 1 || split > 64) continue;
+        table[TuneKey(pr, mc, nc, kc)] = Choice{tile, split, ns};
+    }
+    return true;
+}
+
+// BF16x3 tiles are 128x128 or 64x64 only: with all four shapes inlined in one
+// kernel the hi/lo staging exceeds the 256 architectural VGPRs and spills.
+bool tile_supported(int precision, int tile) {
+    return tile >= 0 && tile <= 3 && (precision == HBS_PREC_FP32 || tile <= 1);
+}
+
+Choice heuristic(int precision, int M, int N, int K) {
+    Choice c{0, 1, 0.0};
+    if (M <= 64 && N <= 64) c.tile = 1;
+    else if (M <= 64) c.tile = 3;
+    else if (N <= 64) c.tile = 2;
+    if (!tile_supported(precision, c.tile)) c.tile = (M <= 64 || N <= 64) ? 1 : 0;
+    const int64_t tiles = (int64_t)((M + kTileBM[c.tile] - 1) / kTileBM[c.tile]) *
+                          ((N + kTileBN[c.tile] - 1) / kTileBN[c.tile]);
+    if (tiles <= 4 && K >= 1024) c.split = 4;
+    else if (tiles <= 16 && K >= 1024) c.split = 2;
+    return c;
+}
+
+int validate(const hbs_gemm_problem &q) {
+    if (q.M < 1 || q.N < 1 || q.K < 1) return HBS_ERR_ARG;
+    if (q.lda < q.K || q.ldb < q.N || q.ldc < q.N) return HBS_ERR_ARG;
+    if (!q.A || !q.B || !q.C) return HBS_ERR_NULL;
+    if (!hbs::aligned16(q.A) || !hbs::aligned16(q.B)) return HBS_ERR_ALIGN;
+    if (q.N % 4 || q.K % 4 || q.lda % 4 || q.ldb % 4) return HBS_ERR_ALIGN;
+    return HBS_SUCCESS;
+}
+
+int build_plan(const hbs_gemm_problem *problems, int count, int precision, const Choice *forced,
+               const std::map<TuneKey, Choice> *table, hbs_grouped_plan **out) {
+    if (!out) return HBS_ERR_NULL;
+    *out = nullptr;
+    if (count < 1) return HBS_ERR_ARG;
+    if (!problems) return HBS_ERR_NULL;
+    if (precision != HBS_PREC_FP32 && precision != HBS_PREC_BF16X3) return HBS_ERR_ARG;
+    for (int i = 0; i < count; ++i) {
+        const int v = validate(problems[i]);
+        if (v) return v;
+    }
+    int st = hbs::check_device();
+    if (st) return st;
+    int dev = 0, cus = 256;
+    HBS_HIP_TRY(hipGetDevice(&dev));
+    HBS_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+
+    std::vector<Prob> probs(count);
+    std::vector<Item> items;
+    std::vector<double> cost;
+    int64_t slots = 0, poff = 0, tiles_total = 0;
+    for (int i = 0; i < count; ++i) {
+        const hbs_gemm_problem &q = problems[i];
+        probs[i] = Prob{q.A, q.B, q.C, q.M, q.N, q.K, q.lda, q.ldb, q.ldc};
+        Choice c = heuristic(precision, q.M, q.N, q.K);
+        if (forced) {
+            c = *forced;
+        } else if (table) {
+            auto f = table->find(TuneKey(precision, size_class(q.M), size_class(q.N), size_class(q.K)));
+            if (f != table->end() && tile_supported(precision, f->second.tile)) c = f->second;
+        }
+        const int BM = kTileBM[c.tile], BN = kTileBN[c.tile];
+        const int BK = precision == HBS_PREC_FP32 ? 16 : 32;
+        const int tm_n = (q.M + BM - 1) / BM, tn_n = (q.N + BN - 1) / BN;
+        int chunk = (q.K + c.split - 1) / c.split;
+        chunk = (chunk + BK - 1) / BK * BK;
+        const int ns = (q.K + chunk - 1) / chunk;
+        // relative cost model: bigger tiles run closer to the matrix-core peak
+        const double eff = (c.tile == 0) ? 1.0 : (c.tile == 1 ? 0.55 : 0.8);
+        for (int tm = 0; tm < tm_n; ++tm) {
+            for (int tn = 0; tn < tn_n; ++tn) {
+                ++tiles_total;
+                for (int s = 0; s < ns; ++s) {
+                    Item it;
+                    it.prob = i;
+                    it.tm = tm;
+                    it.tn = tn;
+                    it.k0 = s * chunk;
+                    it.k1 = std::min(q.K, (s + 1) * chunk);
+                    it.cfg = c.tile;
+                    it.nsplit = ns;
+                    it.split = s;
+                    it.slot = ns > 1 ? (int)slots : -1;
+                    it.poff = ns > 1 ? poff : 0;
+                    items.push_back(it);
+                    cost.push_back((double)BM * BN * (it.k1 - it.k0) / eff + 2.0e4);
+                }
+                if (ns > 1) {
+                    ++slots;
+                    poff += (int64_t)ns * BM * BN;
+                }
+            }
+        }
+    }
+    if (items.size() > 0x7fffffffULL) return HBS_ERR_ARG;
+
+    // LPT: items in decreasing cost, each to the least-loaded workgroup.
+    // FP32 tiles fit two workgroups per CU (<= 256 VGPRs); BF16x3 tiles need
+    // more registers for the hi/lo staging and run one workgroup per CU.
+    const int per_cu = precision == HBS_PREC_FP32 ? 2 : 1;
+    const int nblocks = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)per_cu * cus, (int64_t)items.size()));
+    std::vector<int64_t> order(items.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = (int64_t)i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int64_t a, int64_t b) { return cost[a] > cost[b]; });
+    using Load = std::pair<double, int>;
+    std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
+    for (int b = 0; b < nblocks; ++b) heap.push(Load(0.0, b));
+    std::vector<std::vector<int64_t>> lists(nblocks);
+    for (int64_t idx : order) {
+        Load l = heap.top();
+        heap.pop();
+        lists[l.second].push_back(idx);
+        l.first += cost[idx];
+        heap.push(l);
+    }
+    double maxl = 0.0, suml = 0.0;
+    while (!heap.empty()) {
+        maxl = std::max(maxl, heap.top().first);
+        suml += heap.top().first;
+        heap.pop();
+    }
+    std::vector<Item> sorted;
+    sorted.reserve(items.size());
+    std::vector<int> boff(nblocks + 1, 0);
+    for (int b = 0; b < nblocks; ++b) {
+        boff[b] = (int)sorted.size();
+        for (int64_t idx : lists[b]) sorted.push_back(items[idx]);
+    }
+    boff[nblocks] = (int)sorted.size();
+
+    // one device allocation: problems | items | offsets | counters | partials
+    auto align = [](size_t x) { return (x + 255) / 256 * 256; };
+    const size_t b_probs = align(probs.size() * sizeof(Prob));
+    const size_t b_items = align(sorted.size() * sizeof(Item));
+    const size_t b_boff = align(boff.size() * sizeof(int));
+    const size_t b_cnt = align(std::max<int64_t>(slots, 1) * sizeof(int));
+    const size_t b_part = align(std::max<int64_t>(poff, 1) * sizeof(float));
+    char *dmem = nullptr;
+    HBS_HIP_TRY(hipMalloc(&dmem, b_probs + b_items + b_boff + b_cnt + b_part));
+    hbs_grouped_plan *plan = new hbs_grouped_plan();
+    plan->device = dev;
+    plan->precision = precision;
+    plan->count = count;
+    plan->nblocks = nblocks;
+    plan->items = (int64_t)sorted.size();
+    plan->tiles = tiles_total;
+    plan->split_tiles = slots;
+    plan->est_imbalance = suml > 0.0 ? maxl / (suml / nblocks) : 1.0;
+    plan->dmem = dmem;
+    plan->d_probs = dmem;
+    plan->d_items = dmem + b_probs;
+    plan->d_boff = (const int *)(dmem + b_probs + b_items);
+    plan->d_counters = (int *)(dmem + b_probs + b_items + b_boff);
+    plan->d_partials = (float *)(dmem + b_probs + b_items + b_boff + b_cnt);
+    hipError_t e = hipMemcpy(dmem, probs.data(), probs.size() * sizeof(Prob), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(dmem + b_probs, sorted.data(), sorted.size() * sizeof(Item), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(dmem + b_probs + b_items, boff.data(), boff.size() * sizeof(int),
+                      hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(plan->d_counters, 0, b_cnt);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        (void)hipFree(dmem);
+        delete plan;
+        return (int)e;
+    }
+    *out = plan;
+    return HBS_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" int hbs_grouped_plan_create(const hbs_gemm_problem *problems, int count, int precision,
+                                       const char *tuning_file, hbs_grouped_plan **plan) {
+    std::map<TuneKey, Choice> table;
+    const bool have = tuning_file && tuning_file[0] && load_tuning(tuning_file, table);
+    if (tuning_file && tuning_file[0] && !have) {
+        if (plan) *plan = nullptr;
+        return HBS_ERR_ARG;
+    }
+    return build_plan(problems, count, precision, nullptr, have ? &table : nullptr, plan);
+}
+
+extern "C" int hbs_grouped_plan_create_forced(const hbs_gemm_problem *problems, int count,
+                                              int precision, int tile, int split_k,
+                                              hbs_grouped_plan **plan) {
+    if (!tile_supported(precision, tile) || split_k < 1 || split_k > 64) {
+        if (plan) *plan = nullptr;
+        return HBS_ERR_ARG;
+    }
+    const Choice c{tile, split_k, 0.0};
+    return build_plan(problems, count, precision, &c, nullptr, plan);
+}
+
+extern "C" int hbs_grouped_plan_info(const hbs_grouped_plan *plan, hbs_grouped_info *info) {
+    if (!plan || !info) return HBS_ERR_NULL;
+    info->problems = plan->count;
+    info->workgroups = plan->nblocks;
+    info->tiles = plan->tiles;
+    info->work_items = plan->items;
+    info->split_tiles = plan->split_tiles;
+    info->est_imbalance = plan->est_imbalance;
+    return HBS_SUCCESS;
+}
+
+extern "C" int hbs_grouped_run(const hbs_grouped_plan *plan, void *stream) {
+    if (!plan) return HBS_ERR_NULL;
+    int dev = 0;
+    HBS_HIP_TRY(hipGetDevice(&dev));
+    if (dev != plan->device) return HBS_ERR_DEVICE;
+    hipStream_t s = (hipStream_t)stream;
+    const Prob *pr = (const Prob *)plan->d_probs;
+    const Item *it = (const Item *)plan->d_items;
+    if (plan->precision == HBS_PREC_FP32)
+        grouped_kernel<HBS_PREC_FP32><<<plan->nblocks, kThreads, 0, s>>>(pr, it, plan->d_boff,
+                                                                         plan->d_partials,
+                                                                         plan->d_counters);
+    else
+        grouped_kernel<HBS_PREC_BF16X3><<<plan->nblocks, kThreads, 0, s>>>(pr, it, plan->d_boff,
+                                                                           plan->d_partials,
+                                                                           plan->d_counters);
+    return hbs::launch_status();
+}
+
+extern "C" int hbs_grouped_plan_destroy(hbs_grouped_plan *plan) {
+    if (!plan) return HBS_SUCCESS;
+    hipError_t e = hipFree(plan->dmem);
+    delete plan;
+    return e == hipSuccess ? HBS_SUCCESS : (int)e;
+}
+
+// hipGraph helpers: capture one grouped run into an executable graph.
+extern "C" int hbs_grouped_graph_create(const hbs_grouped_plan *plan, void **graph_exec) {
+    if (!plan || !graph_exec) return HBS_ERR_NULL;
+    *graph_exec = nullptr;
+    hipStream_t s;
+    HBS_HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    int rc = HBS_SUCCESS;
+    if (e == hipSuccess) {
+        rc = hbs_grouped_run(plan, s);
+        e = hipStreamEndCapture(s, &g);
+    }
+    hipGraphExec_t ex = nullptr;
+    if (e == hipSuccess && rc == HBS_SUCCESS) e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipStreamDestroy(s);
+    if (rc != HBS_SUCCESS) return rc;
+    if (e != hipSuccess) return (int)e;
+    *graph_exec = (void *)ex;
+    return HBS_SUCCESS;
+}
+
+extern "C" int hbs_graph_launch(void *graph_exec, void *stream) {
+    if (!graph_exec) return HBS_ERR_NULL;
+    HBS_HIP_TRY(hipGraphLaunch((hipGraphExec_t)graph_exec, (hipStream_t)stream));
+    return HBS_SUCCESS;
+}
+
+extern "C" int hbs_graph_destroy(void *graph_exec) {
+    if (!graph_exec) return HBS_SUCCESS;
+    HBS_HIP_TRY(hipGraphExecDestroy((hipGraphExec_t)graph_exec));
+    return HBS_SUCCESS;
+}

@@ -87,6 +87,61 @@ int hbs_sgemm_bf16x3(const float *A, const float *B, float *C, int N, int niter,
 
 
 /* ------------------------------------------------------------------------ */
+/* Grouped persistent GEMM (no reference counterpart): many independent        */
+/* problems of uneven M/N/K, FP32-MFMA or BF16x3 arithmetic, one launch.       */
+/* Generalises the reference's NITER equal slices in one launch               */
+/* (apps/sgemm/kernel.cpp:293-297) to uneven problems.                        */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    const float *A; /* M x K, row-major, leading dimension lda             */
+    const float *B; /* K x N, row-major, leading dimension ldb             */
+    float *C;       /* M x N, row-major, leading dimension ldc (written)   */
+    int M, N, K;
+    int lda, ldb, ldc;
+} hbs_gemm_problem;
+
+enum { HBS_PREC_FP32 = 0, HBS_PREC_BF16X3 = 1 };
+/* Tile shapes of the grouped kernel (BM x BN). */
+enum { HBS_TILE_128x128 = 0, HBS_TILE_64x64 = 1, HBS_TILE_128x64 = 2, HBS_TILE_64x128 = 3 };
+
+typedef struct hbs_grouped_plan hbs_grouped_plan;
+
+typedef struct {
+    int problems;
+    int workgroups;       /* persistent workgroups of one run               */
+    int64_t tiles;        /* output tiles over all problems                 */
+    int64_t work_items;   /* tiles x K splits                               */
+    int64_t split_tiles;  /* tiles computed with split-K                    */
+    double est_imbalance; /* max / mean modelled workgroup load             */
+} hbs_grouped_info;
+
+/* Builds a plan for `count` problems (host array; the matrices are device
+ * pointers that must stay valid while the plan is used).  Requirements per
+ * problem: M, N, K >= 1, lda >= K, ldb >= N, ldc >= N; A and B 16-byte aligned
+ * with N, K, lda, ldb multiples of 4 (HBS_ERR_ALIGN otherwise).  Tile shape
+ * and split-K come from `tuning_file` (written by the autotuner) when it is
+ * non-NULL -- HBS_ERR_ARG if it cannot be read -- else from a size heuristic.
+ * Allocates and uploads the plan's device buffers (synchronous). */
+int hbs_grouped_plan_create(const hbs_gemm_problem *problems, int count, int precision,
+                            const char *tuning_file, hbs_grouped_plan **plan);
+/* Same with one tile shape and split-K factor (1..64) for every problem (the
+ * autotuner's probe).  BF16x3 supports HBS_TILE_128x128 and HBS_TILE_64x64;
+ * other shapes give HBS_ERR_ARG. */
+int hbs_grouped_plan_create_forced(const hbs_gemm_problem *problems, int count, int precision,
+                                   int tile, int split_k, hbs_grouped_plan **plan);
+int hbs_grouped_plan_info(const hbs_grouped_plan *plan, hbs_grouped_info *info);
+/* Computes every problem of the plan in one kernel launch on `stream`.
+ * Deterministic; runs of one plan must not overlap (they share counters). */
+int hbs_grouped_run(const hbs_grouped_plan *plan, void *stream);
+int hbs_grouped_plan_destroy(hbs_grouped_plan *plan);
+
+/* Captures one hbs_grouped_run of `plan` into an executable hipGraph
+ * (returned as hipGraphExec_t in *graph_exec); replay with hbs_graph_launch. */
+int hbs_grouped_graph_create(const hbs_grouped_plan *plan, void **graph_exec);
+int hbs_graph_launch(void *graph_exec, void *stream);
+int hbs_graph_destroy(void *graph_exec);
+
+/* ------------------------------------------------------------------------ */
 /* Input generation on the device.                                            */
 /* ------------------------------------------------------------------------ */
 

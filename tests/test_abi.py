@@ -103,3 +103,32 @@ def test_bf16x3_workspace_contract():
     assert L.hbs_sgemm_bf16x3(P, P, P, 0, 1, P, 1 << 30, None) == -1
     assert L.hbs_sgemm_bf16x3(None, P, P, 64, 1, P, 1 << 30, None) == -2
     assert L.hbs_sgemm_bf16x3(P, P, P, 64, 1, ctypes.c_void_p(24), 1 << 30, None) == -3
+
+
+def test_grouped_plan_argument_checks():
+    L = hb.lib()
+    h = ctypes.c_void_p()
+    good = hb.Problem(16, 32, 48, 8, 8, 8, 8, 8, 8)
+    arr = (hb.Problem * 1)(good)
+    assert L.hbs_grouped_plan_create(arr, 0, 0, None, ctypes.byref(h)) == -1
+    assert L.hbs_grouped_plan_create(None, 1, 0, None, ctypes.byref(h)) == -2
+    assert L.hbs_grouped_plan_create(arr, 1, 7, None, ctypes.byref(h)) == -1
+    assert L.hbs_grouped_plan_create(arr, 1, 0, None, None) == -2
+    bad = (hb.Problem * 1)(hb.Problem(16, 32, 48, 8, 6, 8, 8, 6, 8))    # N % 4
+    assert L.hbs_grouped_plan_create(bad, 1, 0, None, ctypes.byref(h)) == -3
+    bad = (hb.Problem * 1)(hb.Problem(20, 32, 48, 8, 8, 8, 8, 8, 8))    # A misaligned
+    assert L.hbs_grouped_plan_create(bad, 1, 0, None, ctypes.byref(h)) == -3
+    bad = (hb.Problem * 1)(hb.Problem(16, 32, 48, 8, 8, 8, 4, 8, 8))    # lda < K
+    assert L.hbs_grouped_plan_create(bad, 1, 0, None, ctypes.byref(h)) == -1
+    bad = (hb.Problem * 1)(hb.Problem(None, 32, 48, 8, 8, 8, 8, 8, 8))
+    assert L.hbs_grouped_plan_create(bad, 1, 0, None, ctypes.byref(h)) == -2
+    assert L.hbs_grouped_plan_create(arr, 1, 0, b"/nonexistent/tuning.txt", ctypes.byref(h)) == -1
+    assert L.hbs_grouped_plan_create_forced(arr, 1, 0, 4, 1, ctypes.byref(h)) == -1
+    assert L.hbs_grouped_plan_create_forced(arr, 1, 1, 2, 1, ctypes.byref(h)) == -1
+    assert L.hbs_grouped_plan_create_forced(arr, 1, 0, 0, 0, ctypes.byref(h)) == -1
+    assert L.hbs_grouped_run(None, None) == -2
+    assert L.hbs_grouped_plan_destroy(None) == 0
+    assert L.hbs_grouped_plan_info(None, None) == -2
+    assert L.hbs_grouped_graph_create(None, ctypes.byref(h)) == -2
+    assert L.hbs_graph_launch(None, None) == -2
+    assert L.hbs_graph_destroy(None) == 0
