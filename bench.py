@@ -29,8 +29,8 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--variant", choices=["fp32", "bf16x3", "grouped-fp32", "grouped-bf16x3"],
-                   default="fp32")
+    p.add_argument("--variant", choices=["fp32", "bf16x3", "grouped-fp32", "grouped-bf16x3",
+                                         "fp8res"], default="fp32")
     p.add_argument("--problems", type=int, default=512, help="grouped: number of problems")
     p.add_argument("--tuning", default=None, help="grouped: autotuner tuning file")
     p.add_argument("--n", type=int, default=8192)
@@ -180,6 +180,33 @@ def setup_grouped(args, hb, torch):
     return step, flops, workload, {"verify": verify, "config": cfg, "keep": (plan, graph)}
 
 
+def setup_fp8res(args, hb, torch):
+    """FP8-e4m3 + BF16 residual, M = N = 16384, K = 4096, normal(0,1) inputs with
+    1% of entries scaled by 1e3; quantisation pre-passes are inside the step."""
+    M = N = 16384
+    K = 4096
+    a = torch.empty(M * K, dtype=torch.float32, device="cuda")
+    b = torch.empty(K * N, dtype=torch.float32, device="cuda")
+    c = torch.empty(M * N, dtype=torch.float32, device="cuda")
+    hb.fill_random(a, M * K, hb.GEN_NORMAL_OUTLIER, args.seed)
+    hb.fill_random(b, K * N, hb.GEN_NORMAL_OUTLIER, args.seed + 1)
+    ws = torch.empty(hb.fp8res_workspace_size(M, N, K), dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+
+    def step():
+        hb.gemm_fp8res(a, b, c, M, N, K, workspace=ws)
+
+    def verify():
+        st = hb.verify_f64(a, b, c, M, N, K, row_begin=8192, row_end=8192 + 512)
+        return {"rows": "8192..8703", "e_cw": st["e_cw"], "e_nw": st["e_nw"], "e_F": st["e_F"],
+                "oracle": "FP64 on device (hbs_verify_f64)"}
+
+    workload = (f"FP8-e4m3 per-128x128-tile scales + BF16 residual GEMM, M=N=16384, K=4096, "
+                f"normal(0,1) with 1% x1e3 outliers seed {args.seed}")
+    return step, 2.0 * M * N * K, workload, {"verify": verify,
+                                             "config": {"M": M, "N": N, "K": K}}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -196,6 +223,8 @@ def main():
         return
     if args.variant.startswith("grouped"):
         setup = setup_grouped(args, hb, torch)
+    elif args.variant == "fp8res":
+        setup = setup_fp8res(args, hb, torch)
     else:
         setup = setup_square(args, hb, torch)
     step, flops, workload, extra = setup
@@ -220,7 +249,13 @@ def main():
 
     value = flops * args.steps / wall / 1e12
     achieved = flops / (avg_ms / 1e3) / 1e12
-    peak = PEAK_TFLOPS["fp32"] if args.variant.endswith("fp32") else PEAK_TFLOPS["bf16"] / 3.0
+    if args.variant.endswith("fp32"):
+        peak = PEAK_TFLOPS["fp32"]
+    elif args.variant == "fp8res":
+        # one FP8 product and two BF16 products per FP32-equivalent flop
+        peak = 1.0 / (1.0 / PEAK_TFLOPS["fp8"] + 2.0 / PEAK_TFLOPS["bf16"])
+    else:
+        peak = PEAK_TFLOPS["bf16"] / 3.0
     line = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -232,7 +267,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if args.variant.endswith("fp32") else "bf16x3->f32",
+        "dtype": {"fp8res": "e4m3+bf16->f32"}.get(
+            args.variant, "f32" if args.variant.endswith("fp32") else "bf16x3->f32"),
         "data": "synthetic",
         "config": dict(workload=workload, variant=args.variant, flops_per_step=flops,
                        **extra.get("config", {})),

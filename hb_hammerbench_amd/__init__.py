@@ -17,7 +17,7 @@ __all__ = [
     "HbsError", "lib", "library_path", "sgemm", "sgemm_strided", "sgemm_bf16x3",
     "bf16x3_workspace_size", "fill_pattern", "fill_random", "verify_f64",
     "GEN_UNIFORM", "GEN_LOGUNIFORM", "GEN_NORMAL_OUTLIER", "PREC_FP32", "PREC_BF16X3",
-    "GroupedPlan", "Graph",
+    "GroupedPlan", "Graph", "gemm_fp8res", "fp8res_workspace_size", "gemm_bf16x3",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -86,6 +86,10 @@ SIGNATURES = {
     "hbs_grouped_graph_create": (_i, [_vp, ctypes.POINTER(_vp)]),
     "hbs_graph_launch": (_i, [_vp, _vp]),
     "hbs_graph_destroy": (_i, [_vp]),
+    "hbs_gemm_bf16x3_workspace_size": (_sz, [_i, _i, _i]),
+    "hbs_gemm_bf16x3": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _vp, _sz, _vp]),
+    "hbs_gemm_fp8res_workspace_size": (_sz, [_i, _i, _i]),
+    "hbs_gemm_fp8res": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _vp, _sz, _vp]),
     "hbs_fill_pattern": (_i, [_vp, _vp, _i64, _i64, _vp]),
     "hbs_fill_random": (_i, [_vp, _i64, _i, _u64, _u64, _vp]),
     "hbs_verify_f64": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _i,
@@ -161,6 +165,39 @@ def sgemm_bf16x3(mat1, mat2, result, N, niter=1, workspace=None, stream=None):
     _check(lib().hbs_sgemm_bf16x3(_ptr(mat1), _ptr(mat2), _ptr(result), N, niter,
                                   _ptr(workspace), nbytes, _stream(stream)),
            "hbs_sgemm_bf16x3")
+
+
+def gemm_bf16x3(A, B, C, M, N, K, lda=None, ldb=None, ldc=None, workspace=None, stream=None):
+    """Rectangular BF16x3 split-precision GEMM."""
+    ws_bytes = int(lib().hbs_gemm_bf16x3_workspace_size(M, N, K))
+    if workspace is None:
+        import torch
+        workspace = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device="cuda")
+    nbytes = workspace.numel() * workspace.element_size()
+    _check(lib().hbs_gemm_bf16x3(_ptr(A), _ptr(B), _ptr(C), M, N, K,
+                                 K if lda is None else lda, N if ldb is None else ldb,
+                                 N if ldc is None else ldc, _ptr(workspace), nbytes,
+                                 _stream(stream)),
+           "hbs_gemm_bf16x3")
+
+
+def fp8res_workspace_size(M, N, K):
+    return int(lib().hbs_gemm_fp8res_workspace_size(M, N, K))
+
+
+def gemm_fp8res(A, B, C, M, N, K, lda=None, ldb=None, ldc=None, workspace=None, stream=None):
+    """FP8-e4m3 (per-128x128-tile power-of-two scales) + BF16 residual GEMM."""
+    ws_bytes = fp8res_workspace_size(M, N, K)
+    if workspace is None:
+        import torch
+        workspace = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device="cuda")
+    nbytes = workspace.numel() * workspace.element_size()
+    _check(lib().hbs_gemm_fp8res(_ptr(A), _ptr(B), _ptr(C), M, N, K,
+                                 K if lda is None else lda, N if ldb is None else ldb,
+                                 N if ldc is None else ldc, _ptr(workspace), nbytes,
+                                 _stream(stream)),
+           "hbs_gemm_fp8res")
+    return workspace
 
 
 def fill_pattern(mat1, mat2, n, first=0, stream=None):

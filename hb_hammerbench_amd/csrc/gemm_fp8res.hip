@@ -1,0 +1,368 @@
+// gemm_fp8res.hip -- FP8-e4m3 GEMM with per-tile scaling and a BF16 residual.
+//
+// No reference counterpart: a precision variant of apps/sgemm's C = A.B
+// (kernel.cpp:279-333 call shape, generalised to M x N x K).
+//
+// Quantisation (per 128 x 128 tile T of A, and of B, with amax = max |x| on T):
+//   e    = the power-of-two exponent with 224 < amax / 2^e <= 448
+//   q(x) = 2^e * e4m3_rne(x / 2^e)                 (exact in bf16)
+//   r(x) = bf16_rne(x - q(x))                      (x - q(x) is exact)
+// Product: C = Aq.Bq + Aq.Br + Ar.Bq, FP32 accumulate; Ar.Br is dropped.
+//   Aq.Bq runs on the FP8 matrix cores with the tile exponents applied as
+//   E8M0 block scales by v_mfma_scale_f32_32x32x64_f8f6f4; the two residual
+//   products run on v_mfma_f32_32x32x16_bf16.  Power-of-two scales keep small
+//   integers exact, so the reference pattern (0..6 x 0..2) stays bit-exact.
+//
+// Launches: quantise A, quantise B (transposed, K-contiguous), GEMM.  The
+// workspace holds, per operand X (rows R = M or N, padded to Rp; Kp = K padded
+// to 128):  X8 [Rp][Kp] e4m3 bytes, Xp [Rp][Kp/8][q8 | r8] bf16 groups, and the
+// tile exponents eX [Rp/128][Kp/128] int32 (for B indexed [n tile][k tile]).
+#include "hbs_internal.h"
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef int intx8 __attribute__((ext_vector_type(8)));
+typedef unsigned short ushort8v __attribute__((ext_vector_type(8)));
+
+namespace {
+
+constexpr int kQT = 128;  // quantisation tile edge
+
+__device__ __forceinline__ uint16_t bf16_rne(float x) {
+    uint32_t u = __float_as_uint(x);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (uint16_t)((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+// e4m3 (bias 7, max 448) round-to-nearest-even of v, |v| <= 448.
+__device__ __forceinline__ float e4m3_round(float v) {
+    const float a = fabsf(v);
+    if (a == 0.0f) return v;
+    if (a < 0x1p-6f) return rintf(v * 512.0f) * 0x1p-9f;  // subnormal quantum 2^-9
+    const int E = (int)((__float_as_uint(a) >> 23) & 0xff) - 127;  // floor(log2 a)
+    return ldexpf(rintf(ldexpf(v, 3 - E)), E - 3);                  // 3 fraction bits
+}
+
+__device__ __forceinline__ uint32_t e4m3_bits(float q) {
+    const uint32_t s = (__float_as_uint(q) >> 31) << 7;
+    const float a = fabsf(q);
+    if (a == 0.0f) return s;
+    if (a < 0x1p-6f) return s | (uint32_t)(a * 512.0f);
+    const uint32_t u = __float_as_uint(a);
+    const uint32_t ef = ((u >> 23) & 0xff) - 127 + 7;
+    return s | (ef << 3) | ((u >> 20) & 7u);
+}
+
+__device__ __forceinline__ int tile_exponent(float amax) {
+    if (!(amax >= 0x1p-100f)) return 0;  // all-zero or negligible tile: scale 1
+    const uint32_t u = __float_as_uint(amax);
+    const int E = (int)((u >> 23) & 0xff) - 126;  // amax = f * 2^E, f in [0.5, 1)
+    const float f = __uint_as_float((u & 0x807fffffu) | (126u << 23));
+    return (f * 512.0f > 448.0f) ? E - 8 : E - 9;
+}
+
+struct QuantArgs {
+    const float *X;
+    int rows, cols, ld;  // logical matrix as stored (A: M x K; B: K x N)
+    int Rp, Kp;          // padded output rows / K
+    uint8_t *X8;
+    uint16_t *Xp;
+    int *ex;
+};
+
+// One workgroup per 128 x 128 quantisation tile.  TRANS = false: rows of X are
+// output rows (A).  TRANS = true: X is K x N and output rows are its columns (B).
+template <bool TRANS>
+__global__ __launch_bounds__(256) void quantize_kernel(QuantArgs a) {
+    __shared__ float t[kQT][kQT + 1];
+    __shared__ float red[4];
+    const int tiles_k = a.Kp / kQT;
+    const int tr = blockIdx.x / tiles_k, tk = blockIdx.x % tiles_k;
+    const int r0 = tr * kQT, k0 = tk * kQT;
+    const int tid = threadIdx.x;
+    // stage the tile as t[output row][k]
+    for (int e = tid; e < kQT * kQT; e += 256) {
+        const int i = e / kQT, j = e % kQT;  // i: row of X, j: col of X (coalesced)
+        float v;
+        if (!TRANS) {
+            const int gr = r0 + i, gk = k0 + j;
+            v = (gr < a.rows && gk < a.cols) ? a.X[(int64_t)gr * a.ld + gk] : 0.0f;
+            t[i][j] = v;
+        } else {
+            const int gk = k0 + i, gn = r0 + j;
+            v = (gk < a.rows && gn < a.cols) ? a.X[(int64_t)gk * a.ld + gn] : 0.0f;
+            t[j][i] = v;
+        }
+    }
+    __syncthreads();
+    // amax over the tile
+    float m = 0.0f;
+    const int row = tid >> 1, half = tid & 1;
+    for (int j = 0; j < 64; ++j) m = fmaxf(m, fabsf(t[row][half * 64 + j]));
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    if ((tid & 63) == 0) red[tid >> 6] = m;
+    __syncthreads();
+    const float amax = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    const int e = tile_exponent(amax);
+    if (tid == 0) a.ex[tr * tiles_k + tk] = e;
+    // quantise this thread's 64 values: row `row`, k = k0 + half*64 .. +63
+    const int orow = r0 + row;
+    uint8_t *x8 = a.X8 + (int64_t)orow * a.Kp + k0 + half * 64;
+    uint16_t *xp = a.Xp + ((int64_t)orow * (a.Kp / 8) + (k0 + half * 64) / 8) * 16;
+#pragma unroll 1
+    for (int g = 0; g < 8; ++g) {
+        ushort8v qv, rv;
+        uint32_t b8[2] = {0, 0};
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const float x = t[row][half * 64 + g * 8 + j];
+            const float s = e4m3_round(ldexpf(x, -e));
+            const float q = ldexpf(s, e);
+            qv[j] = bf16_rne(q);  // exact: 4 significant bits
+            rv[j] = bf16_rne(x - q);
+            b8[j >> 2] |= e4m3_bits(s) << (8 * (j & 3));
+        }
+        *(ushort8v *)(xp + g * 16) = qv;
+        *(ushort8v *)(xp + g * 16 + 8) = rv;
+        *(uint2 *)(x8 + g * 8) = make_uint2(b8[0], b8[1]);
+    }
+}
+
+// GEMM: 256 x 128 output tile, 8 waves as 4 x 2 of 64 x 64, BK = 64 (one
+// FP8 MFMA k step and four BF16 k steps per stage), single LDS buffer with
+// the next stage prefetched into registers.
+constexpr int kBM = 256, kBN = 128, kBK = 64;
+constexpr int kWM = 4, kWN = 2, kThreads = kWM * kWN * 64;
+constexpr int kFM = 2, kFN = 2;
+constexpr int kRow8 = kBK + 16;          // bytes per LDS row of the e4m3 plane
+constexpr int kRowP = 2 * kBK + 8;       // bf16 per LDS row of the q|r plane
+constexpr int kA8 = 0;
+constexpr int kB8 = kA8 + kBM * kRow8;
+constexpr int kAP = kB8 + kBN * kRow8;   // byte offsets
+constexpr int kBP = kAP + kBM * kRowP * 2;
+constexpr int kLds = kBP + kBN * kRowP * 2;
+// 16-byte chunks per thread per stage
+constexpr int kCA8 = kBM * (kBK / 16) / kThreads;      // 2
+constexpr int kCB8 = kBN * (kBK / 16) / kThreads;      // 1
+constexpr int kCAP = kBM * (kBK * 4 / 16) / kThreads;  // 8
+constexpr int kCBP = kBN * (kBK * 4 / 16) / kThreads;  // 4
+
+struct GemmArgs {
+    const uint8_t *A8, *B8;
+    const uint16_t *Ap, *Bp;
+    const int *eA, *eB;
+    float *C;
+    int M, N, Kp, ldc;
+    int tiles_m, tiles_n;
+};
+
+template <bool EDGE>
+__global__ __launch_bounds__(kThreads) void gemm_fp8res_kernel(GemmArgs p) {
+    __shared__ __attribute__((aligned(16))) uint8_t smem[kLds];
+    const int bid = hbs::xcd_remap(blockIdx.x, gridDim.x);
+    int tm, tn;
+    hbs::tile_coords(bid, p.tiles_m, p.tiles_n, 8, tm, tn);
+    const int m0 = tm * kBM, n0 = tn * kBN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave % kWM, wn = wave / kWM;
+    const int l32 = lane & 31, h = lane >> 5;
+    const int ktiles = p.Kp / kQT;
+
+    floatx16 acc[kFM][kFN];
+#pragma unroll
+    for (int r = 0; r < kFM; ++r)
+#pragma unroll
+        for (int c = 0; c < kFN; ++c) acc[r][c] = floatx16{0};
+
+    floatx4 ra8[kCA8], rb8[kCB8], rap[kCAP], rbp[kCBP];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < kCA8; ++i) {
+            const int f = tid + i * kThreads, row = f / 4, c = f % 4;
+            ra8[i] = *(const floatx4 *)(p.A8 + (int64_t)(m0 + row) * p.Kp + k0 + c * 16);
+        }
+#pragma unroll
+        for (int i = 0; i < kCB8; ++i) {
+            const int f = tid + i * kThreads, row = f / 4, c = f % 4;
+            rb8[i] = *(const floatx4 *)(p.B8 + (int64_t)(n0 + row) * p.Kp + k0 + c * 16);
+        }
+#pragma unroll
+        for (int i = 0; i < kCAP; ++i) {
+            const int f = tid + i * kThreads, row = f / 16, c = f % 16;
+            rap[i] = *(const floatx4 *)(p.Ap + (int64_t)(m0 + row) * p.Kp * 2 + k0 * 2 + c * 8);
+        }
+#pragma unroll
+        for (int i = 0; i < kCBP; ++i) {
+            const int f = tid + i * kThreads, row = f / 16, c = f % 16;
+            rbp[i] = *(const floatx4 *)(p.Bp + (int64_t)(n0 + row) * p.Kp * 2 + k0 * 2 + c * 8);
+        }
+    };
+    auto sstore = [&]() {
+#pragma unroll
+        for (int i = 0; i < kCA8; ++i) {
+            const int f = tid + i * kThreads, row = f / 4, c = f % 4;
+            *(floatx4 *)(smem + kA8 + row * kRow8 + c * 16) = ra8[i];
+        }
+#pragma unroll
+        for (int i = 0; i < kCB8; ++i) {
+            const int f = tid + i * kThreads, row = f / 4, c = f % 4;
+            *(floatx4 *)(smem + kB8 + row * kRow8 + c * 16) = rb8[i];
+        }
+#pragma unroll
+        for (int i = 0; i < kCAP; ++i) {
+            const int f = tid + i * kThreads, row = f / 16, c = f % 16;
+            *(floatx4 *)(smem + kAP + (row * kRowP + c * 8) * 2) = rap[i];
+        }
+#pragma unroll
+        for (int i = 0; i < kCBP; ++i) {
+            const int f = tid + i * kThreads, row = f / 16, c = f % 16;
+            *(floatx4 *)(smem + kBP + (row * kRowP + c * 8) * 2) = rbp[i];
+        }
+    };
+
+    const int KT = p.Kp / kBK;
+    const int arow = wm * 64 + l32, brow = wn * 64 + l32;
+    gload(0);
+    for (int kt = 0; kt < KT; ++kt) {
+        sstore();
+        __syncthreads();
+        if (kt + 1 < KT) gload((kt + 1) * kBK);
+        const int kq = (kt * kBK) / kQT;
+        // E8M0 block scales (biased exponents) of this wave's A rows and B cols
+        const int sa = p.eA[((m0 + wm * 64) / kQT) * ktiles + kq] + 127;
+        const int sb = p.eB[((n0 + wn * 64) / kQT) * ktiles + kq] + 127;
+        // FP8: one 64-deep step; lane half h holds k = 32h .. 32h+31
+        {
+            intx8 a8[kFM], b8[kFN];
+#pragma unroll
+            for (int r = 0; r < kFM; ++r) {
+                const uint8_t *src = smem + kA8 + (arow + r * 32) * kRow8 + 32 * h;
+                const floatx4 lo = *(const floatx4 *)src, hi = *(const floatx4 *)(src + 16);
+                a8[r] = intx8{__float_as_int(lo[0]), __float_as_int(lo[1]), __float_as_int(lo[2]),
+                              __float_as_int(lo[3]), __float_as_int(hi[0]), __float_as_int(hi[1]),
+                              __float_as_int(hi[2]), __float_as_int(hi[3])};
+            }
+#pragma unroll
+            for (int c = 0; c < kFN; ++c) {
+                const uint8_t *src = smem + kB8 + (brow + c * 32) * kRow8 + 32 * h;
+                const floatx4 lo = *(const floatx4 *)src, hi = *(const floatx4 *)(src + 16);
+                b8[c] = intx8{__float_as_int(lo[0]), __float_as_int(lo[1]), __float_as_int(lo[2]),
+                              __float_as_int(lo[3]), __float_as_int(hi[0]), __float_as_int(hi[1]),
+                              __float_as_int(hi[2]), __float_as_int(hi[3])};
+            }
+#pragma unroll
+            for (int r = 0; r < kFM; ++r)
+#pragma unroll
+                for (int c = 0; c < kFN; ++c)
+                    acc[r][c] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
+                        a8[r], b8[c], acc[r][c], 0, 0, 0, sa, 0, sb);
+        }
+        // BF16 residual: Ar.Bq + Aq.Br, four 16-deep steps
+        const uint16_t *ap = (const uint16_t *)(smem + kAP) + arow * kRowP;
+        const uint16_t *bp = (const uint16_t *)(smem + kBP) + brow * kRowP;
+#pragma unroll
+        for (int st = 0; st < kBK / 16; ++st) {
+            const int off = (2 * st + h) * 16;
+            bf16x8 aq[kFM], ar[kFM], bq[kFN], br[kFN];
+#pragma unroll
+            for (int r = 0; r < kFM; ++r) {
+                aq[r] = *(const bf16x8 *)(ap + r * 32 * kRowP + off);
+                ar[r] = *(const bf16x8 *)(ap + r * 32 * kRowP + off + 8);
+            }
+#pragma unroll
+            for (int c = 0; c < kFN; ++c) {
+                bq[c] = *(const bf16x8 *)(bp + c * 32 * kRowP + off);
+                br[c] = *(const bf16x8 *)(bp + c * 32 * kRowP + off + 8);
+            }
+#pragma unroll
+            for (int r = 0; r < kFM; ++r)
+#pragma unroll
+                for (int c = 0; c < kFN; ++c) {
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ar[r], bq[c], acc[r][c], 0, 0, 0);
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aq[r], br[c], acc[r][c], 0, 0, 0);
+                }
+        }
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int r = 0; r < kFM; ++r) {
+#pragma unroll
+        for (int c = 0; c < kFN; ++c) {
+            const int col = n0 + wn * 64 + c * 32 + l32;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int row = m0 + wm * 64 + r * 32 + 8 * (q >> 2) + 4 * h + (q & 3);
+                if (!EDGE || (row < p.M && col < p.N)) p.C[(int64_t)row * p.ldc + col] = acc[r][c][q];
+            }
+        }
+    }
+}
+
+inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+struct Layout {
+    int Mp, Np, Kp;
+    size_t a8, ap, ea, b8, bp, eb, total;  // byte offsets
+};
+
+Layout layout(int M, int N, int K) {
+    Layout L;
+    L.Mp = (int)round_up(M, kBM);
+    L.Np = (int)round_up(N, kBN);
+    L.Kp = (int)round_up(K, kQT);
+    auto al = [](size_t x) { return (size_t)round_up((int64_t)x, 256); };
+    size_t o = 0;
+    L.a8 = o; o = al(o + (size_t)L.Mp * L.Kp);
+    L.ap = o; o = al(o + (size_t)L.Mp * L.Kp * 4);
+    L.ea = o; o = al(o + (size_t)(L.Mp / kQT) * (L.Kp / kQT) * 4);
+    L.b8 = o; o = al(o + (size_t)L.Np * L.Kp);
+    L.bp = o; o = al(o + (size_t)L.Np * L.Kp * 4);
+    L.eb = o; o = al(o + (size_t)(L.Np / kQT) * (L.Kp / kQT) * 4);
+    L.total = o;
+    return L;
+}
+
+}  // namespace
+
+extern "C" size_t hbs_gemm_fp8res_workspace_size(int M, int N, int K) {
+    if (M < 1 || N < 1 || K < 1) return 0;
+    return layout(M, N, K).total;
+}
+
+extern "C" int hbs_gemm_fp8res(const float *A, const float *B, float *C, int M, int N, int K,
+                               int lda, int ldb, int ldc, void *workspace,
+                               size_t workspace_bytes, void *stream) {
+    if (M < 1 || N < 1 || K < 1 || lda < K || ldb < N || ldc < N) return HBS_ERR_ARG;
+    if (!A || !B || !C) return HBS_ERR_NULL;
+    const Layout L = layout(M, N, K);
+    if (!workspace || workspace_bytes < L.total) return HBS_ERR_WORKSPACE;
+    if (!hbs::aligned16(workspace)) return HBS_ERR_ALIGN;
+    const int st = hbs::check_device();
+    if (st) return st;
+    hipStream_t s = (hipStream_t)stream;
+    char *ws = (char *)workspace;
+
+    QuantArgs qa{A, M, K, lda, L.Mp, L.Kp, (uint8_t *)(ws + L.a8), (uint16_t *)(ws + L.ap),
+                 (int *)(ws + L.ea)};
+    quantize_kernel<false><<<(L.Mp / kQT) * (L.Kp / kQT), 256, 0, s>>>(qa);
+    int rc = hbs::launch_status();
+    if (rc) return rc;
+    QuantArgs qb{B, K, N, ldb, L.Np, L.Kp, (uint8_t *)(ws + L.b8), (uint16_t *)(ws + L.bp),
+                 (int *)(ws + L.eb)};
+    quantize_kernel<true><<<(L.Np / kQT) * (L.Kp / kQT), 256, 0, s>>>(qb);
+    rc = hbs::launch_status();
+    if (rc) return rc;
+
+    GemmArgs g{(const uint8_t *)(ws + L.a8), (const uint8_t *)(ws + L.b8),
+               (const uint16_t *)(ws + L.ap), (const uint16_t *)(ws + L.bp),
+               (const int *)(ws + L.ea), (const int *)(ws + L.eb), C, M, N, L.Kp, ldc,
+               L.Mp / kBM, L.Np / kBN};
+    const int64_t blocks = (int64_t)g.tiles_m * g.tiles_n;
+    if (blocks > 0x7fffffffLL) return HBS_ERR_ARG;
+    if (M % kBM == 0 && N % kBN == 0)
+        gemm_fp8res_kernel<false><<<(unsigned)blocks, kThreads, 0, s>>>(g);
+    else
+        gemm_fp8res_kernel<true><<<(unsigned)blocks, kThreads, 0, s>>>(g);
+    return hbs::launch_status();
+}

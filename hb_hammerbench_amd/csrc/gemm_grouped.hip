@@ -528,8 +528,13 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
         int chunk = (q.K + c.split - 1) / c.split;
         chunk = (chunk + BK - 1) / BK * BK;
         const int ns = (q.K + chunk - 1) / chunk;
-        // relative cost model: bigger tiles run closer to the matrix-core peak
-        const double eff = (c.tile == 0) ? 1.0 : (c.tile == 1 ? 0.55 : 0.8);
+        // Work-item cost in ns of one workgroup slot: the autotuner's measured
+        // time when the tuning file has this class, else a throughput model
+        // (FP32: 157.3 TFLOP/s shared by 2 x CUs slots; BF16x3: 2516.6 / 3
+        // TFLOP/s-equivalent by 1 x CUs slots) with a tile-shape efficiency.
+        const double slots_all = (precision == HBS_PREC_FP32 ? 2.0 : 1.0) * cus;
+        const double peak = precision == HBS_PREC_FP32 ? 157.3e12 : 2516.6e12 / 3.0;
+        const double eff = (c.tile == 0) ? 0.7 : (c.tile == 1 ? 0.35 : 0.5);
         for (int tm = 0; tm < tm_n; ++tm) {
             for (int tn = 0; tn < tn_n; ++tn) {
                 ++tiles_total;
@@ -546,7 +551,9 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
                     it.slot = ns > 1 ? (int)slots : -1;
                     it.poff = ns > 1 ? poff : 0;
                     items.push_back(it);
-                    cost.push_back((double)BM * BN * (it.k1 - it.k0) / eff + 2.0e4);
+                    const double klen = it.k1 - it.k0;
+                    cost.push_back(c.ns > 0.0 ? c.ns * klen / chunk
+                                              : 2.0 * BM * BN * klen / (peak / slots_all * eff) * 1e9 + 2000.0);
                 }
                 if (ns > 1) {
                     ++slots;

@@ -283,6 +283,24 @@ int run_bf16x3(const float *A, const float *B, float *C, int M, int N, int K, in
 
 }  // namespace
 
+extern "C" size_t hbs_gemm_bf16x3_workspace_size(int M, int N, int K) {
+    if (M < 1 || N < 1 || K < 1) return 0;
+    return layout(M, N, K, 1).bytes_total;
+}
+
+extern "C" int hbs_gemm_bf16x3(const float *A, const float *B, float *C, int M, int N, int K,
+                               int lda, int ldb, int ldc, void *workspace, size_t workspace_bytes,
+                               void *stream) {
+    if (M < 1 || N < 1 || K < 1 || lda < K || ldb < N || ldc < N) return HBS_ERR_ARG;
+    if (!A || !B || !C) return HBS_ERR_NULL;
+    if (!workspace || workspace_bytes < layout(M, N, K, 1).bytes_total) return HBS_ERR_WORKSPACE;
+    if (!hbs::aligned16(workspace)) return HBS_ERR_ALIGN;
+    const int st = hbs::check_device();
+    if (st) return st;
+    return run_bf16x3(A, B, C, M, N, K, lda, ldb, ldc, 0, 0, 0, 1, workspace, workspace_bytes,
+                      (hipStream_t)stream);
+}
+
 extern "C" size_t hbs_sgemm_bf16x3_workspace_size(int N, int niter) {
     if (N < 1 || niter < 1) return 0;
     return layout(N, N, N, niter).bytes_total;

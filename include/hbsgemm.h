@@ -84,6 +84,11 @@ size_t hbs_sgemm_bf16x3_workspace_size(int N, int niter);
  * is overwritten; HBS_ERR_WORKSPACE if it is missing or too small. */
 int hbs_sgemm_bf16x3(const float *A, const float *B, float *C, int N, int niter,
                      void *workspace, size_t workspace_bytes, void *stream);
+/* Rectangular form: C[M x N] = A[M x K] . B[K x N], row-major with leading
+ * dimensions lda >= K, ldb >= N, ldc >= N; workspace as above. */
+size_t hbs_gemm_bf16x3_workspace_size(int M, int N, int K);
+int hbs_gemm_bf16x3(const float *A, const float *B, float *C, int M, int N, int K, int lda,
+                    int ldb, int ldc, void *workspace, size_t workspace_bytes, void *stream);
 
 
 /* ------------------------------------------------------------------------ */
@@ -140,6 +145,26 @@ int hbs_grouped_plan_destroy(hbs_grouped_plan *plan);
 int hbs_grouped_graph_create(const hbs_grouped_plan *plan, void **graph_exec);
 int hbs_graph_launch(void *graph_exec, void *stream);
 int hbs_graph_destroy(void *graph_exec);
+
+/* ------------------------------------------------------------------------ */
+/* FP8-e4m3 GEMM with per-tile scaling and a BF16 residual (no reference      */
+/* counterpart).  Per 128 x 128 tile of A and of B, with amax its max |x|:     */
+/*   e = power-of-two exponent with 224 < amax / 2^e <= 448,                  */
+/*   q = 2^e * e4m3_rne(x / 2^e),  r = bf16_rne(x - q);                       */
+/*   C = Aq.Bq (FP8 matrix cores, E8M0 tile scales) + Aq.Br + Ar.Bq (BF16),  */
+/* FP32 accumulation; the Ar.Br term is dropped.                              */
+/* ------------------------------------------------------------------------ */
+
+/* Workspace bytes for an M x N x K problem.  Layout (byte offsets aligned to
+ * 256; Mp = M rounded up to 256, Np = N rounded up to 128, Kp = K rounded up
+ * to 128): A8 [Mp][Kp] e4m3 | Ap [Mp][Kp/8][q8 r8] bf16 | eA [Mp/128][Kp/128]
+ * int32 | B8 [Np][Kp] | Bp [Np][Kp/8][q8 r8] | eB [Np/128][Kp/128] (B stored
+ * transposed: row n holds column n of B). */
+size_t hbs_gemm_fp8res_workspace_size(int M, int N, int K);
+/* C[M x N] = A[M x K] . B[K x N] (row-major, leading dimensions lda >= K,
+ * ldb >= N, ldc >= N) by the scheme above. */
+int hbs_gemm_fp8res(const float *A, const float *B, float *C, int M, int N, int K, int lda,
+                    int ldb, int ldc, void *workspace, size_t workspace_bytes, void *stream);
 
 /* ------------------------------------------------------------------------ */
 /* Input generation on the device.                                            */

@@ -313,7 +313,7 @@ float orc_fp8_e4m3_decode(uint8_t b)
 
 int orc_fp8_tile_exponent(float amax)
 {
-    if (!(amax > 0.0f)) return 0;
+    if (!(amax >= 0x1p-100f)) return 0; /* all-zero or negligible tile: scale 1 */
     int E;
     float f = frexpf(amax, &E); /* amax = f * 2^E, f in [0.5,1) */
     /* amax / 2^(E-9) = f * 512 in [256, 512) */

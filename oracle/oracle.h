@@ -78,7 +78,7 @@ float orc_fp8_e4m3_round(float x);
 uint8_t orc_fp8_e4m3_encode(float x);
 float orc_fp8_e4m3_decode(uint8_t b);
 /* Power-of-two tile scale exponent e (scale 2^e) with amax / 2^e <= 448 and
- * amax / 2^e > 224; 0 for an all-zero tile. */
+ * amax / 2^e > 224; 0 when amax < 2^-100 (all-zero or negligible tile). */
 int orc_fp8_tile_exponent(float amax);
 /* Quantise row-major rows x cols matrix X (leading dim ld) with per-tile
  * (tile x tile) power-of-two scales: Xq = 2^e * e4m3(X / 2^e), Xr = bf16(X - Xq).

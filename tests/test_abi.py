@@ -132,3 +132,19 @@ def test_grouped_plan_argument_checks():
     assert L.hbs_grouped_graph_create(None, ctypes.byref(h)) == -2
     assert L.hbs_graph_launch(None, None) == -2
     assert L.hbs_graph_destroy(None) == 0
+
+
+def test_fp8res_and_rect_bf16x3_argument_checks():
+    L = hb.lib()
+    assert L.hbs_gemm_fp8res_workspace_size(0, 8, 8) == 0
+    need = L.hbs_gemm_fp8res_workspace_size(16384, 16384, 4096)
+    assert need >= 2 * 16384 * 4096 * 5  # e4m3 byte + q|r bf16 per element, A and B
+    assert L.hbs_gemm_fp8res(P, P, P, 0, 8, 8, 8, 8, 8, P, 1 << 30, None) == -1
+    assert L.hbs_gemm_fp8res(P, P, P, 8, 8, 8, 7, 8, 8, P, 1 << 30, None) == -1
+    assert L.hbs_gemm_fp8res(None, P, P, 8, 8, 8, 8, 8, 8, P, 1 << 30, None) == -2
+    assert L.hbs_gemm_fp8res(P, P, P, 8, 8, 8, 8, 8, 8, None, 1 << 30, None) == -4
+    assert L.hbs_gemm_fp8res(P, P, P, 8, 8, 8, 8, 8, 8, P, 16, None) == -4
+    assert L.hbs_gemm_fp8res(P, P, P, 8, 8, 8, 8, 8, 8, ctypes.c_void_p(8), 1 << 30, None) == -3
+    assert L.hbs_gemm_bf16x3_workspace_size(8, 0, 8) == 0
+    assert L.hbs_gemm_bf16x3(P, P, P, 8, 8, 8, 8, 8, 7, P, 1 << 30, None) == -1
+    assert L.hbs_gemm_bf16x3(P, P, P, 8, 8, 8, 8, 8, 8, P, 16, None) == -4

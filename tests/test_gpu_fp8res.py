@@ -1,0 +1,149 @@
+"""GPU parity of the FP8-e4m3 + BF16-residual GEMM, through the C-ABI.
+
+No reference function exists for this kernel.  It is pinned by
+ * the quantiser: the device's e4m3 bytes, q|r bf16 planes and tile exponents
+   equal the oracle's (oracle/orc_fp8_quantize, itself pinned against torch's
+   float8_e4m3fn and bfloat16 conversions) bit for bit;
+ * the product: C within the FP32 accumulation bound of the exact value of
+   the algorithm, Aq.Bq + Aq.Br + Ar.Bq, computed in FP64 from the oracle's
+   quantised operands: |C - I| <= gamma_{3K+2} (|Aq||Bq| + |Aq||Br| + |Ar||Bq|);
+ * bit-exactness on the reference's integer pattern (power-of-two tile scales
+   keep 0..6 and 0..2 exact in e4m3, so the residual is zero);
+ * the error against the true product: |C - AB| <= 2^-8 |Aq||dB| +
+   2^-8 |dA||Bq| + |dA||dB| + (the bound above), dX = X - Xq.
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+torch = pytest.importorskip("torch")
+import hb_hammerbench_amd as hb  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+U = 2.0 ** -24
+
+
+def gamma(n):
+    return n * U / (1 - n * U)
+
+
+def rup(x, m):
+    return (x + m - 1) // m * m
+
+
+def planes(ws, M, N, K):
+    """Splits the documented workspace layout into its planes (numpy)."""
+    Mp, Np, Kp = rup(M, 256), rup(N, 128), rup(K, 128)
+    w = ws.cpu().numpy()
+    o = 0
+
+    def take(nbytes):
+        nonlocal o
+        v = w[o:o + nbytes]
+        o = rup(o + nbytes, 256)
+        return v
+    A8 = take(Mp * Kp).reshape(Mp, Kp)
+    Ap = take(Mp * Kp * 4).view(np.uint16).reshape(Mp, Kp // 8, 2, 8)
+    eA = take((Mp // 128) * (Kp // 128) * 4).view(np.int32).reshape(Mp // 128, Kp // 128)
+    B8 = take(Np * Kp).reshape(Np, Kp)
+    Bp = take(Np * Kp * 4).view(np.uint16).reshape(Np, Kp // 8, 2, 8)
+    eB = take((Np // 128) * (Kp // 128) * 4).view(np.int32).reshape(Np // 128, Kp // 128)
+    return A8, Ap, eA, B8, Bp, eB
+
+
+def run(M, N, K, kind="normal_outlier", seed=3):
+    a = torch.empty(M * K, dtype=torch.float32, device="cuda")
+    b = torch.empty(K * N, dtype=torch.float32, device="cuda")
+    if kind == "pattern":
+        n = max(M * K, K * N)
+        ta = torch.empty(n, device="cuda"); tb = torch.empty(n, device="cuda")
+        hb.fill_pattern(ta, tb, n)
+        a.copy_(ta[:M * K]); b.copy_(tb[:K * N])
+    else:
+        k = {"normal_outlier": hb.GEN_NORMAL_OUTLIER, "uniform": hb.GEN_UNIFORM}[kind]
+        hb.fill_random(a, M * K, k, seed)
+        hb.fill_random(b, K * N, k, seed + 1)
+    c = torch.empty(M * N, dtype=torch.float32, device="cuda")
+    ws = hb.gemm_fp8res(a, b, c, M, N, K)
+    torch.cuda.synchronize()
+    return (a.cpu().numpy().reshape(M, K), b.cpu().numpy().reshape(K, N),
+            c.cpu().numpy().reshape(M, N), ws)
+
+
+def test_fp8res_pattern_bit_exact():
+    M, N, K = 512, 256, 384
+    A, B, C, _ = run(M, N, K, "pattern")
+    expect = A.astype(np.int64) @ B.astype(np.int64)
+    assert np.array_equal(C, expect.astype(np.float32))
+
+
+@pytest.mark.parametrize("M,N,K", [(256, 128, 256), (300, 200, 100), (512, 384, 640)])
+def test_fp8res_quantiser_matches_oracle(M, N, K):
+    A, B, C, ws = run(M, N, K)
+    A8, Ap, eA, B8, Bp, eB = planes(ws, M, N, K)
+    Aq, Ar, ea = oracle.fp8_quantize(A)
+    Bq, Br, eb = oracle.fp8_quantize(np.ascontiguousarray(B.T))
+    assert np.array_equal(eA[:ea.shape[0], :ea.shape[1]], ea)
+    assert np.array_equal(eB[:eb.shape[0], :eb.shape[1]], eb)
+    # q|r planes (bf16 bits) against the oracle's float results
+    q = oracle.bf16_bits_to_f32(Ap[:, :, 0, :].reshape(Ap.shape[0], -1))[:M, :K]
+    r = oracle.bf16_bits_to_f32(Ap[:, :, 1, :].reshape(Ap.shape[0], -1))[:M, :K]
+    assert np.array_equal(q, Aq) and np.array_equal(r, Ar)
+    qb = oracle.bf16_bits_to_f32(Bp[:, :, 0, :].reshape(Bp.shape[0], -1))[:N, :K]
+    rb = oracle.bf16_bits_to_f32(Bp[:, :, 1, :].reshape(Bp.shape[0], -1))[:N, :K]
+    assert np.array_equal(qb, Bq) and np.array_equal(rb, Br)
+    # e4m3 bytes decode to q / 2^e
+    for i in range(0, M, 37):
+        for k in range(0, K, 29):
+            s = oracle.fp8_decode(int(A8[i, k]))
+            assert np.float32(np.ldexp(s, int(ea[i // 128, k // 128]))) == Aq[i, k]
+    # padding is zero
+    assert not A8[M:].any() and not A8[:, K:].any()
+
+
+@pytest.mark.parametrize("M,N,K", [(256, 128, 256), (300, 200, 100), (512, 384, 1024)])
+def test_fp8res_matches_algorithm_within_fp32_bound(M, N, K):
+    A, B, C, _ = run(M, N, K, seed=M)
+    Aq, Ar, _ = oracle.fp8_quantize(A)
+    BqT, BrT, _ = oracle.fp8_quantize(np.ascontiguousarray(B.T))
+    Bq, Br = BqT.T.astype(np.float64), BrT.T.astype(np.float64)
+    Aq, Ar = Aq.astype(np.float64), Ar.astype(np.float64)
+    ideal = Aq @ Bq + Aq @ Br + Ar @ Bq
+    absum = np.abs(Aq) @ np.abs(Bq) + np.abs(Aq) @ np.abs(Br) + np.abs(Ar) @ np.abs(Bq)
+    assert np.all(np.abs(C - ideal) <= gamma(3 * K + 2) * absum)
+    dA = A.astype(np.float64) - Aq
+    dB = B.astype(np.float64) - Bq
+    R = A.astype(np.float64) @ B.astype(np.float64)
+    algo = (2.0 ** -8 * (np.abs(Aq) @ np.abs(dB)) + 2.0 ** -8 * (np.abs(dA) @ np.abs(Bq))
+            + np.abs(dA) @ np.abs(dB))
+    assert np.all(np.abs(C - R) <= algo + gamma(3 * K + 2) * absum)
+
+
+def test_fp8res_full_size_outliers_error_report():
+    """M = N = 16384, K = 4096, normal(0,1) with 1% x1e3 outliers: FP8+residual
+    error against FP64 on a row band, beside BF16x3 and FP32 on the same inputs."""
+    M = N = 16384
+    K = 4096
+    a = torch.empty(M * K, device="cuda"); b = torch.empty(K * N, device="cuda")
+    hb.fill_random(a, M * K, hb.GEN_NORMAL_OUTLIER, 5)
+    hb.fill_random(b, K * N, hb.GEN_NORMAL_OUTLIER, 6)
+    c8 = torch.empty(M * N, device="cuda")
+    ws = hb.gemm_fp8res(a, b, c8, M, N, K)
+    del ws
+    c3 = torch.empty(M * N, device="cuda")
+    hb.gemm_bf16x3(a, b, c3, M, N, K)
+    c32 = torch.empty(M * N, device="cuda")
+    hb.sgemm_strided(a, b, c32, M, N, K, K, N, N)
+    torch.cuda.synchronize()
+    rows = dict(row_begin=8192, row_end=8192 + 256)
+    e8 = hb.verify_f64(a, b, c8, M, N, K, **rows)
+    e3 = hb.verify_f64(a, b, c3, M, N, K, **rows)
+    e32 = hb.verify_f64(a, b, c32, M, N, K, **rows)
+    print(f"\nC5 rows 8192..8447: e_cw fp8res={e8['e_cw']:.3e} bf16x3={e3['e_cw']:.3e} "
+          f"fp32={e32['e_cw']:.3e}; e_F fp8res={e8['e_F']:.3e} bf16x3={e3['e_F']:.3e} "
+          f"fp32={e32['e_F']:.3e}", flush=True)
+    assert e8["nonfinite"] == 0
+    # BF16-level: within the unit roundoff of bf16 inputs (2^-8) normwise
+    assert e8["e_F"] <= 2.0 ** -8 and e8["e_nw"] <= 2.0 ** -8
+    assert e3["e_cw"] <= 2.0 * e32["e_cw"] + 1e-7
