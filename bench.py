@@ -83,7 +83,7 @@ def cpu_baseline(N, target_s):
     t0 = time.perf_counter()
     lib.orc_host_mm_rows(C, A, B, N, 1, 1 + rows)
     dt = time.perf_counter() - t0
-    return {
+    single = {
         "value": 2.0 * rows * N * N / dt / 1e12,
         "unit": "TFLOP/s",
         "cores": 1,
@@ -91,6 +91,16 @@ def cpu_baseline(N, target_s):
         "sample": f"host_mm rows 1..{rows} of one {N}x{N} uniform slice "
                   f"({rows}/{N} of a slice, {dt:.1f} s single thread, -O2 -ffp-contract=off)",
     }
+    # the same golden row-parallel over the host's cores (bit-identical per row)
+    threads = max(1, min(os.cpu_count() or 1, 16))
+    rows_mt = min(N - 1 - rows, max(threads, int(rows * threads * 0.5)))
+    if rows_mt > 0:
+        t0 = time.perf_counter()
+        lib.orc_host_mm_omp(C, A, B, N, 1 + rows, 1 + rows + rows_mt, threads)
+        dt_mt = time.perf_counter() - t0
+        single["all_cores"] = {"value": 2.0 * rows_mt * N * N / dt_mt / 1e12, "cores": threads,
+                               "sample": f"{rows_mt} rows, {dt_mt:.1f} s, OpenMP row-parallel"}
+    return single
 
 
 def setup_square(args, hb, torch):

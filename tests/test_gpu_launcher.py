@@ -1,0 +1,44 @@
+"""End-to-end runs of the C++ launcher (hb_hammerbench_amd/apps/sgemm/sgemm),
+the counterpart of running the reference's N_512__NITER_2 test: it initialises,
+computes the host golden, launches, reads back and applies the SSE rule
+(apps/sgemm/main.cpp:30-149), ending in a regression verdict line."""
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+APP = os.path.join(ROOT, "hb_hammerbench_amd", "apps", "sgemm")
+LAUNCHER = os.path.join(APP, "sgemm")
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("args", [
+    ["--N", "512", "--NITER", "2"],                                   # N_512__NITER_2
+    ["--N", "512", "--NITER", "2", "--input", "uniform"],
+    ["--N", "512", "--NITER", "2", "--precision", "bf16x3"],
+    ["--N", "300", "--NITER", "3", "--input", "loguniform", "--precision", "bf16x3"],
+    ["--N", "2048", "--NITER", "2", "--input", "uniform", "--golden", "f64", "--reps", "3"],
+    ["--N", "4096", "--NITER", "1", "--golden", "f64"],               # pattern via FP64 check
+])
+def test_launcher_passes(args):
+    assert os.path.exists(LAUNCHER), "launcher not built"
+    r = subprocess.run([LAUNCHER] + args, capture_output=True, text=True, timeout=240)
+    print(r.stdout[-600:])
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert r.stdout.strip().splitlines()[-1] == "HBS REGRESSION TEST PASSED"
+
+
+def test_make_generate_and_exec_reference_test(tmp_path):
+    """make generate + make N_512__NITER_2.exec, as in the reference workflow."""
+    for f in ["Makefile", "tests.mk", "template.mk"]:
+        with open(os.path.join(APP, f)) as src:
+            (tmp_path / f).write_text(src.read())
+    (tmp_path / "app_path.mk").write_text(f"APP_PATH := {APP}\n")
+    subprocess.check_call(["make", "-s", "-C", str(tmp_path), "generate"], stdout=subprocess.DEVNULL)
+    r = subprocess.run(["make", "-s", "-C", str(tmp_path), "N_512__NITER_2.exec"],
+                       capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    log = (tmp_path / "N_512__NITER_2" / "exec.log").read_text()
+    assert "N=512" in log and "NITER=2" in log and "HBS REGRESSION TEST PASSED" in log
