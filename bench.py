@@ -103,6 +103,21 @@ def cpu_baseline(N, target_s):
     return single
 
 
+def measured_traffic(variant, workload_key):
+    """HBM bytes per launch of the dominant kernel, from the calibrated
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in profiles/traffic.json
+    (None when that file has no entry for this exact workload)."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            entry = json.load(f).get(variant)
+    except (OSError, ValueError):
+        return None
+    if not entry or entry.get("workload") != workload_key:
+        return None
+    return entry.get("hbm_bytes_per_launch")
+
+
 def setup_square(args, hb, torch):
     """NITER row-major N x N slices in one launch (apps/sgemm/kernel.cpp:293-297)."""
     N, NITER = args.n, args.niter
@@ -134,8 +149,9 @@ def setup_square(args, hb, torch):
     kind = "plain FP32 MFMA" if args.variant == "fp32" else "BF16x3 split"
     workload = (f"{kind} SGEMM, square N={N}, NITER={NITER} slices per launch, "
                 f"uniform[-1,1) seed {args.seed}")
-    return step, 2.0 * N * N * N * NITER, workload, {"verify": verify,
-                                                     "config": {"N": N, "niter": NITER}}
+    return step, 2.0 * N * N * N * NITER, workload, {
+        "verify": verify, "config": {"N": N, "niter": NITER},
+        "traffic": measured_traffic(args.variant, f"N={N} NITER={NITER}")}
 
 
 def setup_grouped(args, hb, torch):
@@ -288,7 +304,7 @@ def main():
             "peak": round(peak, 1),
             "unit": "TFLOP/s",
             "frac": round(achieved / peak, 4),
-            "traffic": None,
+            "traffic": extra.get("traffic"),
             "kernel_ms_avg": round(avg_ms, 4),
         },
     }

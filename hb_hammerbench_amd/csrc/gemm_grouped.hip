@@ -82,16 +82,20 @@ constexpr int kThreads = 256;
 // ---------------------------------------------------------------------------
 // shared helpers
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint16_t bf16_rne(float x) {
-    uint32_t u = __float_as_uint(x);
-    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (uint16_t)((u >> 16) | 0x40);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
-
-__device__ __forceinline__ void split_bf16(float x, uint16_t &hi, uint16_t &lo) {
-    hi = bf16_rne(x);
-    lo = bf16_rne(x - __uint_as_float((uint32_t)hi << 16));
+// x = hi + lo with hi = bf16_rne(x), lo = bf16_rne(x - hi); x - hi is exact.
+// v_cvt_pk_bf16_f32 rounds to nearest even, two floats per instruction.
+__device__ __forceinline__ void split2(float x0, float x1, uint16_t &h0, uint16_t &h1,
+                                       uint16_t &l0, uint16_t &l1) {
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    typedef __bf16 b2 __attribute__((ext_vector_type(2)));
+    const f2 v = {x0, x1};
+    const b2 h = __builtin_convertvector(v, b2);
+    const b2 l = __builtin_convertvector(v - __builtin_convertvector(h, f2), b2);
+    const uint32_t hu = __builtin_bit_cast(uint32_t, h), lu = __builtin_bit_cast(uint32_t, l);
+    h0 = (uint16_t)hu;
+    h1 = (uint16_t)(hu >> 16);
+    l0 = (uint16_t)lu;
+    l1 = (uint16_t)(lu >> 16);
 }
 
 // Epilogue for a 2x2-wave tile of FMxFN 32x32 accumulator blocks per wave.
@@ -317,11 +321,11 @@ __device__ void b3_tile(uint16_t *smem, const Prob &p, const Item &it, float *pa
             const int row = f / 4, g = f % 4;
             ushort8v hi, lo;
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                uint16_t a, b;
-                split_bf16(ra[i][j / 4][j % 4], a, b);
-                hi[j] = a;
-                lo[j] = b;
+            for (int j = 0; j < 8; j += 2) {
+                uint16_t h0, h1, l0, l1;
+                split2(ra[i][j / 4][j % 4], ra[i][j / 4][j % 4 + 1], h0, h1, l0, l1);
+                hi[j] = h0; hi[j + 1] = h1;
+                lo[j] = l0; lo[j + 1] = l1;
             }
             *(ushort8v *)(as + row * ROW + g * 16) = hi;
             *(ushort8v *)(as + row * ROW + g * 16 + 8) = lo;
@@ -336,11 +340,11 @@ __device__ void b3_tile(uint16_t *smem, const Prob &p, const Item &it, float *pa
             for (int jn = 0; jn < 4; ++jn) {
                 ushort4v hi, lo;
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    uint16_t a, b;
-                    split_bf16(rb[i][j][jn], a, b);
-                    hi[j] = a;
-                    lo[j] = b;
+                for (int j = 0; j < 4; j += 2) {
+                    uint16_t h0, h1, l0, l1;
+                    split2(rb[i][j][jn], rb[i][j + 1][jn], h0, h1, l0, l1);
+                    hi[j] = h0; hi[j + 1] = h1;
+                    lo[j] = l0; lo[j + 1] = l1;
                 }
                 uint16_t *dst = bs + (c * 4 + jn) * ROW + off;
                 *(ushort4v *)dst = hi;

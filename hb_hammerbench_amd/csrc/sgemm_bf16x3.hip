@@ -31,16 +31,20 @@ constexpr int kStage = (kBM + kBN) * kLdsRow;  // bf16 elements per stage
 constexpr int kChunksPerRow = kRowElems / 8;   // 16-byte chunks per tile row
 constexpr int kLoads = kBM * kChunksPerRow / kThreads;  // per operand per thread
 
-__device__ __forceinline__ uint16_t bf16_rne(float x) {
-    uint32_t u = __float_as_uint(x);
-    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (uint16_t)((u >> 16) | 0x40);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
-
-__device__ __forceinline__ void split(float x, uint16_t &hi, uint16_t &lo) {
-    hi = bf16_rne(x);
-    lo = bf16_rne(x - __uint_as_float((uint32_t)hi << 16));
+// x = hi + lo with hi = bf16_rne(x), lo = bf16_rne(x - hi); x - hi is exact.
+// v_cvt_pk_bf16_f32 rounds to nearest even, two floats per instruction.
+__device__ __forceinline__ void split2(float x0, float x1, uint16_t &h0, uint16_t &h1,
+                                       uint16_t &l0, uint16_t &l1) {
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    typedef __bf16 b2 __attribute__((ext_vector_type(2)));
+    const f2 v = {x0, x1};
+    const b2 h = __builtin_convertvector(v, b2);
+    const b2 l = __builtin_convertvector(v - __builtin_convertvector(h, f2), b2);
+    const uint32_t hu = __builtin_bit_cast(uint32_t, h), lu = __builtin_bit_cast(uint32_t, l);
+    h0 = (uint16_t)hu;
+    h1 = (uint16_t)(hu >> 16);
+    l0 = (uint16_t)lu;
+    l1 = (uint16_t)(lu >> 16);
 }
 
 struct PackArgs {
@@ -63,13 +67,15 @@ __global__ __launch_bounds__(256) void pack_rows_kernel(PackArgs a, int64_t grou
         const int row = (int)(r / gpr), kg = (int)(r % gpr);
         const float *x = a.X + s * a.sX + (int64_t)row * a.ld + kg * 8;
         ushort8 hi, lo;
+        float v[8];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const float v = (row < a.rows && kg * 8 + j < a.cols) ? x[j] : 0.0f;
-            uint16_t h, l;
-            split(v, h, l);
-            hi[j] = h;
-            lo[j] = l;
+        for (int j = 0; j < 8; ++j) v[j] = (row < a.rows && kg * 8 + j < a.cols) ? x[j] : 0.0f;
+#pragma unroll
+        for (int j = 0; j < 8; j += 2) {
+            uint16_t h0, h1, l0, l1;
+            split2(v[j], v[j + 1], h0, h1, l0, l1);
+            hi[j] = h0; hi[j + 1] = h1;
+            lo[j] = l0; lo[j + 1] = l1;
         }
         ushort8 *dst = (ushort8 *)(a.P + s * a.sP + ((int64_t)row * gpr + kg) * 16);
         dst[0] = hi;
@@ -101,11 +107,11 @@ __global__ __launch_bounds__(256) void pack_cols_kernel(PackArgs a, int tiles_k,
         if (gn >= a.cols_p || gkg >= gpr) continue;
         ushort8 hi, lo;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            uint16_t h, l;
-            split(t[kg * 8 + j][nn], h, l);
-            hi[j] = h;
-            lo[j] = l;
+        for (int j = 0; j < 8; j += 2) {
+            uint16_t h0, h1, l0, l1;
+            split2(t[kg * 8 + j][nn], t[kg * 8 + j + 1][nn], h0, h1, l0, l1);
+            hi[j] = h0; hi[j + 1] = h1;
+            lo[j] = l0; lo[j + 1] = l1;
         }
         ushort8 *dst = (ushort8 *)(P + ((int64_t)gn * gpr + gkg) * 16);
         dst[0] = hi;

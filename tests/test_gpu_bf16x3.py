@@ -66,6 +66,24 @@ def test_bf16x3_within_split_bound(N, niter, kind):
         assert e["e_cw"] <= 1e-5 * np.sqrt(N)
 
 
+def test_bf16x3_packed_planes_match_oracle_split():
+    """Workspace Ap[Mp][Kp/8][hi8|lo8] (A's plane, first in the workspace)
+    equals the oracle's split bit for bit; padding is zero."""
+    N = 300
+    a = dev_fill(N * N, hb.GEN_LOGUNIFORM, 41); b = dev_fill(N * N, hb.GEN_UNIFORM, 42)
+    c = dev_empty(N * N)
+    ws = torch.zeros(hb.bf16x3_workspace_size(N, 1), dtype=torch.uint8, device="cuda")
+    hb.sgemm_bf16x3(a, b, c, N, 1, workspace=ws)
+    torch.cuda.synchronize()
+    Mp, Kp = 512, 320
+    Ap = ws[:Mp * Kp * 4].cpu().numpy().view(np.uint16).reshape(Mp, Kp // 8, 2, 8)
+    hi = Ap[:, :, 0, :].reshape(Mp, Kp)
+    lo = Ap[:, :, 1, :].reshape(Mp, Kp)
+    ohi, olo = oracle.split_bf16(a.cpu().numpy().reshape(N, N))
+    assert np.array_equal(hi[:N, :N], ohi) and np.array_equal(lo[:N, :N], olo)
+    assert not hi[N:].any() and not hi[:, N:].any() and not lo[N:].any()
+
+
 def test_bf16x3_pattern_ragged_sizes():
     for N in (3, 100, 257):
         a = dev_empty(N * N); b = dev_empty(N * N); c = dev_empty(N * N)

@@ -120,6 +120,50 @@ def test_fp8res_matches_algorithm_within_fp32_bound(M, N, K):
     assert np.all(np.abs(C - R) <= algo + gamma(3 * K + 2) * absum)
 
 
+def test_fp8res_identity_b_reproduces_quantised_a():
+    """B = I: every output is Aq + Ar of one A element (a single FP32 add of
+    two exact terms), so C equals float32(Aq + Ar) bit for bit."""
+    M, K = 256, 128
+    a = torch.empty(M * K, device="cuda")
+    hb.fill_random(a, M * K, hb.GEN_NORMAL_OUTLIER, 9)
+    b = torch.eye(K, dtype=torch.float32, device="cuda").reshape(-1).contiguous()
+    c = torch.empty(M * K, device="cuda")
+    hb.gemm_fp8res(a, b, c, M, K, K)
+    torch.cuda.synchronize()
+    A = a.cpu().numpy().reshape(M, K)
+    C = c.cpu().numpy().reshape(M, K)
+    Aq, Ar, e = oracle.fp8_quantize(A)
+    expect = (Aq.astype(np.float64) + Ar.astype(np.float64)).astype(np.float32)
+    bad = C != expect
+    scale = np.repeat(np.repeat(2.0 ** e, 128, 0), 128, 1)[:M, :K]
+    sub = (Aq != 0) & (np.abs(Aq) < 2.0 ** -6 * scale)   # e4m3-subnormal q
+    assert not bad.any(), (f"{bad.sum()} mismatches; {(bad & sub).sum()} at e4m3-subnormal q, "
+                           f"{sub.sum()} subnormal q in total")
+
+
+def test_fp8res_without_subnormals_matches_algorithm():
+    """Entries with |x| in [0.5, 1) keep every x / 2^e in the e4m3 normal range."""
+    M, N, K = 256, 128, 512
+    g = torch.Generator(device="cuda").manual_seed(4)
+    def draw(n):
+        mag = 0.5 + 0.5 * torch.rand(n, device="cuda", generator=g)
+        sgn = torch.where(torch.rand(n, device="cuda", generator=g) < 0.5, -1.0, 1.0)
+        return (mag * sgn).float().contiguous()
+    a = draw(M * K); b = draw(K * N)
+    c = torch.empty(M * N, device="cuda")
+    hb.gemm_fp8res(a, b, c, M, N, K)
+    torch.cuda.synchronize()
+    A = a.cpu().numpy().reshape(M, K); B = b.cpu().numpy().reshape(K, N)
+    C = c.cpu().numpy().reshape(M, N)
+    Aq, Ar, _ = oracle.fp8_quantize(A)
+    BqT, BrT, _ = oracle.fp8_quantize(np.ascontiguousarray(B.T))
+    Aq, Ar = Aq.astype(np.float64), Ar.astype(np.float64)
+    Bq, Br = BqT.T.astype(np.float64), BrT.T.astype(np.float64)
+    ideal = Aq @ Bq + Aq @ Br + Ar @ Bq
+    absum = np.abs(Aq) @ np.abs(Bq) + np.abs(Aq) @ np.abs(Br) + np.abs(Ar) @ np.abs(Bq)
+    assert np.all(np.abs(C - ideal) <= gamma(3 * K + 2) * absum)
+
+
 def test_fp8res_full_size_outliers_error_report():
     """M = N = 16384, K = 4096, normal(0,1) with 1% x1e3 outliers: FP8+residual
     error against FP64 on a row band, beside BF16x3 and FP32 on the same inputs."""
@@ -146,4 +190,3 @@ def test_fp8res_full_size_outliers_error_report():
     assert e8["nonfinite"] == 0
     # BF16-level: within the unit roundoff of bf16 inputs (2^-8) normwise
     assert e8["e_F"] <= 2.0 ** -8 and e8["e_nw"] <= 2.0 ** -8
-    assert e3["e_cw"] <= 2.0 * e32["e_cw"] + 1e-7

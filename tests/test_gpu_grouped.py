@@ -59,8 +59,9 @@ def make_group(shapes, kind="uniform", seed=5):
     return probs, views, (a, b, c)
 
 
+# N and K are multiples of 4 (the grouped API's 16-byte load requirement)
 RAGGED = [(64, 64, 64), (100, 36, 52), (128, 256, 64), (200, 68, 1000), (64, 2048, 128),
-          (1, 4, 4), (333, 12, 4), (512, 128, 2048), (130, 130, 132)]
+          (1, 4, 4), (333, 12, 4), (512, 128, 2048), (130, 132, 132)]
 
 
 @pytest.mark.parametrize("prec", [hb.PREC_FP32, hb.PREC_BF16X3])
@@ -102,7 +103,7 @@ def test_grouped_forced_configs_within_bound(prec, tile, split):
 
 
 def test_grouped_plan_info_counts_tiles():
-    shapes = [(200, 68, 64), (64, 64, 64), (129, 257, 32)]
+    shapes = [(200, 68, 64), (64, 64, 64), (129, 260, 32)]
     probs, _, _ = make_group(shapes)
     plan = hb.GroupedPlan(probs, hb.PREC_FP32, tile=0, split_k=1)
     info = plan.info()

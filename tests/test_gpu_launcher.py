@@ -18,7 +18,10 @@ pytestmark = pytest.mark.gpu
     ["--N", "512", "--NITER", "2"],                                   # N_512__NITER_2
     ["--N", "512", "--NITER", "2", "--input", "uniform"],
     ["--N", "512", "--NITER", "2", "--precision", "bf16x3"],
-    ["--N", "300", "--NITER", "3", "--input", "loguniform", "--precision", "bf16x3"],
+    # log-uniform values reach 1e6: the absolute SSE rule only fits the pattern,
+    # so these inputs are checked against FP64 (componentwise gate)
+    ["--N", "300", "--NITER", "3", "--input", "loguniform", "--precision", "bf16x3",
+     "--golden", "f64"],
     ["--N", "2048", "--NITER", "2", "--input", "uniform", "--golden", "f64", "--reps", "3"],
     ["--N", "4096", "--NITER", "1", "--golden", "f64"],               # pattern via FP64 check
 ])
