@@ -6,7 +6,8 @@
 // candidate (tile shape x split-K) per size class on the device and writes the
 // tuning file hbs_grouped_plan_create reads:
 //   <fp32|bf16x3> <M class> <N class> <K class> <tile id> <split> <ns per work item>
-// A size class is the next power of two >= the size, clamped to [64, 2048].
+// A size class is the power of two nearest the size in log scale (sizes in
+// [c / sqrt2, c * sqrt2) map to c), clamped to [64, 2048].
 // Each probe runs a group of identical problems large enough to fill the GPU,
 // timed with HIP events over --reps launches after one warm-up launch.
 #include <hip/hip_runtime.h>

@@ -432,9 +432,11 @@ __global__ __launch_bounds__(kThreads, PREC == HBS_PREC_FP32 ? 2 : 1) void group
 // ---------------------------------------------------------------------------
 // host side: tuning table, plan construction
 // ---------------------------------------------------------------------------
+// Size class of a dimension: the power of two nearest to it in log scale,
+// clamped to [64, 2048] (the autotuner measures at these sizes).
 int size_class(int x) {
     int c = 64;
-    while (c < x && c < 2048) c *= 2;
+    while (c < 2048 && (int64_t)x * x >= (int64_t)2 * c * c) c *= 2;  // x >= c * sqrt(2)
     return c;
 }
 

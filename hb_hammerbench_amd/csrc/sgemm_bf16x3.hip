@@ -54,6 +54,7 @@ struct PackArgs {
     int ld;              // leading dimension of X
     int rows_p, cols_p;  // padded packed extents (rows_p x cols_p, cols_p % 8 == 0)
     int64_t sX, sP;      // slice strides (elements)
+    int vec;             // X, ld and sX allow 16-byte loads
 };
 
 // A [M][K] -> Ap[Mp][Kp/8][hi8 lo8]; one thread per 8-element group.
@@ -68,8 +69,17 @@ __global__ __launch_bounds__(256) void pack_rows_kernel(PackArgs a, int64_t grou
         const float *x = a.X + s * a.sX + (int64_t)row * a.ld + kg * 8;
         ushort8 hi, lo;
         float v[8];
+        if (a.vec && row < a.rows && kg * 8 + 8 <= a.cols) {
+            const floatx4 p0 = *(const floatx4 *)x, p1 = *(const floatx4 *)(x + 4);
 #pragma unroll
-        for (int j = 0; j < 8; ++j) v[j] = (row < a.rows && kg * 8 + j < a.cols) ? x[j] : 0.0f;
+            for (int j = 0; j < 4; ++j) {
+                v[j] = p0[j];
+                v[j + 4] = p1[j];
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = (row < a.rows && kg * 8 + j < a.cols) ? x[j] : 0.0f;
+        }
 #pragma unroll
         for (int j = 0; j < 8; j += 2) {
             uint16_t h0, h1, l0, l1;
@@ -260,7 +270,8 @@ int run_bf16x3(const float *A, const float *B, float *C, int M, int N, int K, in
     uint16_t *Ap = (uint16_t *)ws;
     uint16_t *Bp = (uint16_t *)((char *)ws + L.bytes_a);
 
-    PackArgs pa{A, Ap, M, K, lda, L.Mp, L.Kp, sA, L.sAp};
+    const int vec_a = hbs::aligned16(A) && lda % 4 == 0 && sA % 4 == 0;
+    PackArgs pa{A, Ap, M, K, lda, L.Mp, L.Kp, sA, L.sAp, vec_a};
     const int64_t gps = (int64_t)L.Mp * (L.Kp / 8);
     const int64_t total = gps * batch;
     int64_t blocks = (total + 255) / 256;
@@ -269,7 +280,7 @@ int run_bf16x3(const float *A, const float *B, float *C, int M, int N, int K, in
     int st = hbs::launch_status();
     if (st) return st;
 
-    PackArgs pb{B, Bp, K, N, ldb, L.Kp, L.Np, sB, L.sBp};
+    PackArgs pb{B, Bp, K, N, ldb, L.Kp, L.Np, sB, L.sBp, 0};
     const int tk = L.Kp / 64 + (L.Kp % 64 ? 1 : 0), tnb = L.Np / 64;
     const int64_t pblocks = (int64_t)tk * tnb * batch;
     if (pblocks > 0x7fffffffLL) return HBS_ERR_ARG;

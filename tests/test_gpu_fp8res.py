@@ -6,7 +6,13 @@ No reference function exists for this kernel.  It is pinned by
    float8_e4m3fn and bfloat16 conversions) bit for bit;
  * the product: C within the FP32 accumulation bound of the exact value of
    the algorithm, Aq.Bq + Aq.Br + Ar.Bq, computed in FP64 from the oracle's
-   quantised operands: |C - I| <= gamma_{3K+2} (|Aq||Bq| + |Aq||Br| + |Ar||Bq|);
+   quantised operands: |C - I| <= gamma_{3K+2} (|Aq||Bq| + |Aq||Br| + |Ar||Bq|)
+   plus, for the FP8 matrix-core term, 32 * 2^-13 * max|aq bq| per 32-element k
+   block.  The FP8 MFMA sums each 32-element block with about 14 significant
+   bits relative to the block's largest product, truncating (measured on
+   gfx950: 448 + 2^-6 in one block returns 448, in two blocks 448.015625);
+   inputs whose magnitudes stay within one binade of each other meet the plain
+   FP32 bound, which test_fp8res_without_subnormals_matches_algorithm checks;
  * bit-exactness on the reference's integer pattern (power-of-two tile scales
    keep 0..6 and 0..2 exact in e4m3, so the residual is zero);
  * the error against the true product: |C - AB| <= 2^-8 |Aq||dB| +
@@ -50,6 +56,17 @@ def planes(ws, M, N, K):
     Bp = take(Np * Kp * 4).view(np.uint16).reshape(Np, Kp // 8, 2, 8)
     eB = take((Np // 128) * (Kp // 128) * 4).view(np.int32).reshape(Np // 128, Kp // 128)
     return A8, Ap, eA, B8, Bp, eB
+
+
+def fp8_block_term(Aq, Bq):
+    """sum over 32-element k blocks of 32 * 2^-13 * max_k |Aq[i,k] Bq[k,j]|."""
+    M, K = Aq.shape
+    out = np.zeros((M, Bq.shape[1]))
+    for k0 in range(0, K, 32):
+        a = np.abs(Aq[:, k0:k0 + 32])[:, :, None]
+        b = np.abs(Bq[k0:k0 + 32, :])[None, :, :]
+        out += (a * b).max(axis=1)
+    return 32 * 2.0 ** -13 * out
 
 
 def run(M, N, K, kind="normal_outlier", seed=3):
@@ -111,13 +128,17 @@ def test_fp8res_matches_algorithm_within_fp32_bound(M, N, K):
     Aq, Ar = Aq.astype(np.float64), Ar.astype(np.float64)
     ideal = Aq @ Bq + Aq @ Br + Ar @ Bq
     absum = np.abs(Aq) @ np.abs(Bq) + np.abs(Aq) @ np.abs(Br) + np.abs(Ar) @ np.abs(Bq)
-    assert np.all(np.abs(C - ideal) <= gamma(3 * K + 2) * absum)
+    hw = fp8_block_term(Aq, Bq)
+    err = np.abs(C - ideal)
+    assert np.all(err <= gamma(3 * K + 2) * absum + hw)
+    print(f"\n{M}x{N}x{K}: max |C - I| / (gamma absum + hw) = "
+          f"{(err / (gamma(3 * K + 2) * absum + hw)).max():.3f}", flush=True)
     dA = A.astype(np.float64) - Aq
     dB = B.astype(np.float64) - Bq
     R = A.astype(np.float64) @ B.astype(np.float64)
     algo = (2.0 ** -8 * (np.abs(Aq) @ np.abs(dB)) + 2.0 ** -8 * (np.abs(dA) @ np.abs(Bq))
             + np.abs(dA) @ np.abs(dB))
-    assert np.all(np.abs(C - R) <= algo + gamma(3 * K + 2) * absum)
+    assert np.all(np.abs(C - R) <= algo + gamma(3 * K + 2) * absum + hw)
 
 
 def test_fp8res_identity_b_reproduces_quantised_a():
