@@ -83,7 +83,7 @@ int main(int argc, char **argv) {
     const int maxc = *std::max_element(classes.begin(), classes.end());
     const size_t elems = (size_t)maxc * maxc;
     // Enough C space for the largest group probed (see `count` below).
-    const size_t c_elems = (size_t)4 * cus * 64 * 64 + elems;
+    const size_t c_elems = (size_t)8 * cus * 128 * 128 + elems;
     float *A, *B, *C;
     HIP_CHECK(hipMalloc(&A, elems * sizeof(float)));
     HIP_CHECK(hipMalloc(&B, elems * sizeof(float)));
@@ -109,9 +109,10 @@ int main(int argc, char **argv) {
         for (int mc : classes) {
             for (int nc : classes) {
                 for (int kc : classes) {
-                    // identical problems until ~4 x CUs 64x64 tiles are in flight
-                    const int64_t t64 = (int64_t)(mc / 64) * (nc / 64);
-                    const int count = (int)std::max<int64_t>(1, std::min<int64_t>(4 * cus / t64 + 1, 1024));
+                    // identical problems until the largest tile shape (128 x 128) has
+                    // >= 8 work items per CU, so every candidate runs on a full GPU
+                    const int64_t t128 = (int64_t)((mc + 127) / 128) * ((nc + 127) / 128);
+                    const int count = (int)std::max<int64_t>(1, std::min<int64_t>((8 * cus + t128 - 1) / t128, 4096));
                     std::vector<hbs_gemm_problem> probs(count);
                     for (int i = 0; i < count; ++i) {
                         float *Ci = C + (size_t)i * mc * nc;

@@ -11,8 +11,8 @@
 //  1. pack: A -> Ap[Mp][Kp/8][hi8|lo8] and B -> Bp[Np][Kp/8][hi8|lo8] (B is
 //     transposed so both operands are K-contiguous); rows and K are zero-padded
 //     to the tile multiples, so the GEMM needs no load bounds checks;
-//  2. GEMM: 256x256 output tile per workgroup, BK = 32, global -> registers ->
-//     LDS double buffer of the packed rows, 8 waves each owning 128x64.
+//  2. GEMM (B3Main): 256x256 output tile per workgroup, BK = 32, global ->
+//     registers -> LDS double buffer of the packed rows, 8 waves each 128x64.
 #include "hbs_internal.h"
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -20,16 +20,29 @@ typedef unsigned short ushort8 __attribute__((ext_vector_type(8)));
 
 namespace {
 
-constexpr int kBM = 256, kBN = 256, kBK = 32;
-constexpr int kWM = 2, kWN = 4;
-constexpr int kThreads = kWM * kWN * 64;
-constexpr int kWTM = kBM / kWM, kWTN = kBN / kWN;
-constexpr int kFM = kWTM / 32, kFN = kWTN / 32;
-constexpr int kRowElems = kBK * 2;        // packed bf16 per tile row (hi + lo)
-constexpr int kLdsRow = kRowElems + 8;    // + 16 B pad
-constexpr int kStage = (kBM + kBN) * kLdsRow;  // bf16 elements per stage
-constexpr int kChunksPerRow = kRowElems / 8;   // 16-byte chunks per tile row
-constexpr int kLoads = kBM * kChunksPerRow / kThreads;  // per operand per thread
+// Packed operands are padded to these multiples so that every GEMM tile
+// configuration below can read them without bounds checks.
+constexpr int kRowPad = 256, kKPad = 64;
+
+template <int BM_, int BN_, int BK_, int WM_, int WN_>
+struct B3Cfg {
+    static constexpr int BM = BM_, BN = BN_, BK = BK_, WM = WM_, WN = WN_;
+    static constexpr int kThreads = WM * WN * 64;
+    static constexpr int kWTM = BM / WM, kWTN = BN / WN;
+    static constexpr int kFM = kWTM / 32, kFN = kWTN / 32;
+    static constexpr int kRowElems = BK * 2;              // packed bf16 per tile row (hi + lo)
+    static constexpr int kLdsRow = kRowElems + 8;         // + 16 B pad
+    static constexpr int kStage = (BM + BN) * kLdsRow;    // bf16 elements per stage
+    static constexpr int kChunksPerRow = kRowElems / 8;   // 16-byte chunks per tile row
+    static constexpr int kLoadsA = BM * kChunksPerRow / kThreads;
+    static constexpr int kLoadsB = BN * kChunksPerRow / kThreads;
+    static_assert(kWTM % 32 == 0 && kWTN % 32 == 0, "wave tile");
+    static_assert(BM * kChunksPerRow % kThreads == 0 && BN * kChunksPerRow % kThreads == 0, "loads");
+    static_assert(kRowPad % BM == 0 && kRowPad % BN == 0 && kKPad % BK == 0, "padding");
+    static_assert(2 * kStage * 2 <= 160 * 1024, "LDS");
+};
+// Production configuration: 256 x 256 tile, BK = 32, 8 waves of 128 x 64.
+using B3Main = B3Cfg<256, 256, 32, 2, 4>;
 
 // x = hi + lo with hi = bf16_rne(x), lo = bf16_rne(x - hi); x - hi is exact.
 // v_cvt_pk_bf16_f32 rounds to nearest even, two floats per instruction.
@@ -138,8 +151,11 @@ struct GemmArgs {
     int tiles_m, tiles_n, group_m;
 };
 
-template <bool EDGE>
-__global__ __launch_bounds__(kThreads) void gemm_bf16x3_kernel(GemmArgs p) {
+template <class G, bool EDGE>
+__global__ __launch_bounds__(G::kThreads) void gemm_bf16x3_kernel(GemmArgs p) {
+    constexpr int kBM = G::BM, kBN = G::BN, kBK = G::BK, kWM = G::WM, kThreads = G::kThreads;
+    constexpr int kWTM = G::kWTM, kWTN = G::kWTN, kFM = G::kFM, kFN = G::kFN;
+    constexpr int kLdsRow = G::kLdsRow, kStage = G::kStage, kChunksPerRow = G::kChunksPerRow;
     __shared__ __attribute__((aligned(16))) uint16_t smem[2 * kStage];
 
     const int tiles = p.tiles_m * p.tiles_n;
@@ -164,26 +180,35 @@ __global__ __launch_bounds__(kThreads) void gemm_bf16x3_kernel(GemmArgs p) {
 #pragma unroll
         for (int c = 0; c < kFN; ++c) acc[r][c] = floatx16{0};
 
-    floatx4 ra[kLoads], rb[kLoads];
+    floatx4 ra[G::kLoadsA], rb[G::kLoadsB];
 
     auto gload = [&](int k0) {
 #pragma unroll
-        for (int i = 0; i < kLoads; ++i) {
+        for (int i = 0; i < G::kLoadsA; ++i) {
             const int f = tid + i * kThreads;
             const int row = f / kChunksPerRow, c = f % kChunksPerRow;
-            const int64_t off = (int64_t)row * row_elems + (int64_t)k0 * 2 + c * 8;
-            ra[i] = *(const floatx4 *)(Ag + off);
-            rb[i] = *(const floatx4 *)(Bg + off);
+            ra[i] = *(const floatx4 *)(Ag + (int64_t)row * row_elems + (int64_t)k0 * 2 + c * 8);
+        }
+#pragma unroll
+        for (int i = 0; i < G::kLoadsB; ++i) {
+            const int f = tid + i * kThreads;
+            const int row = f / kChunksPerRow, c = f % kChunksPerRow;
+            rb[i] = *(const floatx4 *)(Bg + (int64_t)row * row_elems + (int64_t)k0 * 2 + c * 8);
         }
     };
     auto sstore = [&](int buf) {
         uint16_t *as = smem + buf * kStage;
         uint16_t *bsm = as + kBM * kLdsRow;
 #pragma unroll
-        for (int i = 0; i < kLoads; ++i) {
+        for (int i = 0; i < G::kLoadsA; ++i) {
             const int f = tid + i * kThreads;
             const int row = f / kChunksPerRow, c = f % kChunksPerRow;
             *(floatx4 *)(as + row * kLdsRow + c * 8) = ra[i];
+        }
+#pragma unroll
+        for (int i = 0; i < G::kLoadsB; ++i) {
+            const int f = tid + i * kThreads;
+            const int row = f / kChunksPerRow, c = f % kChunksPerRow;
             *(floatx4 *)(bsm + row * kLdsRow + c * 8) = rb[i];
         }
     };
@@ -251,14 +276,27 @@ struct Layout {
 
 Layout layout(int M, int N, int K, int batch) {
     Layout L;
-    L.Mp = (int)round_up(M, kBM);
-    L.Np = (int)round_up(N, kBN);
-    L.Kp = (int)round_up(K, kBK);
+    L.Mp = (int)round_up(M, kRowPad);
+    L.Np = (int)round_up(N, kRowPad);
+    L.Kp = (int)round_up(K, kKPad);
     L.sAp = (int64_t)L.Mp * L.Kp * 2;
     L.sBp = (int64_t)L.Np * L.Kp * 2;
     L.bytes_a = (size_t)round_up(L.sAp * batch * 2, 256);
     L.bytes_total = L.bytes_a + (size_t)L.sBp * batch * 2;
     return L;
+}
+
+template <class G>
+int launch_gemm(const uint16_t *Ap, const uint16_t *Bp, float *C, int M, int N, const Layout &L,
+                int ldc, int64_t sC, int batch, int group_m, hipStream_t stream) {
+    GemmArgs g{Ap, Bp, C, M, N, L.Kp, ldc, L.sAp, L.sBp, sC, L.Mp / G::BM, L.Np / G::BN, group_m};
+    const int64_t gblocks = (int64_t)g.tiles_m * g.tiles_n * batch;
+    if (gblocks > 0x7fffffffLL) return HBS_ERR_ARG;
+    if (M % G::BM == 0 && N % G::BN == 0)
+        gemm_bf16x3_kernel<G, false><<<(unsigned)gblocks, G::kThreads, 0, stream>>>(g);
+    else
+        gemm_bf16x3_kernel<G, true><<<(unsigned)gblocks, G::kThreads, 0, stream>>>(g);
+    return hbs::launch_status();
 }
 
 int run_bf16x3(const float *A, const float *B, float *C, int M, int N, int K, int lda, int ldb,
@@ -288,14 +326,7 @@ int run_bf16x3(const float *A, const float *B, float *C, int M, int N, int K, in
     st = hbs::launch_status();
     if (st) return st;
 
-    GemmArgs g{Ap, Bp, C, M, N, L.Kp, ldc, L.sAp, L.sBp, sC, L.Mp / kBM, L.Np / kBN, 8};
-    const int64_t gblocks = (int64_t)g.tiles_m * g.tiles_n * batch;
-    if (gblocks > 0x7fffffffLL) return HBS_ERR_ARG;
-    if (M % kBM == 0 && N % kBN == 0)
-        gemm_bf16x3_kernel<false><<<(unsigned)gblocks, kThreads, 0, stream>>>(g);
-    else
-        gemm_bf16x3_kernel<true><<<(unsigned)gblocks, kThreads, 0, stream>>>(g);
-    return hbs::launch_status();
+    return launch_gemm<B3Main>(Ap, Bp, C, M, N, L, ldc, sC, batch, 8, stream);
 }
 
 }  // namespace
