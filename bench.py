@@ -32,7 +32,9 @@ def parse():
     p.add_argument("--variant", choices=["fp32", "bf16x3", "grouped-fp32", "grouped-bf16x3",
                                          "fp8res"], default="fp32")
     p.add_argument("--problems", type=int, default=512, help="grouped: number of problems")
-    p.add_argument("--tuning", default=None, help="grouped: autotuner tuning file")
+    p.add_argument("--tuning", default=os.path.join(ROOT, "hb_hammerbench_amd", "tuning",
+                                                    "mi355x_grouped.txt"),
+                   help="grouped: autotuner tuning file ('' for the size heuristic)")
     p.add_argument("--n", type=int, default=8192)
     p.add_argument("--niter", type=int, default=10)
     p.add_argument("--seed", type=int, default=20241)
@@ -177,7 +179,8 @@ def setup_grouped(args, hb, torch):
         views.append((A, B, C, M, N, K))
         oa += M * K; ob += K * N; oc += M * N
     prec = hb.PREC_FP32 if args.variant == "grouped-fp32" else hb.PREC_BF16X3
-    plan = hb.GroupedPlan(probs, prec, tuning_file=args.tuning)
+    tuning = args.tuning if args.tuning and os.path.exists(args.tuning) else None
+    plan = hb.GroupedPlan(probs, prec, tuning_file=tuning)
     graph = plan.graph()
     torch.cuda.synchronize()
     flops = float(sum(2 * M * N * K for M, N, K in shapes))
@@ -202,7 +205,7 @@ def setup_grouped(args, hb, torch):
                 f"(numpy default_rng({args.seed})), one persistent hipGraph launch")
     cfg = {"problems": args.problems, "plan": info,
            "bytes_algorithmic": float(sum(4 * (M * K + K * N + M * N) for M, N, K in shapes)),
-           "tuning": args.tuning}
+           "tuning": os.path.relpath(tuning, ROOT) if tuning else None}
     return step, flops, workload, {"verify": verify, "config": cfg, "keep": (plan, graph)}
 
 

@@ -15,8 +15,11 @@
 //
 // Launches: quantise A, quantise B (transposed, K-contiguous), GEMM.  The
 // workspace holds, per operand X (rows R = M or N, padded to Rp; Kp = K padded
-// to 128):  X8 [Rp][Kp] e4m3 bytes, Xp [Rp][Kp/8][q8 | r8] bf16 groups, and the
-// tile exponents eX [Rp/128][Kp/128] int32 (for B indexed [n tile][k tile]).
+// to 128):  X8 [Rp][Kp] e4m3 bytes, Xr [Rp][Kp] bf16 residuals, and the tile
+// exponents eX [Rp/128][Kp/128] int32 (for B indexed [n tile][k tile]) -- three
+// bytes per element.  The bf16 copy of q that the residual products need is
+// rebuilt from the e4m3 bytes in registers (v_cvt_scalef32_pk_bf16_fp8, exact:
+// q has four significant bits and the scale is a power of two).
 #include "hbs_internal.h"
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -66,7 +69,7 @@ struct QuantArgs {
     int rows, cols, ld;  // logical matrix as stored (A: M x K; B: K x N)
     int Rp, Kp;          // padded output rows / K
     uint8_t *X8;
-    uint16_t *Xp;
+    uint16_t *Xr;
     int *ex;
 };
 
@@ -83,15 +86,12 @@ __global__ __launch_bounds__(256) void quantize_kernel(QuantArgs a) {
     // stage the tile as t[output row][k]
     for (int e = tid; e < kQT * kQT; e += 256) {
         const int i = e / kQT, j = e % kQT;  // i: row of X, j: col of X (coalesced)
-        float v;
         if (!TRANS) {
             const int gr = r0 + i, gk = k0 + j;
-            v = (gr < a.rows && gk < a.cols) ? a.X[(int64_t)gr * a.ld + gk] : 0.0f;
-            t[i][j] = v;
+            t[i][j] = (gr < a.rows && gk < a.cols) ? a.X[(int64_t)gr * a.ld + gk] : 0.0f;
         } else {
             const int gk = k0 + i, gn = r0 + j;
-            v = (gk < a.rows && gn < a.cols) ? a.X[(int64_t)gk * a.ld + gn] : 0.0f;
-            t[j][i] = v;
+            t[j][i] = (gk < a.rows && gn < a.cols) ? a.X[(int64_t)gk * a.ld + gn] : 0.0f;
         }
     }
     __syncthreads();
@@ -106,59 +106,71 @@ __global__ __launch_bounds__(256) void quantize_kernel(QuantArgs a) {
     const int e = tile_exponent(amax);
     if (tid == 0) a.ex[tr * tiles_k + tk] = e;
     // quantise this thread's 64 values: row `row`, k = k0 + half*64 .. +63
-    const int orow = r0 + row;
-    uint8_t *x8 = a.X8 + (int64_t)orow * a.Kp + k0 + half * 64;
-    uint16_t *xp = a.Xp + ((int64_t)orow * (a.Kp / 8) + (k0 + half * 64) / 8) * 16;
+    const int64_t base = (int64_t)(r0 + row) * a.Kp + k0 + half * 64;
 #pragma unroll 1
     for (int g = 0; g < 8; ++g) {
-        ushort8v qv, rv;
+        ushort8v rv;
         uint32_t b8[2] = {0, 0};
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const float x = t[row][half * 64 + g * 8 + j];
             const float s = e4m3_round(ldexpf(x, -e));
             const float q = ldexpf(s, e);
-            qv[j] = bf16_rne(q);  // exact: 4 significant bits
             rv[j] = bf16_rne(x - q);
             b8[j >> 2] |= e4m3_bits(s) << (8 * (j & 3));
         }
-        *(ushort8v *)(xp + g * 16) = qv;
-        *(ushort8v *)(xp + g * 16 + 8) = rv;
-        *(uint2 *)(x8 + g * 8) = make_uint2(b8[0], b8[1]);
+        *(ushort8v *)(a.Xr + base + g * 8) = rv;
+        *(uint2 *)(a.X8 + base + g * 8) = make_uint2(b8[0], b8[1]);
     }
 }
 
-// GEMM: 256 x 128 output tile, 8 waves as 4 x 2 of 64 x 64, BK = 64 (one
-// FP8 MFMA k step and four BF16 k steps per stage), single LDS buffer with
-// the next stage prefetched into registers.
+// GEMM: 256 x 128 output tile, 8 waves as 4 x 2 of 64 x 64, BK = 64 (one FP8
+// MFMA step and four BF16 steps per stage), LDS double buffer.  LDS rows are
+// unpadded; 16-byte chunks are XOR-swizzled by row so that the 16 lanes of a
+// 128-bit read touch 16 distinct bank groups.
 constexpr int kBM = 256, kBN = 128, kBK = 64;
 constexpr int kWM = 4, kWN = 2, kThreads = kWM * kWN * 64;
 constexpr int kFM = 2, kFN = 2;
-constexpr int kRow8 = kBK + 16;          // bytes per LDS row of the e4m3 plane
-constexpr int kRowP = 2 * kBK + 8;       // bf16 per LDS row of the q|r plane
+constexpr int kRow8 = kBK;          // bytes per LDS row, e4m3 plane (4 chunks)
+constexpr int kRowR = kBK * 2;      // bytes per LDS row, residual plane (8 chunks)
 constexpr int kA8 = 0;
 constexpr int kB8 = kA8 + kBM * kRow8;
-constexpr int kAP = kB8 + kBN * kRow8;   // byte offsets
-constexpr int kBP = kAP + kBM * kRowP * 2;
-constexpr int kLds = kBP + kBN * kRowP * 2;
+constexpr int kAR = kB8 + kBN * kRow8;
+constexpr int kBR = kAR + kBM * kRowR;
+constexpr int kStage = kBR + kBN * kRowR;  // bytes per stage
 // 16-byte chunks per thread per stage
-constexpr int kCA8 = kBM * (kBK / 16) / kThreads;      // 2
-constexpr int kCB8 = kBN * (kBK / 16) / kThreads;      // 1
-constexpr int kCAP = kBM * (kBK * 4 / 16) / kThreads;  // 8
-constexpr int kCBP = kBN * (kBK * 4 / 16) / kThreads;  // 4
+constexpr int kCA8 = kBM * 4 / kThreads;   // 2
+constexpr int kCB8 = kBN * 4 / kThreads;   // 1
+constexpr int kCAR = kBM * 8 / kThreads;   // 4
+constexpr int kCBR = kBN * 8 / kThreads;   // 2
+static_assert(2 * kStage <= 160 * 1024, "LDS");
+
+// swizzled byte offset of 16-byte chunk c of row r
+__device__ __forceinline__ int sw8(int r, int c) { return r * kRow8 + ((c ^ ((r >> 2) & 3)) << 4); }
+__device__ __forceinline__ int swr(int r, int c) { return r * kRowR + ((c ^ ((r >> 1) & 7)) << 4); }
 
 struct GemmArgs {
     const uint8_t *A8, *B8;
-    const uint16_t *Ap, *Bp;
+    const uint16_t *Ar, *Br;
     const int *eA, *eB;
     float *C;
     int M, N, Kp, ldc;
     int tiles_m, tiles_n;
 };
 
+// 8 e4m3 bytes (two dwords) -> 8 bf16 values times 2^e
+__device__ __forceinline__ bf16x8 fp8x8_to_bf16(uint32_t w0, uint32_t w1, float scale) {
+    typedef __bf16 b2 __attribute__((ext_vector_type(2)));
+    const b2 p0 = __builtin_amdgcn_cvt_scalef32_pk_bf16_fp8(w0, scale, false);
+    const b2 p1 = __builtin_amdgcn_cvt_scalef32_pk_bf16_fp8(w0, scale, true);
+    const b2 p2 = __builtin_amdgcn_cvt_scalef32_pk_bf16_fp8(w1, scale, false);
+    const b2 p3 = __builtin_amdgcn_cvt_scalef32_pk_bf16_fp8(w1, scale, true);
+    return bf16x8{p0[0], p0[1], p1[0], p1[1], p2[0], p2[1], p3[0], p3[1]};
+}
+
 template <bool EDGE>
 __global__ __launch_bounds__(kThreads) void gemm_fp8res_kernel(GemmArgs p) {
-    __shared__ __attribute__((aligned(16))) uint8_t smem[kLds];
+    __shared__ __attribute__((aligned(16))) uint8_t smem[2 * kStage];
     const int bid = hbs::xcd_remap(blockIdx.x, gridDim.x);
     int tm, tn;
     hbs::tile_coords(bid, p.tiles_m, p.tiles_n, 8, tm, tn);
@@ -174,78 +186,76 @@ __global__ __launch_bounds__(kThreads) void gemm_fp8res_kernel(GemmArgs p) {
 #pragma unroll
         for (int c = 0; c < kFN; ++c) acc[r][c] = floatx16{0};
 
-    floatx4 ra8[kCA8], rb8[kCB8], rap[kCAP], rbp[kCBP];
+    floatx4 ra8[kCA8], rb8[kCB8], rar[kCAR], rbr[kCBR];
     auto gload = [&](int k0) {
 #pragma unroll
         for (int i = 0; i < kCA8; ++i) {
-            const int f = tid + i * kThreads, row = f / 4, c = f % 4;
+            const int f = tid + i * kThreads, row = f >> 2, c = f & 3;
             ra8[i] = *(const floatx4 *)(p.A8 + (int64_t)(m0 + row) * p.Kp + k0 + c * 16);
         }
 #pragma unroll
         for (int i = 0; i < kCB8; ++i) {
-            const int f = tid + i * kThreads, row = f / 4, c = f % 4;
+            const int f = tid + i * kThreads, row = f >> 2, c = f & 3;
             rb8[i] = *(const floatx4 *)(p.B8 + (int64_t)(n0 + row) * p.Kp + k0 + c * 16);
         }
 #pragma unroll
-        for (int i = 0; i < kCAP; ++i) {
-            const int f = tid + i * kThreads, row = f / 16, c = f % 16;
-            rap[i] = *(const floatx4 *)(p.Ap + (int64_t)(m0 + row) * p.Kp * 2 + k0 * 2 + c * 8);
+        for (int i = 0; i < kCAR; ++i) {
+            const int f = tid + i * kThreads, row = f >> 3, c = f & 7;
+            rar[i] = *(const floatx4 *)(p.Ar + (int64_t)(m0 + row) * p.Kp + k0 + c * 8);
         }
 #pragma unroll
-        for (int i = 0; i < kCBP; ++i) {
-            const int f = tid + i * kThreads, row = f / 16, c = f % 16;
-            rbp[i] = *(const floatx4 *)(p.Bp + (int64_t)(n0 + row) * p.Kp * 2 + k0 * 2 + c * 8);
+        for (int i = 0; i < kCBR; ++i) {
+            const int f = tid + i * kThreads, row = f >> 3, c = f & 7;
+            rbr[i] = *(const floatx4 *)(p.Br + (int64_t)(n0 + row) * p.Kp + k0 + c * 8);
         }
     };
-    auto sstore = [&]() {
+    auto sstore = [&](int buf) {
+        uint8_t *st = smem + buf * kStage;
 #pragma unroll
         for (int i = 0; i < kCA8; ++i) {
-            const int f = tid + i * kThreads, row = f / 4, c = f % 4;
-            *(floatx4 *)(smem + kA8 + row * kRow8 + c * 16) = ra8[i];
+            const int f = tid + i * kThreads, row = f >> 2, c = f & 3;
+            *(floatx4 *)(st + kA8 + sw8(row, c)) = ra8[i];
         }
 #pragma unroll
         for (int i = 0; i < kCB8; ++i) {
-            const int f = tid + i * kThreads, row = f / 4, c = f % 4;
-            *(floatx4 *)(smem + kB8 + row * kRow8 + c * 16) = rb8[i];
+            const int f = tid + i * kThreads, row = f >> 2, c = f & 3;
+            *(floatx4 *)(st + kB8 + sw8(row, c)) = rb8[i];
         }
 #pragma unroll
-        for (int i = 0; i < kCAP; ++i) {
-            const int f = tid + i * kThreads, row = f / 16, c = f % 16;
-            *(floatx4 *)(smem + kAP + (row * kRowP + c * 8) * 2) = rap[i];
+        for (int i = 0; i < kCAR; ++i) {
+            const int f = tid + i * kThreads, row = f >> 3, c = f & 7;
+            *(floatx4 *)(st + kAR + swr(row, c)) = rar[i];
         }
 #pragma unroll
-        for (int i = 0; i < kCBP; ++i) {
-            const int f = tid + i * kThreads, row = f / 16, c = f % 16;
-            *(floatx4 *)(smem + kBP + (row * kRowP + c * 8) * 2) = rbp[i];
+        for (int i = 0; i < kCBR; ++i) {
+            const int f = tid + i * kThreads, row = f >> 3, c = f & 7;
+            *(floatx4 *)(st + kBR + swr(row, c)) = rbr[i];
         }
     };
 
-    const int KT = p.Kp / kBK;
     const int arow = wm * 64 + l32, brow = wn * 64 + l32;
-    gload(0);
-    for (int kt = 0; kt < KT; ++kt) {
-        sstore();
-        __syncthreads();
-        if (kt + 1 < KT) gload((kt + 1) * kBK);
+    auto compute = [&](int buf, int kt) {
+        const uint8_t *st = smem + buf * kStage;
         const int kq = (kt * kBK) / kQT;
-        // E8M0 block scales (biased exponents) of this wave's A rows and B cols
-        const int sa = p.eA[((m0 + wm * 64) / kQT) * ktiles + kq] + 127;
-        const int sb = p.eB[((n0 + wn * 64) / kQT) * ktiles + kq] + 127;
-        // FP8: one 64-deep step; lane half h holds k = 32h .. 32h+31
+        const int ea = p.eA[((m0 + wm * 64) / kQT) * ktiles + kq];
+        const int eb = p.eB[((n0 + wn * 64) / kQT) * ktiles + kq];
+        // FP8: lane half h holds k = 32h .. 32h + 31 (chunks 2h, 2h + 1)
         {
             intx8 a8[kFM], b8[kFN];
 #pragma unroll
             for (int r = 0; r < kFM; ++r) {
-                const uint8_t *src = smem + kA8 + (arow + r * 32) * kRow8 + 32 * h;
-                const floatx4 lo = *(const floatx4 *)src, hi = *(const floatx4 *)(src + 16);
+                const int row = arow + r * 32;
+                const floatx4 lo = *(const floatx4 *)(st + kA8 + sw8(row, 2 * h));
+                const floatx4 hi = *(const floatx4 *)(st + kA8 + sw8(row, 2 * h + 1));
                 a8[r] = intx8{__float_as_int(lo[0]), __float_as_int(lo[1]), __float_as_int(lo[2]),
                               __float_as_int(lo[3]), __float_as_int(hi[0]), __float_as_int(hi[1]),
                               __float_as_int(hi[2]), __float_as_int(hi[3])};
             }
 #pragma unroll
             for (int c = 0; c < kFN; ++c) {
-                const uint8_t *src = smem + kB8 + (brow + c * 32) * kRow8 + 32 * h;
-                const floatx4 lo = *(const floatx4 *)src, hi = *(const floatx4 *)(src + 16);
+                const int row = brow + c * 32;
+                const floatx4 lo = *(const floatx4 *)(st + kB8 + sw8(row, 2 * h));
+                const floatx4 hi = *(const floatx4 *)(st + kB8 + sw8(row, 2 * h + 1));
                 b8[c] = intx8{__float_as_int(lo[0]), __float_as_int(lo[1]), __float_as_int(lo[2]),
                               __float_as_int(lo[3]), __float_as_int(hi[0]), __float_as_int(hi[1]),
                               __float_as_int(hi[2]), __float_as_int(hi[3])};
@@ -255,24 +265,27 @@ __global__ __launch_bounds__(kThreads) void gemm_fp8res_kernel(GemmArgs p) {
 #pragma unroll
                 for (int c = 0; c < kFN; ++c)
                     acc[r][c] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
-                        a8[r], b8[c], acc[r][c], 0, 0, 0, sa, 0, sb);
+                        a8[r], b8[c], acc[r][c], 0, 0, 0, ea + 127, 0, eb + 127);
         }
-        // BF16 residual: Ar.Bq + Aq.Br, four 16-deep steps
-        const uint16_t *ap = (const uint16_t *)(smem + kAP) + arow * kRowP;
-        const uint16_t *bp = (const uint16_t *)(smem + kBP) + brow * kRowP;
+        // BF16 residual: Ar.Bq + Aq.Br, four 16-deep steps; lane half h holds
+        // k = 16 st + 8h .. +7 (residual chunk 2 st + h; e4m3 chunk st, bytes 8h..)
+        const float sa = ldexpf(1.0f, ea), sb = ldexpf(1.0f, eb);
 #pragma unroll
-        for (int st = 0; st < kBK / 16; ++st) {
-            const int off = (2 * st + h) * 16;
+        for (int s = 0; s < kBK / 16; ++s) {
             bf16x8 aq[kFM], ar[kFM], bq[kFN], br[kFN];
 #pragma unroll
             for (int r = 0; r < kFM; ++r) {
-                aq[r] = *(const bf16x8 *)(ap + r * 32 * kRowP + off);
-                ar[r] = *(const bf16x8 *)(ap + r * 32 * kRowP + off + 8);
+                const int row = arow + r * 32;
+                ar[r] = *(const bf16x8 *)(st + kAR + swr(row, 2 * s + h));
+                const uint2 w = *(const uint2 *)(st + kA8 + sw8(row, s) + 8 * h);
+                aq[r] = fp8x8_to_bf16(w.x, w.y, sa);
             }
 #pragma unroll
             for (int c = 0; c < kFN; ++c) {
-                bq[c] = *(const bf16x8 *)(bp + c * 32 * kRowP + off);
-                br[c] = *(const bf16x8 *)(bp + c * 32 * kRowP + off + 8);
+                const int row = brow + c * 32;
+                br[c] = *(const bf16x8 *)(st + kBR + swr(row, 2 * s + h));
+                const uint2 w = *(const uint2 *)(st + kB8 + sw8(row, s) + 8 * h);
+                bq[c] = fp8x8_to_bf16(w.x, w.y, sb);
             }
 #pragma unroll
             for (int r = 0; r < kFM; ++r)
@@ -282,6 +295,17 @@ __global__ __launch_bounds__(kThreads) void gemm_fp8res_kernel(GemmArgs p) {
                     acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aq[r], br[c], acc[r][c], 0, 0, 0);
                 }
         }
+    };
+
+    const int KT = p.Kp / kBK;
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    for (int kt = 0; kt < KT; ++kt) {
+        const int cur = kt & 1;
+        if (kt + 1 < KT) gload((kt + 1) * kBK);
+        compute(cur, kt);
+        if (kt + 1 < KT) sstore(cur ^ 1);
         __syncthreads();
     }
 
@@ -303,7 +327,7 @@ inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
 struct Layout {
     int Mp, Np, Kp;
-    size_t a8, ap, ea, b8, bp, eb, total;  // byte offsets
+    size_t a8, ar, ea, b8, br, eb, total;  // byte offsets
 };
 
 Layout layout(int M, int N, int K) {
@@ -314,10 +338,10 @@ Layout layout(int M, int N, int K) {
     auto al = [](size_t x) { return (size_t)round_up((int64_t)x, 256); };
     size_t o = 0;
     L.a8 = o; o = al(o + (size_t)L.Mp * L.Kp);
-    L.ap = o; o = al(o + (size_t)L.Mp * L.Kp * 4);
+    L.ar = o; o = al(o + (size_t)L.Mp * L.Kp * 2);
     L.ea = o; o = al(o + (size_t)(L.Mp / kQT) * (L.Kp / kQT) * 4);
     L.b8 = o; o = al(o + (size_t)L.Np * L.Kp);
-    L.bp = o; o = al(o + (size_t)L.Np * L.Kp * 4);
+    L.br = o; o = al(o + (size_t)L.Np * L.Kp * 2);
     L.eb = o; o = al(o + (size_t)(L.Np / kQT) * (L.Kp / kQT) * 4);
     L.total = o;
     return L;
@@ -343,19 +367,19 @@ extern "C" int hbs_gemm_fp8res(const float *A, const float *B, float *C, int M, 
     hipStream_t s = (hipStream_t)stream;
     char *ws = (char *)workspace;
 
-    QuantArgs qa{A, M, K, lda, L.Mp, L.Kp, (uint8_t *)(ws + L.a8), (uint16_t *)(ws + L.ap),
+    QuantArgs qa{A, M, K, lda, L.Mp, L.Kp, (uint8_t *)(ws + L.a8), (uint16_t *)(ws + L.ar),
                  (int *)(ws + L.ea)};
     quantize_kernel<false><<<(L.Mp / kQT) * (L.Kp / kQT), 256, 0, s>>>(qa);
     int rc = hbs::launch_status();
     if (rc) return rc;
-    QuantArgs qb{B, K, N, ldb, L.Np, L.Kp, (uint8_t *)(ws + L.b8), (uint16_t *)(ws + L.bp),
+    QuantArgs qb{B, K, N, ldb, L.Np, L.Kp, (uint8_t *)(ws + L.b8), (uint16_t *)(ws + L.br),
                  (int *)(ws + L.eb)};
     quantize_kernel<true><<<(L.Np / kQT) * (L.Kp / kQT), 256, 0, s>>>(qb);
     rc = hbs::launch_status();
     if (rc) return rc;
 
     GemmArgs g{(const uint8_t *)(ws + L.a8), (const uint8_t *)(ws + L.b8),
-               (const uint16_t *)(ws + L.ap), (const uint16_t *)(ws + L.bp),
+               (const uint16_t *)(ws + L.ar), (const uint16_t *)(ws + L.br),
                (const int *)(ws + L.ea), (const int *)(ws + L.eb), C, M, N, L.Kp, ldc,
                L.Mp / kBM, L.Np / kBN};
     const int64_t blocks = (int64_t)g.tiles_m * g.tiles_n;
