@@ -105,7 +105,7 @@ int main(int argc, char **argv) {
     fprintf(out, "# hbsgemm grouped tuning file: prec Mclass Nclass Kclass tile split ns_per_item\n");
     for (int prec : precs) {
         const char *pname = prec == HBS_PREC_FP32 ? "fp32" : "bf16x3";
-        const int ntiles = prec == HBS_PREC_FP32 ? 4 : 2;
+        const int ntiles = 4;
         for (int mc : classes) {
             for (int nc : classes) {
                 for (int kc : classes) {

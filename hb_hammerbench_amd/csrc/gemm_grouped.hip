@@ -11,14 +11,16 @@
 //    size heuristic -- expands every problem into work items (one output tile
 //    times one K range) and assigns them to a fixed number of persistent
 //    workgroups by longest-processing-time-first bin packing;
-//  * the kernel is one launch of (1 or 2 x CUs) workgroups, each walking its own
+//  * the kernel is one launch of (3 or 2 x CUs) workgroups, each walking its own
 //    item list, so the launch is capturable in a hipGraph and needs no host
 //    work or memset between replays;
 //  * split-K tiles write FP32 partials; the last of a tile's splits to finish
 //    (an agent-scope counter) adds them in split order -- deterministic -- and
 //    resets the counter to zero for the next run;
-//  * BF16x3 splits each float into bf16 hi/lo while staging A and B into LDS,
-//    so the group stays a single launch.
+//  * BF16x3 first runs one pre-pass launch that splits every operand into
+//    packed bf16 hi/lo rows in plan memory (B transposed, zero-padded), so the
+//    persistent GEMM streams 16-byte packed chunks with no split work; the two
+//    launches are captured together in the hipGraph.
 #include <stdlib.h>
 #include <string.h>
 
@@ -52,6 +54,8 @@ struct hbs_grouped_plan {
     const int *d_boff;
     int *d_counters;
     float *d_partials;
+    const void *d_tasks;  // BF16x3 pack tasks
+    int ntasks;
 };
 
 namespace {
@@ -61,6 +65,18 @@ struct Prob {
     const float *B;
     float *C;
     int M, N, K, lda, ldb, ldc;
+    // BF16x3 only: packed hi/lo operands in plan memory, rows padded to 128,
+    // K padded to Kp (a multiple of 32): Ap[Mp][Kp/8][hi8|lo8], Bp[Np][Kp/8][hi8|lo8]
+    // with B transposed (row n holds column n).
+    const uint16_t *Ap;
+    const uint16_t *Bp;
+    int Kp;
+};
+
+// One 64 x 64 packing task of the BF16x3 pre-pass: op 0 packs rows r0..r0+63
+// of A over k0..k0+63; op 1 packs columns r0..r0+63 of B over k0..k0+63.
+struct PackTask {
+    int prob, op, r0, k0;
 };
 
 struct Item {
@@ -129,34 +145,48 @@ __device__ __forceinline__ void finish_tile(floatx16 (&acc)[BM / 64][BN / 64], c
         __syncthreads();
         if (!*s_flag) return;
         __threadfence();
+        // last split: sum the partials in split order straight into C, one
+        // 32x32 block at a time (the accumulators are no longer needed)
+#pragma unroll 1
+        for (int b = 0; b < FM * FN; ++b) {
+            const int r = b / FN, c = b % FN;
+            const int row0 = m0 + wm * (BM / 2) + r * 32 + 4 * h;
+            const int col = n0 + wn * (BN / 2) + c * 32 + l32;
+            float *cp = p.C + (int64_t)row0 * p.ldc + col;
+            const float *src0 = partials + it.poff + wave * (FM * FN * 16 * 64) + b * 16 * 64 + lane;
+            float v[16];
 #pragma unroll
-        for (int r = 0; r < FM; ++r)
+            for (int q = 0; q < 16; ++q) v[q] = 0.0f;
+#pragma unroll 1
+            for (int sp = 0; sp < it.nsplit; ++sp) {
+                const float *src = src0 + (int64_t)sp * BM * BN;
 #pragma unroll
-            for (int c = 0; c < FN; ++c)
+                for (int q = 0; q < 16; ++q) v[q] += __builtin_nontemporal_load(&src[q * 64]);
+            }
 #pragma unroll
-                for (int q = 0; q < 16; ++q) acc[r][c][q] = 0.0f;
-        for (int s = 0; s < it.nsplit; ++s) {
-            const float *src = partials + it.poff + (int64_t)s * BM * BN + wave * (FM * FN * 16 * 64);
-#pragma unroll
-            for (int r = 0; r < FM; ++r)
-#pragma unroll
-                for (int c = 0; c < FN; ++c)
-#pragma unroll
-                    for (int q = 0; q < 16; ++q)
-                        acc[r][c][q] += __builtin_nontemporal_load(&src[((r * FN + c) * 16 + q) * 64 + lane]);
+            for (int q = 0; q < 16; ++q) {
+                const int dr = 8 * (q >> 2) + (q & 3);
+                if (col < p.N && row0 + dr < p.M) cp[(int64_t)dr * p.ldc] = v[q];
+            }
         }
         if (tid == 0)
             __hip_atomic_store(&counters[it.slot], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
     }
+    // one base pointer per 32x32 block; the per-register row offsets are
+    // wave-uniform multiples of ldc
 #pragma unroll
     for (int r = 0; r < FM; ++r) {
+        const int row0 = m0 + wm * (BM / 2) + r * 32 + 4 * h;
 #pragma unroll
         for (int c = 0; c < FN; ++c) {
             const int col = n0 + wn * (BN / 2) + c * 32 + l32;
+            float *cp = p.C + (int64_t)row0 * p.ldc + col;
+            const bool colok = col < p.N;
 #pragma unroll
             for (int q = 0; q < 16; ++q) {
-                const int row = m0 + wm * (BM / 2) + r * 32 + 8 * (q >> 2) + 4 * h + (q & 3);
-                if (row < p.M && col < p.N) p.C[(int64_t)row * p.ldc + col] = acc[r][c][q];
+                const int dr = 8 * (q >> 2) + (q & 3);
+                if (colok && row0 + dr < p.M) cp[(int64_t)dr * p.ldc] = acc[r][c][q];
             }
         }
     }
@@ -259,15 +289,92 @@ __device__ void f32_tile(float *smem, const Prob &p, const Item &it, float *part
 }
 
 // ---------------------------------------------------------------------------
-// BF16x3 tile: BK = 32, floats split into packed hi/lo rows while staging,
-// three v_mfma_f32_32x32x16_bf16 products per k step.
+// BF16x3 pre-pass: split every float into bf16 hi/lo and pack 8-k groups
+// [hi8|lo8], B transposed, zero-padded (one 64 x 64 task per workgroup).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void grouped_pack_kernel(const Prob *__restrict__ probs,
+                                                           const PackTask *__restrict__ tasks) {
+    __shared__ float t[64][65];
+    const PackTask tk = tasks[blockIdx.x];
+    const Prob p = probs[tk.prob];
+    const int tid = threadIdx.x;
+    const int gpr = p.Kp / 8;  // 8-k groups per packed row
+    if (tk.op == 0) {
+        // A rows: thread -> (row, group) pairs, 8 consecutive k each
+        for (int e = tid; e < 64 * 8; e += 256) {
+            const int row = e >> 3, g = e & 7;
+            const int gr = tk.r0 + row, gk = tk.k0 + g * 8;
+            if (gk >= p.Kp) continue;
+            float v[8];
+            const float *src = p.A + (int64_t)gr * p.lda + gk;
+            if (gr < p.M && gk + 8 <= p.K) {
+                const floatx4 a0 = *(const floatx4 *)src, a1 = *(const floatx4 *)(src + 4);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    v[j] = a0[j];
+                    v[j + 4] = a1[j];
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) v[j] = (gr < p.M && gk + j < p.K) ? src[j] : 0.0f;
+            }
+            ushort8v hi, lo;
+#pragma unroll
+            for (int j = 0; j < 8; j += 2) {
+                uint16_t h0, h1, l0, l1;
+                split2(v[j], v[j + 1], h0, h1, l0, l1);
+                hi[j] = h0; hi[j + 1] = h1;
+                lo[j] = l0; lo[j + 1] = l1;
+            }
+            uint16_t *dst = (uint16_t *)p.Ap + ((int64_t)gr * gpr + gk / 8) * 16;
+            *(ushort8v *)dst = hi;
+            *(ushort8v *)(dst + 8) = lo;
+        }
+    } else {
+        // B: stage B[k0..k0+63][r0..r0+63] and write columns as packed rows
+        for (int e = tid; e < 64 * 16; e += 256) {
+            const int kk = e >> 4, c4 = (e & 15) * 4;
+            const int gk = tk.k0 + kk, gn = tk.r0 + c4;
+            floatx4 v = {0, 0, 0, 0};
+            if (gk < p.K && gn + 4 <= p.N) {
+                v = *(const floatx4 *)(p.B + (int64_t)gk * p.ldb + gn);
+            } else if (gk < p.K) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) v[j] = (gn + j < p.N) ? p.B[(int64_t)gk * p.ldb + gn + j] : 0.0f;
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) t[kk][c4 + j] = v[j];
+        }
+        __syncthreads();
+        for (int e = tid; e < 64 * 8; e += 256) {
+            const int nn = e >> 3, g = e & 7;
+            const int gkg = tk.k0 / 8 + g;
+            if (gkg >= gpr) continue;
+            ushort8v hi, lo;
+#pragma unroll
+            for (int j = 0; j < 8; j += 2) {
+                uint16_t h0, h1, l0, l1;
+                split2(t[g * 8 + j][nn], t[g * 8 + j + 1][nn], h0, h1, l0, l1);
+                hi[j] = h0; hi[j + 1] = h1;
+                lo[j] = l0; lo[j + 1] = l1;
+            }
+            uint16_t *dst = (uint16_t *)p.Bp + ((int64_t)(tk.r0 + nn) * gpr + gkg) * 16;
+            *(ushort8v *)dst = hi;
+            *(ushort8v *)(dst + 8) = lo;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// BF16x3 tile on packed operands: BK = 32, global -> registers -> LDS double
+// buffer, three v_mfma_f32_32x32x16_bf16 products per k step.
 // ---------------------------------------------------------------------------
 template <int BM, int BN>
 struct B3T {
     static constexpr int BK = 32, ROW = 2 * BK + 8;  // packed bf16 per LDS row (+16 B pad)
     static constexpr int STAGE = (BM + BN) * ROW;     // bf16 elements
-    static constexpr int AT = BM * 4 / kThreads;      // A tasks (row, 8-k group) per thread
-    static constexpr int BT = (2 * BN + kThreads - 1) / kThreads;  // B tasks (4 k x 4 n)
+    static constexpr int LA = BM * 8 / kThreads;      // 16-byte chunks of A per thread
+    static constexpr int LB = BN * 8 / kThreads;
     static constexpr int LDS_BYTES = 2 * STAGE * 2;
 };
 
@@ -280,6 +387,9 @@ __device__ void b3_tile(uint16_t *smem, const Prob &p, const Item &it, float *pa
     const int wm = wave & 1, wn = wave >> 1;
     const int l32 = lane & 31, h = lane >> 5;
     const int m0 = it.tm * BM, n0 = it.tn * BN;
+    const int64_t rowe = (int64_t)p.Kp * 2;  // packed elements per row
+    const uint16_t *Ag = p.Ap + (int64_t)m0 * rowe;
+    const uint16_t *Bg = p.Bp + (int64_t)n0 * rowe;
 
     floatx16 acc[FM][FN];
 #pragma unroll
@@ -287,69 +397,29 @@ __device__ void b3_tile(uint16_t *smem, const Prob &p, const Item &it, float *pa
 #pragma unroll
         for (int c = 0; c < FN; ++c) acc[r][c] = floatx16{0};
 
-    floatx4 ra[T::AT][2], rb[T::BT][4];
+    floatx4 ra[T::LA], rb[T::LB];
+    // thread tid loads chunk (tid & 7) of rows (tid >> 3) + 32 i
+    const uint16_t *asrc = Ag + (tid >> 3) * rowe + (tid & 7) * 8;
+    const uint16_t *bsrc = Bg + (tid >> 3) * rowe + (tid & 7) * 8;
+    const int64_t step = 32 * rowe;
     auto gload = [&](int k) {
 #pragma unroll
-        for (int i = 0; i < T::AT; ++i) {
-            const int f = tid + i * kThreads;
-            const int row = f / 4, gk = k + (f % 4) * 8;
-            const bool rv = m0 + row < p.M;
-            const float *src = p.A + (int64_t)(m0 + row) * p.lda + gk;
-            ra[i][0] = (rv && gk < it.k1) ? *(const floatx4 *)src : floatx4{0, 0, 0, 0};
-            ra[i][1] = (rv && gk + 4 < it.k1) ? *(const floatx4 *)(src + 4) : floatx4{0, 0, 0, 0};
-        }
+        for (int i = 0; i < T::LA; ++i) ra[i] = *(const floatx4 *)(asrc + i * step + 2 * k);
 #pragma unroll
-        for (int i = 0; i < T::BT; ++i) {
-            const int f = tid + i * kThreads;
-            const int hg = f / (BN / 4), gn = n0 + (f % (BN / 4)) * 4;
-            const bool task = f < 2 * BN;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int gk = k + hg * 4 + j;
-                rb[i][j] = (task && gk < it.k1 && gn < p.N)
-                               ? *(const floatx4 *)(p.B + (int64_t)gk * p.ldb + gn)
-                               : floatx4{0, 0, 0, 0};
-            }
-        }
+        for (int i = 0; i < T::LB; ++i) rb[i] = *(const floatx4 *)(bsrc + i * step + 2 * k);
     };
     auto sstore = [&](int buf) {
         uint16_t *as = smem + buf * T::STAGE;
         uint16_t *bs = as + BM * ROW;
 #pragma unroll
-        for (int i = 0; i < T::AT; ++i) {
+        for (int i = 0; i < T::LA; ++i) {
             const int f = tid + i * kThreads;
-            const int row = f / 4, g = f % 4;
-            ushort8v hi, lo;
-#pragma unroll
-            for (int j = 0; j < 8; j += 2) {
-                uint16_t h0, h1, l0, l1;
-                split2(ra[i][j / 4][j % 4], ra[i][j / 4][j % 4 + 1], h0, h1, l0, l1);
-                hi[j] = h0; hi[j + 1] = h1;
-                lo[j] = l0; lo[j + 1] = l1;
-            }
-            *(ushort8v *)(as + row * ROW + g * 16) = hi;
-            *(ushort8v *)(as + row * ROW + g * 16 + 8) = lo;
+            *(floatx4 *)(as + (f >> 3) * ROW + (f & 7) * 8) = ra[i];
         }
 #pragma unroll
-        for (int i = 0; i < T::BT; ++i) {
+        for (int i = 0; i < T::LB; ++i) {
             const int f = tid + i * kThreads;
-            if (f >= 2 * BN) break;
-            const int hg = f / (BN / 4), c = f % (BN / 4);
-            const int off = (hg >> 1) * 16 + (hg & 1) * 4;
-#pragma unroll
-            for (int jn = 0; jn < 4; ++jn) {
-                ushort4v hi, lo;
-#pragma unroll
-                for (int j = 0; j < 4; j += 2) {
-                    uint16_t h0, h1, l0, l1;
-                    split2(rb[i][j][jn], rb[i][j + 1][jn], h0, h1, l0, l1);
-                    hi[j] = h0; hi[j + 1] = h1;
-                    lo[j] = l0; lo[j + 1] = l1;
-                }
-                uint16_t *dst = bs + (c * 4 + jn) * ROW + off;
-                *(ushort4v *)dst = hi;
-                *(ushort4v *)(dst + 8) = lo;
-            }
+            *(floatx4 *)(bs + (f >> 3) * ROW + (f & 7) * 8) = rb[i];
         }
     };
     auto compute = [&](int buf) {
@@ -399,7 +469,7 @@ constexpr int kLdsB3 = B3T<128, 128>::LDS_BYTES;
 static_assert(F32T<64, 128>::LDS_BYTES <= kLdsF32 && F32T<128, 64>::LDS_BYTES <= kLdsF32, "lds");
 
 template <int PREC>
-__global__ __launch_bounds__(kThreads, PREC == HBS_PREC_FP32 ? 2 : 1) void grouped_kernel(const Prob *__restrict__ probs,
+__global__ __launch_bounds__(kThreads, PREC == HBS_PREC_FP32 ? 3 : 2) void grouped_kernel(const Prob *__restrict__ probs,
                                                               const Item *__restrict__ items,
                                                               const int *__restrict__ boff,
                                                               float *partials, int *counters) {
@@ -422,7 +492,9 @@ __global__ __launch_bounds__(kThreads, PREC == HBS_PREC_FP32 ? 2 : 1) void group
             uint16_t *s = (uint16_t *)smem;
             switch (it.cfg) {
                 case 0: b3_tile<128, 128>(s, p, it, partials, counters, &s_flag); break;
-                default: b3_tile<64, 64>(s, p, it, partials, counters, &s_flag); break;
+                case 1: b3_tile<64, 64>(s, p, it, partials, counters, &s_flag); break;
+                case 2: b3_tile<128, 64>(s, p, it, partials, counters, &s_flag); break;
+                default: b3_tile<64, 128>(s, p, it, partials, counters, &s_flag); break;
             }
         }
         __syncthreads();  // s_flag and LDS are reused by the next item
@@ -469,10 +541,9 @@ bool load_tuning(const char *path, std::map<TuneKey, Choice> &table) {
     return true;
 }
 
-// BF16x3 tiles are 128x128 or 64x64 only: with all four shapes inlined in one
-// kernel the hi/lo staging exceeds the 256 architectural VGPRs and spills.
 bool tile_supported(int precision, int tile) {
-    return tile >= 0 && tile <= 3 && (precision == HBS_PREC_FP32 || tile <= 1);
+    (void)precision;
+    return tile >= 0 && tile <= 3;
 }
 
 Choice heuristic(int precision, int M, int N, int K) {
@@ -480,7 +551,6 @@ Choice heuristic(int precision, int M, int N, int K) {
     if (M <= 64 && N <= 64) c.tile = 1;
     else if (M <= 64) c.tile = 3;
     else if (N <= 64) c.tile = 2;
-    if (!tile_supported(precision, c.tile)) c.tile = (M <= 64 || N <= 64) ? 1 : 0;
     const int64_t tiles = (int64_t)((M + kTileBM[c.tile] - 1) / kTileBM[c.tile]) *
                           ((N + kTileBN[c.tile] - 1) / kTileBN[c.tile]);
     if (tiles <= 4 && K >= 1024) c.split = 4;
@@ -515,12 +585,28 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
     HBS_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
 
     std::vector<Prob> probs(count);
+    std::vector<PackTask> tasks;
+    std::vector<int64_t> pack_off_a(count, 0), pack_off_b(count, 0);
+    int64_t pack_elems = 0;  // bf16 elements of packed operands
     std::vector<Item> items;
     std::vector<double> cost;
     int64_t slots = 0, poff = 0, tiles_total = 0;
     for (int i = 0; i < count; ++i) {
         const hbs_gemm_problem &q = problems[i];
-        probs[i] = Prob{q.A, q.B, q.C, q.M, q.N, q.K, q.lda, q.ldb, q.ldc};
+        probs[i] = Prob{q.A, q.B, q.C, q.M, q.N, q.K, q.lda, q.ldb, q.ldc, nullptr, nullptr, 0};
+        if (precision == HBS_PREC_BF16X3) {
+            const int Kp = (q.K + 31) / 32 * 32, Mp = (q.M + 127) / 128 * 128,
+                      Np = (q.N + 127) / 128 * 128;
+            probs[i].Kp = Kp;
+            pack_off_a[i] = pack_elems;
+            pack_elems += (int64_t)Mp * Kp * 2;
+            pack_off_b[i] = pack_elems;
+            pack_elems += (int64_t)Np * Kp * 2;
+            for (int r0 = 0; r0 < Mp; r0 += 64)
+                for (int k0 = 0; k0 < Kp; k0 += 64) tasks.push_back(PackTask{i, 0, r0, k0});
+            for (int r0 = 0; r0 < Np; r0 += 64)
+                for (int k0 = 0; k0 < Kp; k0 += 64) tasks.push_back(PackTask{i, 1, r0, k0});
+        }
         Choice c = heuristic(precision, q.M, q.N, q.K);
         if (forced) {
             c = *forced;
@@ -536,9 +622,9 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
         const int ns = (q.K + chunk - 1) / chunk;
         // Work-item cost in ns of one workgroup slot: the autotuner's measured
         // time when the tuning file has this class, else a throughput model
-        // (FP32: 157.3 TFLOP/s shared by 2 x CUs slots; BF16x3: 2516.6 / 3
-        // TFLOP/s-equivalent by 1 x CUs slots) with a tile-shape efficiency.
-        const double slots_all = (precision == HBS_PREC_FP32 ? 2.0 : 1.0) * cus;
+        // (FP32: 157.3 TFLOP/s shared by 3 x CUs slots; BF16x3: 2516.6 / 3
+        // TFLOP/s-equivalent by 2 x CUs slots) with a tile-shape efficiency.
+        const double slots_all = (precision == HBS_PREC_FP32 ? 3.0 : 2.0) * cus;
         const double peak = precision == HBS_PREC_FP32 ? 157.3e12 : 2516.6e12 / 3.0;
         const double eff = (c.tile == 0) ? 0.7 : (c.tile == 1 ? 0.35 : 0.5);
         for (int tm = 0; tm < tm_n; ++tm) {
@@ -568,12 +654,12 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
             }
         }
     }
-    if (items.size() > 0x7fffffffULL) return HBS_ERR_ARG;
+    if (items.size() > 0x7fffffffULL || tasks.size() > 0x7fffffffULL) return HBS_ERR_ARG;
 
     // LPT: items in decreasing cost, each to the least-loaded workgroup.
-    // FP32 tiles fit two workgroups per CU (<= 256 VGPRs); BF16x3 tiles need
-    // more registers for the hi/lo staging and run one workgroup per CU.
-    const int per_cu = precision == HBS_PREC_FP32 ? 2 : 1;
+    // resident workgroups per CU: FP32 tiles use <= 168 VGPRs and 38 KB LDS
+    // (three fit), BF16x3 tiles <= 256 VGPRs and 74 KB LDS (two fit)
+    const int per_cu = precision == HBS_PREC_FP32 ? 3 : 2;
     const int nblocks = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)per_cu * cus, (int64_t)items.size()));
     std::vector<int64_t> order(items.size());
     for (size_t i = 0; i < order.size(); ++i) order[i] = (int64_t)i;
@@ -612,8 +698,15 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
     const size_t b_boff = align(boff.size() * sizeof(int));
     const size_t b_cnt = align(std::max<int64_t>(slots, 1) * sizeof(int));
     const size_t b_part = align(std::max<int64_t>(poff, 1) * sizeof(float));
+    const size_t b_tasks = align(std::max<size_t>(tasks.size(), 1) * sizeof(PackTask));
+    const size_t b_pack = align((size_t)std::max<int64_t>(pack_elems, 1) * sizeof(uint16_t));
     char *dmem = nullptr;
-    HBS_HIP_TRY(hipMalloc(&dmem, b_probs + b_items + b_boff + b_cnt + b_part));
+    HBS_HIP_TRY(hipMalloc(&dmem, b_probs + b_items + b_boff + b_cnt + b_part + b_tasks + b_pack));
+    char *packed = dmem + b_probs + b_items + b_boff + b_cnt + b_part + b_tasks;
+    for (int i = 0; i < count && precision == HBS_PREC_BF16X3; ++i) {
+        probs[i].Ap = (const uint16_t *)packed + pack_off_a[i];
+        probs[i].Bp = (const uint16_t *)packed + pack_off_b[i];
+    }
     hbs_grouped_plan *plan = new hbs_grouped_plan();
     plan->device = dev;
     plan->precision = precision;
@@ -629,11 +722,16 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
     plan->d_boff = (const int *)(dmem + b_probs + b_items);
     plan->d_counters = (int *)(dmem + b_probs + b_items + b_boff);
     plan->d_partials = (float *)(dmem + b_probs + b_items + b_boff + b_cnt);
+    plan->d_tasks = dmem + b_probs + b_items + b_boff + b_cnt + b_part;
+    plan->ntasks = (int)tasks.size();
     hipError_t e = hipMemcpy(dmem, probs.data(), probs.size() * sizeof(Prob), hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(dmem + b_probs, sorted.data(), sorted.size() * sizeof(Item), hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(dmem + b_probs + b_items, boff.data(), boff.size() * sizeof(int),
+                      hipMemcpyHostToDevice);
+    if (e == hipSuccess && !tasks.empty())
+        e = hipMemcpy((void *)plan->d_tasks, tasks.data(), tasks.size() * sizeof(PackTask),
                       hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(plan->d_counters, 0, b_cnt);
     if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -693,10 +791,14 @@ extern "C" int hbs_grouped_run(const hbs_grouped_plan *plan, void *stream) {
         grouped_kernel<HBS_PREC_FP32><<<plan->nblocks, kThreads, 0, s>>>(pr, it, plan->d_boff,
                                                                          plan->d_partials,
                                                                          plan->d_counters);
-    else
+    else {
+        grouped_pack_kernel<<<plan->ntasks, 256, 0, s>>>(pr, (const PackTask *)plan->d_tasks);
+        const int rc = hbs::launch_status();
+        if (rc) return rc;
         grouped_kernel<HBS_PREC_BF16X3><<<plan->nblocks, kThreads, 0, s>>>(pr, it, plan->d_boff,
                                                                            plan->d_partials,
                                                                            plan->d_counters);
+    }
     return hbs::launch_status();
 }
 

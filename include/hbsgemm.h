@@ -130,13 +130,14 @@ typedef struct {
 int hbs_grouped_plan_create(const hbs_gemm_problem *problems, int count, int precision,
                             const char *tuning_file, hbs_grouped_plan **plan);
 /* Same with one tile shape and split-K factor (1..64) for every problem (the
- * autotuner's probe).  BF16x3 supports HBS_TILE_128x128 and HBS_TILE_64x64;
- * other shapes give HBS_ERR_ARG. */
+ * autotuner's probe). */
 int hbs_grouped_plan_create_forced(const hbs_gemm_problem *problems, int count, int precision,
                                    int tile, int split_k, hbs_grouped_plan **plan);
 int hbs_grouped_plan_info(const hbs_grouped_plan *plan, hbs_grouped_info *info);
-/* Computes every problem of the plan in one kernel launch on `stream`.
- * Deterministic; runs of one plan must not overlap (they share counters). */
+/* Computes every problem of the plan on `stream`: one persistent kernel launch
+ * (FP32), or a packing pre-pass plus the persistent launch (BF16x3; the packed
+ * hi/lo operands live in plan memory).  Deterministic; runs of one plan must
+ * not overlap (they share counters and packed buffers). */
 int hbs_grouped_run(const hbs_grouped_plan *plan, void *stream);
 int hbs_grouped_plan_destroy(hbs_grouped_plan *plan);
 
