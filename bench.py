@@ -32,6 +32,8 @@ def parse():
     p.add_argument("--variant", choices=["fp32", "bf16x3", "grouped-fp32", "grouped-bf16x3",
                                          "fp8res"], default="fp32")
     p.add_argument("--problems", type=int, default=512, help="grouped: number of problems")
+    p.add_argument("--dump-shapes", default=None,
+                   help="grouped: write the seeded problem shapes (M N K per line) and exit")
     p.add_argument("--tuning", default=os.path.join(ROOT, "hb_hammerbench_amd", "tuning",
                                                     "mi355x_grouped.txt"),
                    help="grouped: autotuner tuning file ('' for the size heuristic)")
@@ -156,13 +158,18 @@ def setup_square(args, hb, torch):
         "traffic": measured_traffic(args.variant, f"N={N} NITER={NITER}")}
 
 
+def grouped_shapes(seed, problems):
+    """M, N, K each uniform over {64, 128, ..., 2048} from numpy default_rng(seed)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    return [tuple(int(x) for x in rng.integers(1, 33, 3) * 64) for _ in range(problems)]
+
+
 def setup_grouped(args, hb, torch):
     """`--problems` independent GEMMs, M, N, K each uniform over {64, 128, ..., 2048}
     (numpy default_rng(seed)), uniform[-1,1) inputs, one persistent launch replayed
     from a hipGraph."""
-    import numpy as np
-    rng = np.random.default_rng(args.seed)
-    shapes = [tuple(int(x) for x in rng.integers(1, 33, 3) * 64) for _ in range(args.problems)]
+    shapes = grouped_shapes(args.seed, args.problems)
     ta = sum(M * K for M, N, K in shapes)
     tb = sum(K * N for M, N, K in shapes)
     tc = sum(M * N for M, N, K in shapes)
@@ -238,6 +245,11 @@ def setup_fp8res(args, hb, torch):
 
 def main():
     args = parse()
+    if args.dump_shapes:
+        with open(args.dump_shapes, "w") as f:
+            for M, N, K in grouped_shapes(args.seed, args.problems):
+                f.write(f"{M} {N} {K}\n")
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(this build runs the path on a single MI355X)", file=sys.stderr)

@@ -8,14 +8,25 @@
 //   <fp32|bf16x3> <M class> <N class> <K class> <tile id> <split> <ns per work item>
 // A size class is the power of two nearest the size in log scale (sizes in
 // [c / sqrt2, c * sqrt2) map to c), clamped to [64, 2048].
-// Each probe runs a group of identical problems large enough to fill the GPU,
-// timed with HIP events over --reps launches after one warm-up launch.
+//
+// Two modes:
+//  * --classes (default): every candidate per size class on a group of
+//    identical problems large enough to fill the GPU, timed with HIP events
+//    over --reps launches after one warm-up launch; writes one line per class
+//    with the measured time per work item (the plan's load-balancing cost);
+//  * --group FILE (lines "M N K"): tunes for that actual problem set.  Starting
+//    from the plan's size heuristic it visits the size classes in decreasing
+//    share of the flops, times the whole group with each candidate override of
+//    that class, and keeps an override only when the group runs >= 1% faster,
+//    so the result is never slower than the heuristic on that workload.
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
+#include <map>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -48,8 +59,178 @@ static std::vector<int> parse_list(const char *s) {
     return v;
 }
 
+static int size_class(int x) {  // same rule as the plan
+    int c = 64;
+    while (c < 2048 && (long long)x * x >= 2LL * c * c) c *= 2;
+    return c;
+}
+
+struct Override {
+    int mc, nc, kc, tile, split;
+};
+
+static void write_table(const char *path, const char *pname, const std::vector<Override> &ov,
+                        const char *comment) {
+    FILE *f = fopen(path, "w");
+    if (!f) {
+        perror(path);
+        exit(2);
+    }
+    fprintf(f, "# hbsgemm grouped tuning file: prec Mclass Nclass Kclass tile split ns_per_item\n");
+    if (comment) fprintf(f, "%s", comment);
+    for (const Override &o : ov)
+        fprintf(f, "%s %d %d %d %d %d 0\n", pname, o.mc, o.nc, o.kc, o.tile, o.split);
+    fclose(f);
+}
+
+// Median time (ms) of `reps` runs of the whole group under tuning file `path`
+// ("" = the plan's heuristic).
+static double time_group(const std::vector<hbs_gemm_problem> &probs, int prec, const char *path,
+                         int reps, hipEvent_t e0, hipEvent_t e1) {
+    hbs_grouped_plan *plan = nullptr;
+    int rc = hbs_grouped_plan_create(probs.data(), (int)probs.size(), prec,
+                                     path && path[0] ? path : nullptr, &plan);
+    if (rc) {
+        fprintf(stderr, "plan: %s\n", hbs_strerror(rc));
+        exit(2);
+    }
+    rc = hbs_grouped_run(plan, nullptr);
+    std::vector<double> t;
+    for (int r = 0; r < reps && rc == 0; ++r) {
+        HIP_CHECK(hipEventRecord(e0, nullptr));
+        rc = hbs_grouped_run(plan, nullptr);
+        HIP_CHECK(hipEventRecord(e1, nullptr));
+        HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        t.push_back(ms);
+    }
+    if (rc) {
+        fprintf(stderr, "run: %s\n", hbs_strerror(rc));
+        exit(2);
+    }
+    hbs_grouped_plan_destroy(plan);
+    std::sort(t.begin(), t.end());
+    return t[t.size() / 2];
+}
+
+static int group_mode(const char *shapes_path, const char *out_path, const std::vector<int> &precs,
+                      int reps) {
+    std::vector<int> M, N, K;
+    FILE *f = fopen(shapes_path, "r");
+    if (!f) {
+        perror(shapes_path);
+        return 2;
+    }
+    int m, n, k;
+    while (fscanf(f, "%d %d %d", &m, &n, &k) == 3) {
+        if (m < 1 || n < 1 || k < 1 || n % 4 || k % 4) {
+            fprintf(stderr, "bad shape %d %d %d\n", m, n, k);
+            return 2;
+        }
+        M.push_back(m);
+        N.push_back(n);
+        K.push_back(k);
+    }
+    fclose(f);
+    const int count = (int)M.size();
+    if (!count) return 2;
+    size_t ta = 0, tb = 0, tc = 0;
+    for (int i = 0; i < count; ++i) {
+        ta += ((size_t)M[i] * K[i] + 63) / 64 * 64;
+        tb += ((size_t)K[i] * N[i] + 63) / 64 * 64;
+        tc += ((size_t)M[i] * N[i] + 63) / 64 * 64;
+    }
+    float *A, *B, *C;
+    HIP_CHECK(hipMalloc(&A, ta * 4));
+    HIP_CHECK(hipMalloc(&B, tb * 4));
+    HIP_CHECK(hipMalloc(&C, tc * 4));
+    if (hbs_fill_random(A, (int64_t)ta, HBS_GEN_UNIFORM, 1, 0, nullptr) ||
+        hbs_fill_random(B, (int64_t)tb, HBS_GEN_UNIFORM, 2, 0, nullptr))
+        return 2;
+    std::vector<hbs_gemm_problem> probs(count);
+    size_t oa = 0, ob = 0, oc = 0;
+    std::map<std::tuple<int, int, int>, double> share;
+    double total = 0.0;
+    for (int i = 0; i < count; ++i) {
+        probs[i] = hbs_gemm_problem{A + oa, B + ob, C + oc, M[i], N[i], K[i], K[i], N[i], N[i]};
+        oa += ((size_t)M[i] * K[i] + 63) / 64 * 64;
+        ob += ((size_t)K[i] * N[i] + 63) / 64 * 64;
+        oc += ((size_t)M[i] * N[i] + 63) / 64 * 64;
+        const double fl = 2.0 * M[i] * N[i] * K[i];
+        share[std::make_tuple(size_class(M[i]), size_class(N[i]), size_class(K[i]))] += fl;
+        total += fl;
+    }
+    std::vector<std::pair<double, std::tuple<int, int, int>>> classes;
+    for (auto &kv : share) classes.push_back({kv.second / total, kv.first});
+    std::sort(classes.begin(), classes.end(), [](auto &a, auto &b) { return a.first > b.first; });
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    std::string tmp = std::string(out_path) + ".probe";
+    std::string merged;
+    for (int prec : precs) {
+        const char *pname = prec == HBS_PREC_FP32 ? "fp32" : "bf16x3";
+        std::vector<Override> ov;
+        const double base = time_group(probs, prec, "", reps, e0, e1);
+        double best = base;
+        printf("%s group of %d: heuristic %.3f ms (%.1f TFLOP/s)\n", pname, count, base,
+               total / (base * 1e-3) / 1e12);
+        fflush(stdout);
+        for (auto &cl : classes) {
+            if (cl.first < 0.003) break;
+            const int mc = std::get<0>(cl.second), nc = std::get<1>(cl.second),
+                      kc = std::get<2>(cl.second);
+            Override keep{mc, nc, kc, -1, 0};
+            for (int tile = 0; tile < 4; ++tile) {
+                for (int split : {1, 2, 4}) {
+                    if (split > 1 && kc < 256) continue;
+                    std::vector<Override> trial = ov;
+                    trial.push_back(Override{mc, nc, kc, tile, split});
+                    write_table(tmp.c_str(), pname, trial, nullptr);
+                    const double t = time_group(probs, prec, tmp.c_str(), reps, e0, e1);
+                    if (t < best * 0.99) {
+                        best = t;
+                        keep.tile = tile;
+                        keep.split = split;
+                    }
+                }
+            }
+            if (keep.tile >= 0) ov.push_back(keep);
+            printf("%s class M%d N%d K%d (%.1f%% of flops): %s -> %.3f ms\n", pname, mc, nc, kc,
+                   100.0 * cl.first, keep.tile >= 0 ? "override" : "heuristic", best);
+            fflush(stdout);
+        }
+        printf("%s tuned: %.3f ms (%.1f TFLOP/s), heuristic %.3f ms, %zu overrides\n", pname, best,
+               total / (best * 1e-3) / 1e12, base, ov.size());
+        fflush(stdout);
+        char line[256];
+        snprintf(line, sizeof line, "# %s: group of %d problems from %s: heuristic %.3f ms, tuned %.3f ms\n",
+                 pname, count, shapes_path, base, best);
+        merged += line;
+        for (const Override &o : ov) {
+            snprintf(line, sizeof line, "%s %d %d %d %d %d 0\n", pname, o.mc, o.nc, o.kc, o.tile, o.split);
+            merged += line;
+        }
+    }
+    remove(tmp.c_str());
+    FILE *out = fopen(out_path, "w");
+    if (!out) {
+        perror(out_path);
+        return 2;
+    }
+    fprintf(out, "# hbsgemm grouped tuning file: prec Mclass Nclass Kclass tile split ns_per_item\n%s",
+            merged.c_str());
+    fclose(out);
+    HIP_CHECK(hipFree(A));
+    HIP_CHECK(hipFree(B));
+    HIP_CHECK(hipFree(C));
+    return 0;
+}
+
 int main(int argc, char **argv) {
     const char *out_path = "tuning.txt";
+    const char *group_path = nullptr;
     std::string prec_arg = "both";
     std::vector<int> classes = {64, 128, 256, 512, 1024, 2048};
     int reps = 5;
@@ -59,10 +240,11 @@ int main(int argc, char **argv) {
         else if (a == "--prec" && i + 1 < argc) prec_arg = argv[++i];
         else if (a == "--classes" && i + 1 < argc) classes = parse_list(argv[++i]);
         else if (a == "--reps" && i + 1 < argc) reps = atoi(argv[++i]);
+        else if (a == "--group" && i + 1 < argc) group_path = argv[++i];
         else {
             fprintf(stderr,
-                    "usage: %s [--out file] [--prec fp32|bf16x3|both] [--classes 64,128,...] "
-                    "[--reps r]\n",
+                    "usage: %s [--out file] [--prec fp32|bf16x3|both] [--reps r]\n"
+                    "          [--classes 64,128,...] | [--group shapes.txt]\n",
                     argv[0]);
             return 2;
         }
@@ -77,6 +259,7 @@ int main(int argc, char **argv) {
     if (prec_arg == "fp32" || prec_arg == "both") precs.push_back(HBS_PREC_FP32);
     if (prec_arg == "bf16x3" || prec_arg == "both") precs.push_back(HBS_PREC_BF16X3);
     if (precs.empty()) return 2;
+    if (group_path) return group_mode(group_path, out_path, precs, reps);
 
     int cus = 256;
     HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0));
