@@ -125,26 +125,54 @@ __global__ __launch_bounds__(WM *WN * 64) void sgemm_f32_kernel(GemmArgs p) {
         }
     };
 
+    // Fragments for one 8-wide k chunk: A from one 128-bit read per 32-row
+    // block, B as four 32-bit reads per 32-col block.  The reads for chunk
+    // kc + 1 are issued before the MFMAs of chunk kc so LDS latency overlaps
+    // matrix-core work.
+    auto load_frag = [&](const float *as, const float *bs, int kc, floatx4 (&a)[G::kFM],
+                         float (&b)[G::kFN][4]) {
+#pragma unroll
+        for (int r = 0; r < G::kFM; ++r) a[r] = *(const floatx4 *)(as + r * 32 * G::kLdA + kc * 8);
+#pragma unroll
+        for (int c = 0; c < G::kFN; ++c)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) b[c][q] = bs[(kc * 8 + q) * G::kLdB + c * 32];
+    };
+    auto mfma_chunk = [&](const floatx4 (&a)[G::kFM], const float (&b)[G::kFN][4]) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int r = 0; r < G::kFM; ++r)
+#pragma unroll
+                for (int c = 0; c < G::kFN; ++c)
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[r][q], b[c][q], acc[r][c], 0, 0, 0);
+    };
+    // Scheduling hint: spread the next chunk's LDS reads (kFM 128-bit and
+    // 2 kFN paired 32-bit reads) between this chunk's 4 kFM kFN MFMAs.
+    auto interleave = [&]() {
+        constexpr int kReads = G::kFM + 2 * G::kFN;
+        constexpr int kMfma = 4 * G::kFM * G::kFN;
+        constexpr int kPer = kMfma / kReads > 0 ? kMfma / kReads : 1;
+#pragma unroll
+        for (int i = 0; i < kReads; ++i) {
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);     // one LDS read
+            __builtin_amdgcn_sched_group_barrier(0x008, kPer, 0);  // then MFMAs
+        }
+    };
     auto compute = [&](int buf) {
         const float *as = As + buf * G::kStageA + (wm * G::kWTM + l32) * G::kLdA + 4 * h;
         const float *bs = Bs + buf * G::kStageB + 4 * h * G::kLdB + wn * G::kWTN + l32;
+        floatx4 a0[G::kFM], a1[G::kFM];
+        float b0[G::kFN][4], b1[G::kFN][4];
+        load_frag(as, bs, 0, a0, b0);
 #pragma unroll
-        for (int kc = 0; kc < BK / 8; ++kc) {
-            floatx4 a[G::kFM];
-            float b[G::kFN][4];
-#pragma unroll
-            for (int r = 0; r < G::kFM; ++r) a[r] = *(const floatx4 *)(as + r * 32 * G::kLdA + kc * 8);
-#pragma unroll
-            for (int c = 0; c < G::kFN; ++c)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) b[c][q] = bs[(kc * 8 + q) * G::kLdB + c * 32];
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-#pragma unroll
-                for (int r = 0; r < G::kFM; ++r)
-#pragma unroll
-                    for (int c = 0; c < G::kFN; ++c)
-                        acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[r][q], b[c][q], acc[r][c], 0, 0, 0);
+        for (int kc = 0; kc < BK / 8; kc += 2) {
+            if (kc + 1 < BK / 8) load_frag(as, bs, kc + 1, a1, b1);
+            mfma_chunk(a0, b0);
+            interleave();
+            if (kc + 2 < BK / 8) load_frag(as, bs, kc + 2, a0, b0);
+            if (kc + 1 < BK / 8) mfma_chunk(a1, b1);
+            interleave();
         }
     };
 
