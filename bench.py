@@ -213,7 +213,9 @@ def setup_grouped(args, hb, torch):
     cfg = {"problems": args.problems, "plan": info,
            "bytes_algorithmic": float(sum(4 * (M * K + K * N + M * N) for M, N, K in shapes)),
            "tuning": os.path.relpath(tuning, ROOT) if tuning else None}
-    return step, flops, workload, {"verify": verify, "config": cfg, "keep": (plan, graph)}
+    return step, flops, workload, {
+        "verify": verify, "config": cfg, "keep": (plan, graph),
+        "traffic": measured_traffic(args.variant, f"{args.problems} problems seed {args.seed}")}
 
 
 def setup_fp8res(args, hb, torch):
@@ -239,8 +241,9 @@ def setup_fp8res(args, hb, torch):
 
     workload = (f"FP8-e4m3 per-128x128-tile scales + BF16 residual GEMM, M=N=16384, K=4096, "
                 f"normal(0,1) with 1% x1e3 outliers seed {args.seed}")
-    return step, 2.0 * M * N * K, workload, {"verify": verify,
-                                             "config": {"M": M, "N": N, "K": K}}
+    return step, 2.0 * M * N * K, workload, {
+        "verify": verify, "config": {"M": M, "N": N, "K": K},
+        "traffic": measured_traffic("fp8res", f"M={M} N={N} K={K}")}
 
 
 def main():
