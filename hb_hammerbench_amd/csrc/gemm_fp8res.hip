@@ -127,7 +127,9 @@ __global__ __launch_bounds__(256) void quantize_kernel(QuantArgs a) {
 // GEMM: 256 x 128 output tile, 8 waves as 4 x 2 of 64 x 64, BK = 64 (one FP8
 // MFMA step and four BF16 steps per stage), LDS double buffer.  LDS rows are
 // unpadded; 16-byte chunks are XOR-swizzled by row so that the 16 lanes of a
-// 128-bit read touch 16 distinct bank groups.
+// 128-bit read touch 16 distinct bank groups.  Each stage also carries the
+// three tile exponents it needs (two A row tiles, one B column tile), loaded
+// with the stage's data, so that no step waits on a separate memory load.
 constexpr int kBM = 256, kBN = 128, kBK = 64;
 constexpr int kWM = 4, kWN = 2, kThreads = kWM * kWN * 64;
 constexpr int kFM = 2, kFN = 2;
@@ -137,7 +139,8 @@ constexpr int kA8 = 0;
 constexpr int kB8 = kA8 + kBM * kRow8;
 constexpr int kAR = kB8 + kBN * kRow8;
 constexpr int kBR = kAR + kBM * kRowR;
-constexpr int kStage = kBR + kBN * kRowR;  // bytes per stage
+constexpr int kExp = kBR + kBN * kRowR;    // 3 int32 tile exponents (+ pad)
+constexpr int kStage = kExp + 16;          // bytes per stage
 // 16-byte chunks per thread per stage
 constexpr int kCA8 = kBM * 4 / kThreads;   // 2
 constexpr int kCB8 = kBN * 4 / kThreads;   // 1
@@ -187,7 +190,12 @@ __global__ __launch_bounds__(kThreads) void gemm_fp8res_kernel(GemmArgs p) {
         for (int c = 0; c < kFN; ++c) acc[r][c] = floatx16{0};
 
     floatx4 ra8[kCA8], rb8[kCB8], rar[kCAR], rbr[kCBR];
+    int rexp = 0;
+    // lanes 0, 1: exponents of the two 128-row A tiles; lane 2: the B column tile
+    const int *exp_src = tid < 2 ? p.eA + (int64_t)(m0 / kQT + tid) * ktiles
+                                 : p.eB + (int64_t)(n0 / kQT) * ktiles;
     auto gload = [&](int k0) {
+        if (tid < 3) rexp = exp_src[k0 / kQT];
 #pragma unroll
         for (int i = 0; i < kCA8; ++i) {
             const int f = tid + i * kThreads, row = f >> 2, c = f & 3;
@@ -211,6 +219,7 @@ __global__ __launch_bounds__(kThreads) void gemm_fp8res_kernel(GemmArgs p) {
     };
     auto sstore = [&](int buf) {
         uint8_t *st = smem + buf * kStage;
+        if (tid < 3) ((int *)(st + kExp))[tid] = rexp;
 #pragma unroll
         for (int i = 0; i < kCA8; ++i) {
             const int f = tid + i * kThreads, row = f >> 2, c = f & 3;
@@ -234,14 +243,13 @@ __global__ __launch_bounds__(kThreads) void gemm_fp8res_kernel(GemmArgs p) {
     };
 
     const int arow = wm * 64 + l32, brow = wn * 64 + l32;
-    auto compute = [&](int buf, int kt) {
+    auto compute = [&](int buf) {
         const uint8_t *st = smem + buf * kStage;
-        const int kq = (kt * kBK) / kQT;
-        const int ea = p.eA[((m0 + wm * 64) / kQT) * ktiles + kq];
-        const int eb = p.eB[((n0 + wn * 64) / kQT) * ktiles + kq];
+        const int ea = ((const int *)(st + kExp))[wm >> 1];
+        const int eb = ((const int *)(st + kExp))[2];
         // FP8: lane half h holds k = 32h .. 32h + 31 (chunks 2h, 2h + 1)
+        intx8 a8[kFM], b8[kFN];
         {
-            intx8 a8[kFM], b8[kFN];
 #pragma unroll
             for (int r = 0; r < kFM; ++r) {
                 const int row = arow + r * 32;
@@ -267,25 +275,25 @@ __global__ __launch_bounds__(kThreads) void gemm_fp8res_kernel(GemmArgs p) {
                     acc[r][c] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
                         a8[r], b8[c], acc[r][c], 0, 0, 0, ea + 127, 0, eb + 127);
         }
-        // BF16 residual: Ar.Bq + Aq.Br, four 16-deep steps; lane half h holds
-        // k = 16 st + 8h .. +7 (residual chunk 2 st + h; e4m3 chunk st, bytes 8h..)
+        // BF16 residual: Ar.Bq + Aq.Br, four 16-deep steps.  In step s lane
+        // half h supplies k = 32h + 8s .. +7 for both operands -- a permutation
+        // of the BF16 instruction's k slots applied to A and B alike, so the
+        // products pair up unchanged -- which lets q come from the e4m3 bytes
+        // already in a8 / b8 (dwords 2s, 2s + 1) and the residual from chunk
+        // 4h + s.
         const float sa = ldexpf(1.0f, ea), sb = ldexpf(1.0f, eb);
 #pragma unroll
         for (int s = 0; s < kBK / 16; ++s) {
             bf16x8 aq[kFM], ar[kFM], bq[kFN], br[kFN];
 #pragma unroll
             for (int r = 0; r < kFM; ++r) {
-                const int row = arow + r * 32;
-                ar[r] = *(const bf16x8 *)(st + kAR + swr(row, 2 * s + h));
-                const uint2 w = *(const uint2 *)(st + kA8 + sw8(row, s) + 8 * h);
-                aq[r] = fp8x8_to_bf16(w.x, w.y, sa);
+                ar[r] = *(const bf16x8 *)(st + kAR + swr(arow + r * 32, 4 * h + s));
+                aq[r] = fp8x8_to_bf16(a8[r][2 * s], a8[r][2 * s + 1], sa);
             }
 #pragma unroll
             for (int c = 0; c < kFN; ++c) {
-                const int row = brow + c * 32;
-                br[c] = *(const bf16x8 *)(st + kBR + swr(row, 2 * s + h));
-                const uint2 w = *(const uint2 *)(st + kB8 + sw8(row, s) + 8 * h);
-                bq[c] = fp8x8_to_bf16(w.x, w.y, sb);
+                br[c] = *(const bf16x8 *)(st + kBR + swr(brow + c * 32, 4 * h + s));
+                bq[c] = fp8x8_to_bf16(b8[c][2 * s], b8[c][2 * s + 1], sb);
             }
 #pragma unroll
             for (int r = 0; r < kFM; ++r)
@@ -304,7 +312,7 @@ __global__ __launch_bounds__(kThreads) void gemm_fp8res_kernel(GemmArgs p) {
     for (int kt = 0; kt < KT; ++kt) {
         const int cur = kt & 1;
         if (kt + 1 < KT) gload((kt + 1) * kBK);
-        compute(cur, kt);
+        compute(cur);
         if (kt + 1 < KT) sstore(cur ^ 1);
         __syncthreads();
     }

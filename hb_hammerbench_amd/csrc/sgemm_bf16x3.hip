@@ -267,6 +267,201 @@ __global__ __launch_bounds__(G::kThreads) void gemm_bf16x3_kernel(GemmArgs p) {
     }
 }
 
+// hi and lo planes of two floats, each as one dword holding two bf16 (x0 low).
+__device__ __forceinline__ void split_pk(float x0, float x1, uint32_t &hi, uint32_t &lo) {
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    typedef __bf16 b2 __attribute__((ext_vector_type(2)));
+    const f2 v = {x0, x1};
+    const b2 h = __builtin_convertvector(v, b2);
+    const b2 l = __builtin_convertvector(v - __builtin_convertvector(h, f2), b2);
+    hi = __builtin_bit_cast(uint32_t, h);
+    lo = __builtin_bit_cast(uint32_t, l);
+}
+
+struct FusedArgs {
+    const float *A, *B;
+    float *C;
+    int M, N, K, lda, ldb, ldc;
+    int64_t sA, sB, sC;
+    int tiles_m, tiles_n, group_m;
+};
+
+// The same GEMM reading float32 A and B directly: each workgroup loads its
+// A (rows x 32 k) and B (32 k x cols) tiles, splits every element in
+// registers and stores the split planes to LDS in the packed row layout the
+// packed-operand kernel uses, so the MFMA sequence and the result are
+// identical to pack + gemm_bf16x3_kernel, with no pre-pass and no workspace.
+//   A item: one row, 8 consecutive k (two 16-byte loads).
+//   B item: one column, 8 consecutive k (8 dword loads; the 64 lanes of a wave
+//           read 64 adjacent columns, so each load is one 256-byte row segment).
+// EDGE: ragged M, N or K, or A not 16-byte loadable; out-of-range elements are
+// zero.
+template <class G, bool EDGE>
+__global__ __launch_bounds__(G::kThreads) void gemm_bf16x3_fused_kernel(FusedArgs p) {
+    constexpr int kBM = G::BM, kBN = G::BN, kBK = G::BK, kWM = G::WM, kThreads = G::kThreads;
+    constexpr int kWTM = G::kWTM, kWTN = G::kWTN, kFM = G::kFM, kFN = G::kFN;
+    constexpr int kLdsRow = G::kLdsRow, kStage = G::kStage;
+    constexpr int kGroups = kBK / 8;                       // 8-k groups per tile row
+    constexpr int kItemsA = kBM * kGroups / kThreads;
+    constexpr int kItemsB = kBN * kGroups / kThreads;
+    static_assert(kBM * kGroups % kThreads == 0 && kBN * kGroups % kThreads == 0, "items");
+    static_assert(kBN % 64 == 0 && kThreads % kBN == 0, "B items: one k group per wave");
+    __shared__ __attribute__((aligned(16))) uint16_t smem[2 * kStage];
+
+    const int tiles = p.tiles_m * p.tiles_n;
+    const int bid = hbs::xcd_remap(blockIdx.x, gridDim.x);
+    const int bs = bid / tiles;
+    int tm, tn;
+    hbs::tile_coords(bid - bs * tiles, p.tiles_m, p.tiles_n, p.group_m, tm, tn);
+    const int m0 = tm * kBM, n0 = tn * kBN;
+    const float *__restrict__ Ag = p.A + bs * p.sA + (int64_t)m0 * p.lda;
+    const float *__restrict__ Bg = p.B + bs * p.sB + n0;
+    float *__restrict__ Cg = p.C + bs * p.sC;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm = wave % kWM, wn = wave / kWM;
+    const int l32 = lane & 31, h = lane >> 5;
+
+    floatx16 acc[kFM][kFN];
+#pragma unroll
+    for (int r = 0; r < kFM; ++r)
+#pragma unroll
+        for (int c = 0; c < kFN; ++c) acc[r][c] = floatx16{0};
+
+    float ra[kItemsA][8], rb[kItemsB][8];
+
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < kItemsA; ++i) {
+            const int f = tid + i * kThreads;
+            const int row = f / kGroups, gk = k0 + (f % kGroups) * 8;
+            const float *x = Ag + (int64_t)row * p.lda + gk;
+            if (!EDGE) {
+                const floatx4 v0 = *(const floatx4 *)x, v1 = *(const floatx4 *)(x + 4);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    ra[i][j] = v0[j];
+                    ra[i][j + 4] = v1[j];
+                }
+            } else {
+                const bool rok = m0 + row < p.M;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) ra[i][j] = (rok && gk + j < p.K) ? x[j] : 0.0f;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < kItemsB; ++i) {
+            const int f = tid + i * kThreads;
+            const unsigned col = f % kBN;
+            // the k group is the same for the whole wave (kBN is a multiple of 64)
+            // and each k row pointer is uniform: scalar base + 32-bit lane offset
+            const int gk = k0 + __builtin_amdgcn_readfirstlane(f / kBN) * 8;
+            const bool cok = !EDGE || n0 + col < p.N;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const float *rowp = Bg + (int64_t)(gk + j) * p.ldb;
+                rb[i][j] = (!EDGE || (cok && gk + j < p.K)) ? rowp[col] : 0.0f;
+            }
+        }
+    };
+    auto sstore = [&](int buf) {
+        uint16_t *as = smem + buf * kStage;
+        uint16_t *bsm = as + kBM * kLdsRow;
+#pragma unroll
+        for (int i = 0; i < kItemsA; ++i) {
+            const int f = tid + i * kThreads;
+            const int row = f / kGroups, g = f % kGroups;
+            uint32_t hi[4], lo[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) split_pk(ra[i][2 * j], ra[i][2 * j + 1], hi[j], lo[j]);
+            uint16_t *d = as + row * kLdsRow + g * 16;
+            *(uint4 *)d = uint4{hi[0], hi[1], hi[2], hi[3]};
+            *(uint4 *)(d + 8) = uint4{lo[0], lo[1], lo[2], lo[3]};
+        }
+#pragma unroll
+        for (int i = 0; i < kItemsB; ++i) {
+            const int f = tid + i * kThreads;
+            const int col = f % kBN, g = f / kBN;
+            uint32_t hi[4], lo[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) split_pk(rb[i][2 * j], rb[i][2 * j + 1], hi[j], lo[j]);
+            uint16_t *d = bsm + col * kLdsRow + g * 16;
+            *(uint4 *)d = uint4{hi[0], hi[1], hi[2], hi[3]};
+            *(uint4 *)(d + 8) = uint4{lo[0], lo[1], lo[2], lo[3]};
+        }
+    };
+    auto compute = [&](int buf) {
+        const uint16_t *as = smem + buf * kStage + (wm * kWTM + l32) * kLdsRow;
+        const uint16_t *bsm = smem + buf * kStage + kBM * kLdsRow + (wn * kWTN + l32) * kLdsRow;
+#pragma unroll
+        for (int st = 0; st < kBK / 16; ++st) {
+            const int off = (2 * st + h) * 16;  // this lane half's 8-k group
+            bf16x8 ahi[kFM], alo[kFM], bhi[kFN], blo[kFN];
+#pragma unroll
+            for (int r = 0; r < kFM; ++r) {
+                ahi[r] = *(const bf16x8 *)(as + r * 32 * kLdsRow + off);
+                alo[r] = *(const bf16x8 *)(as + r * 32 * kLdsRow + off + 8);
+            }
+#pragma unroll
+            for (int c = 0; c < kFN; ++c) {
+                bhi[c] = *(const bf16x8 *)(bsm + c * 32 * kLdsRow + off);
+                blo[c] = *(const bf16x8 *)(bsm + c * 32 * kLdsRow + off + 8);
+            }
+#pragma unroll
+            for (int r = 0; r < kFM; ++r)
+#pragma unroll
+                for (int c = 0; c < kFN; ++c) {
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[r], bhi[c], acc[r][c], 0, 0, 0);
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[r], blo[c], acc[r][c], 0, 0, 0);
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[r], bhi[c], acc[r][c], 0, 0, 0);
+                }
+        }
+    };
+
+    const int KT = (p.K + kBK - 1) / kBK;
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    for (int kt = 0; kt < KT; ++kt) {
+        const int cur = kt & 1;
+        if (kt + 1 < KT) gload((kt + 1) * kBK);
+        compute(cur);
+        if (kt + 1 < KT) sstore(cur ^ 1);
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int r = 0; r < kFM; ++r) {
+#pragma unroll
+        for (int c = 0; c < kFN; ++c) {
+            const int col = n0 + wn * kWTN + c * 32 + l32;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int row = m0 + wm * kWTM + r * 32 + 8 * (q >> 2) + 4 * h + (q & 3);
+                if (!EDGE || (row < p.M && col < p.N)) Cg[(int64_t)row * p.ldc + col] = acc[r][c][q];
+            }
+        }
+    }
+}
+
+template <class G>
+int launch_fused(const float *A, const float *B, float *C, int M, int N, int K, int lda, int ldb,
+                 int ldc, int64_t sA, int64_t sB, int64_t sC, int batch, int group_m,
+                 hipStream_t stream) {
+    FusedArgs g{A, B, C, M, N, K, lda, ldb, ldc, sA, sB, sC,
+                (M + G::BM - 1) / G::BM, (N + G::BN - 1) / G::BN, group_m};
+    const int64_t gblocks = (int64_t)g.tiles_m * g.tiles_n * batch;
+    if (gblocks > 0x7fffffffLL) return HBS_ERR_ARG;
+    const bool interior = M % G::BM == 0 && N % G::BN == 0 && K % G::BK == 0;
+    const bool vec_a = hbs::aligned16(A) && lda % 4 == 0 && sA % 4 == 0;
+    if (interior && vec_a)
+        gemm_bf16x3_fused_kernel<G, false><<<(unsigned)gblocks, G::kThreads, 0, stream>>>(g);
+    else
+        gemm_bf16x3_fused_kernel<G, true><<<(unsigned)gblocks, G::kThreads, 0, stream>>>(g);
+    return hbs::launch_status();
+}
+
 inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
 struct Layout {
