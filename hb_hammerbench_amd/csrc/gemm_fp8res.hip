@@ -24,37 +24,10 @@
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef int intx8 __attribute__((ext_vector_type(8)));
-typedef unsigned short ushort8v __attribute__((ext_vector_type(8)));
 
 namespace {
 
 constexpr int kQT = 128;  // quantisation tile edge
-
-__device__ __forceinline__ uint16_t bf16_rne(float x) {
-    uint32_t u = __float_as_uint(x);
-    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (uint16_t)((u >> 16) | 0x40);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
-
-// e4m3 (bias 7, max 448) round-to-nearest-even of v, |v| <= 448.
-__device__ __forceinline__ float e4m3_round(float v) {
-    const float a = fabsf(v);
-    if (a == 0.0f) return v;
-    if (a < 0x1p-6f) return rintf(v * 512.0f) * 0x1p-9f;  // subnormal quantum 2^-9
-    const int E = (int)((__float_as_uint(a) >> 23) & 0xff) - 127;  // floor(log2 a)
-    return ldexpf(rintf(ldexpf(v, 3 - E)), E - 3);                  // 3 fraction bits
-}
-
-__device__ __forceinline__ uint32_t e4m3_bits(float q) {
-    const uint32_t s = (__float_as_uint(q) >> 31) << 7;
-    const float a = fabsf(q);
-    if (a == 0.0f) return s;
-    if (a < 0x1p-6f) return s | (uint32_t)(a * 512.0f);
-    const uint32_t u = __float_as_uint(a);
-    const uint32_t ef = ((u >> 23) & 0xff) - 127 + 7;
-    return s | (ef << 3) | ((u >> 20) & 7u);
-}
 
 __device__ __forceinline__ int tile_exponent(float amax) {
     if (!(amax >= 0x1p-100f)) return 0;  // all-zero or negligible tile: scale 1
@@ -67,60 +40,84 @@ __device__ __forceinline__ int tile_exponent(float amax) {
 struct QuantArgs {
     const float *X;
     int rows, cols, ld;  // logical matrix as stored (A: M x K; B: K x N)
+    int vec;             // X and ld allow 16-byte loads
     int Rp, Kp;          // padded output rows / K
     uint8_t *X8;
     uint16_t *Xr;
     int *ex;
 };
 
-// One workgroup per 128 x 128 quantisation tile.  TRANS = false: rows of X are
-// output rows (A).  TRANS = true: X is K x N and output rows are its columns (B).
+// One workgroup of 256 threads per 128 x 128 quantisation tile, held in
+// registers: load i of wave w covers tile rows 32w + 2i + lane/32 and columns
+// 4 (lane % 32) .. +3 of the stored matrix (512 contiguous bytes per row).
+// TRANS = false: stored rows are output rows (A).  TRANS = true: X is K x N
+// and output rows are its columns (B); the tile is transposed through LDS so
+// that each thread then holds 4 consecutive k of one output row, as for A.
+// Conversions: v_cvt_pk_fp8_f32 (round to nearest even to OCP e4m3, |x| <= 448
+// after scaling), v_cvt_scalef32_pk_f32_fp8 (exact decode times 2^e) and
+// v_cvt_pk_bf16_f32 (round to nearest even) for the residual.
 template <bool TRANS>
 __global__ __launch_bounds__(256) void quantize_kernel(QuantArgs a) {
-    __shared__ float t[kQT][kQT + 1];
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    typedef __bf16 b2 __attribute__((ext_vector_type(2)));
+    __shared__ float t[TRANS ? kQT : 1][kQT + 1];
     __shared__ float red[4];
     const int tiles_k = a.Kp / kQT;
     const int tr = blockIdx.x / tiles_k, tk = blockIdx.x % tiles_k;
     const int r0 = tr * kQT, k0 = tk * kQT;
-    const int tid = threadIdx.x;
-    // stage the tile as t[output row][k]
-    for (int e = tid; e < kQT * kQT; e += 256) {
-        const int i = e / kQT, j = e % kQT;  // i: row of X, j: col of X (coalesced)
-        if (!TRANS) {
-            const int gr = r0 + i, gk = k0 + j;
-            t[i][j] = (gr < a.rows && gk < a.cols) ? a.X[(int64_t)gr * a.ld + gk] : 0.0f;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int c4 = (lane & 31) * 4;
+    const int rsub = 32 * wave + (lane >> 5);  // tile row of load 0; load i adds 2i
+    const int sr0 = TRANS ? k0 : r0, sc0 = TRANS ? r0 : k0;
+
+    float v[16][4];
+    float m = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int sr = sr0 + rsub + 2 * i, sc = sc0 + c4;
+        const float *x = a.X + (int64_t)sr * a.ld + sc;
+        if (a.vec && sr < a.rows && sc + 4 <= a.cols) {
+            const floatx4 q = *(const floatx4 *)x;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) v[i][j] = q[j];
         } else {
-            const int gk = k0 + i, gn = r0 + j;
-            t[j][i] = (gk < a.rows && gn < a.cols) ? a.X[(int64_t)gk * a.ld + gn] : 0.0f;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) v[i][j] = (sr < a.rows && sc + j < a.cols) ? x[j] : 0.0f;
         }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) m = fmaxf(m, fabsf(v[i][j]));
+    }
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    if (lane == 0) red[wave] = m;
+    if (TRANS) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) t[c4 + j][rsub + 2 * i] = v[i][j];
     }
     __syncthreads();
-    // amax over the tile
-    float m = 0.0f;
-    const int row = tid >> 1, half = tid & 1;
-    for (int j = 0; j < 64; ++j) m = fmaxf(m, fabsf(t[row][half * 64 + j]));
-    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-    if ((tid & 63) == 0) red[tid >> 6] = m;
-    __syncthreads();
+    if (TRANS) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) v[i][j] = t[rsub + 2 * i][c4 + j];
+    }
     const float amax = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
     const int e = tile_exponent(amax);
     if (tid == 0) a.ex[tr * tiles_k + tk] = e;
-    // quantise this thread's 64 values: row `row`, k = k0 + half*64 .. +63
-    const int64_t base = (int64_t)(r0 + row) * a.Kp + k0 + half * 64;
-#pragma unroll 1
-    for (int g = 0; g < 8; ++g) {
-        ushort8v rv;
-        uint32_t b8[2] = {0, 0};
+    const float inv = ldexpf(1.0f, -e), scl = ldexpf(1.0f, e);
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const float x = t[row][half * 64 + g * 8 + j];
-            const float s = e4m3_round(ldexpf(x, -e));
-            const float q = ldexpf(s, e);
-            rv[j] = bf16_rne(x - q);
-            b8[j >> 2] |= e4m3_bits(s) << (8 * (j & 3));
-        }
-        *(ushort8v *)(a.Xr + base + g * 8) = rv;
-        *(uint2 *)(a.X8 + base + g * 8) = make_uint2(b8[0], b8[1]);
+    for (int i = 0; i < 16; ++i) {
+        int w = __builtin_amdgcn_cvt_pk_fp8_f32(v[i][0] * inv, v[i][1] * inv, 0, false);
+        w = __builtin_amdgcn_cvt_pk_fp8_f32(v[i][2] * inv, v[i][3] * inv, w, true);
+        const f2 q01 = __builtin_amdgcn_cvt_scalef32_pk_f32_fp8(w, scl, false);
+        const f2 q23 = __builtin_amdgcn_cvt_scalef32_pk_f32_fp8(w, scl, true);
+        const b2 r01 = __builtin_convertvector(f2{v[i][0], v[i][1]} - q01, b2);
+        const b2 r23 = __builtin_convertvector(f2{v[i][2], v[i][3]} - q23, b2);
+        const int64_t o = (int64_t)(r0 + rsub + 2 * i) * a.Kp + k0 + c4;
+        *(int *)(a.X8 + o) = w;
+        *(uint2 *)(a.Xr + o) =
+            make_uint2(__builtin_bit_cast(uint32_t, r01), __builtin_bit_cast(uint32_t, r23));
     }
 }
 
@@ -375,13 +372,13 @@ extern "C" int hbs_gemm_fp8res(const float *A, const float *B, float *C, int M, 
     hipStream_t s = (hipStream_t)stream;
     char *ws = (char *)workspace;
 
-    QuantArgs qa{A, M, K, lda, L.Mp, L.Kp, (uint8_t *)(ws + L.a8), (uint16_t *)(ws + L.ar),
-                 (int *)(ws + L.ea)};
+    QuantArgs qa{A, M, K, lda, hbs::aligned16(A) && lda % 4 == 0, L.Mp, L.Kp,
+                 (uint8_t *)(ws + L.a8), (uint16_t *)(ws + L.ar), (int *)(ws + L.ea)};
     quantize_kernel<false><<<(L.Mp / kQT) * (L.Kp / kQT), 256, 0, s>>>(qa);
     int rc = hbs::launch_status();
     if (rc) return rc;
-    QuantArgs qb{B, K, N, ldb, L.Np, L.Kp, (uint8_t *)(ws + L.b8), (uint16_t *)(ws + L.br),
-                 (int *)(ws + L.eb)};
+    QuantArgs qb{B, K, N, ldb, hbs::aligned16(B) && ldb % 4 == 0, L.Np, L.Kp,
+                 (uint8_t *)(ws + L.b8), (uint16_t *)(ws + L.br), (int *)(ws + L.eb)};
     quantize_kernel<true><<<(L.Np / kQT) * (L.Kp / kQT), 256, 0, s>>>(qb);
     rc = hbs::launch_status();
     if (rc) return rc;
