@@ -131,17 +131,13 @@ def setup_square(args, hb, torch):
     c = torch.empty_like(a)
     hb.fill_random(a, n_total, hb.GEN_UNIFORM, args.seed)
     hb.fill_random(b, n_total, hb.GEN_UNIFORM, args.seed + 1)
-    ws = None
-    if args.variant == "bf16x3":
-        ws = torch.empty(max(1, hb.bf16x3_workspace_size(N, NITER)), dtype=torch.uint8,
-                         device="cuda")
     torch.cuda.synchronize()
 
     def step():
         if args.variant == "fp32":
             hb.sgemm(a, b, c, N, NITER)
         else:
-            hb.sgemm_bf16x3(a, b, c, N, NITER, workspace=ws)
+            hb.sgemm_bf16x3(a, b, c, N, NITER)
 
     def verify():
         it = NITER - 1

@@ -15,7 +15,7 @@ import os
 
 __all__ = [
     "HbsError", "lib", "library_path", "sgemm", "sgemm_strided", "sgemm_bf16x3",
-    "bf16x3_workspace_size", "fill_pattern", "fill_random", "verify_f64",
+    "fill_pattern", "fill_random", "verify_f64",
     "GEN_UNIFORM", "GEN_LOGUNIFORM", "GEN_NORMAL_OUTLIER", "PREC_FP32", "PREC_BF16X3",
     "GroupedPlan", "Graph", "gemm_fp8res", "fp8res_workspace_size", "gemm_bf16x3",
 ]
@@ -74,8 +74,7 @@ SIGNATURES = {
     "hbs_version": (_i, []),
     "hbs_sgemm": (_i, [_vp, _vp, _vp, _i, _i, _vp]),
     "hbs_sgemm_strided": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i64, _i64, _i64, _i, _vp]),
-    "hbs_sgemm_bf16x3_workspace_size": (_sz, [_i, _i]),
-    "hbs_sgemm_bf16x3": (_i, [_vp, _vp, _vp, _i, _i, _vp, _sz, _vp]),
+    "hbs_sgemm_bf16x3": (_i, [_vp, _vp, _vp, _i, _i, _vp]),
     "hbs_grouped_plan_create": (_i, [ctypes.POINTER(Problem), _i, _i, ctypes.c_char_p,
                                      ctypes.POINTER(_vp)]),
     "hbs_grouped_plan_create_forced": (_i, [ctypes.POINTER(Problem), _i, _i, _i, _i,
@@ -86,8 +85,7 @@ SIGNATURES = {
     "hbs_grouped_graph_create": (_i, [_vp, ctypes.POINTER(_vp)]),
     "hbs_graph_launch": (_i, [_vp, _vp]),
     "hbs_graph_destroy": (_i, [_vp]),
-    "hbs_gemm_bf16x3_workspace_size": (_sz, [_i, _i, _i]),
-    "hbs_gemm_bf16x3": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _vp, _sz, _vp]),
+    "hbs_gemm_bf16x3": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _vp]),
     "hbs_gemm_fp8res_workspace_size": (_sz, [_i, _i, _i]),
     "hbs_gemm_fp8res": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _vp, _sz, _vp]),
     "hbs_fill_pattern": (_i, [_vp, _vp, _i64, _i64, _vp]),
@@ -147,37 +145,18 @@ def sgemm_strided(A, B, C, M, N, K, lda, ldb, ldc, strideA=0, strideB=0, strideC
            "hbs_sgemm_strided")
 
 
-def bf16x3_workspace_size(N, niter=1):
-    return int(lib().hbs_sgemm_bf16x3_workspace_size(N, niter))
-
-
-def sgemm_bf16x3(mat1, mat2, result, N, niter=1, workspace=None, stream=None):
-    """BF16x3 split-precision SGEMM with the hbs_sgemm call shape.
-
-    ``workspace`` is a device buffer of at least bf16x3_workspace_size(N, niter)
-    bytes; when None one is allocated with torch (outside any timed region).
-    """
-    ws_bytes = bf16x3_workspace_size(N, niter)
-    if workspace is None:
-        import torch
-        workspace = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device="cuda")
-    nbytes = workspace.numel() * workspace.element_size() if hasattr(workspace, "numel") else ws_bytes
+def sgemm_bf16x3(mat1, mat2, result, N, niter=1, stream=None):
+    """BF16x3 split-precision SGEMM with the hbs_sgemm call shape (no workspace)."""
     _check(lib().hbs_sgemm_bf16x3(_ptr(mat1), _ptr(mat2), _ptr(result), N, niter,
-                                  _ptr(workspace), nbytes, _stream(stream)),
+                                  _stream(stream)),
            "hbs_sgemm_bf16x3")
 
 
-def gemm_bf16x3(A, B, C, M, N, K, lda=None, ldb=None, ldc=None, workspace=None, stream=None):
-    """Rectangular BF16x3 split-precision GEMM."""
-    ws_bytes = int(lib().hbs_gemm_bf16x3_workspace_size(M, N, K))
-    if workspace is None:
-        import torch
-        workspace = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device="cuda")
-    nbytes = workspace.numel() * workspace.element_size()
+def gemm_bf16x3(A, B, C, M, N, K, lda=None, ldb=None, ldc=None, stream=None):
+    """Rectangular BF16x3 split-precision GEMM (no workspace)."""
     _check(lib().hbs_gemm_bf16x3(_ptr(A), _ptr(B), _ptr(C), M, N, K,
                                  K if lda is None else lda, N if ldb is None else ldb,
-                                 N if ldc is None else ldc, _ptr(workspace), nbytes,
-                                 _stream(stream)),
+                                 N if ldc is None else ldc, _stream(stream)),
            "hbs_gemm_bf16x3")
 
 

@@ -75,20 +75,14 @@ int hbs_sgemm_strided(const float *A, const float *B, float *C, int M, int N, in
 /* matrix cores with FP32 accumulation.  Same call shape as hbs_sgemm.        */
 /* ------------------------------------------------------------------------ */
 
-/* Bytes of workspace hbs_sgemm_bf16x3 needs for (N, niter): the packed hi/lo
- * planes of every slice (about 4 * N * N * 2 bytes per slice, rows and K
- * padded to the tile multiples).  0 for invalid sizes. */
-size_t hbs_sgemm_bf16x3_workspace_size(int N, int niter);
 /* C_i = A_i . B_i for i < niter N x N slices, as hbs_sgemm, on BF16 matrix
- * cores.  `workspace` (16-byte aligned, >= hbs_sgemm_bf16x3_workspace_size)
- * is overwritten; HBS_ERR_WORKSPACE if it is missing or too small. */
-int hbs_sgemm_bf16x3(const float *A, const float *B, float *C, int N, int niter,
-                     void *workspace, size_t workspace_bytes, void *stream);
+ * cores.  One launch; the split happens in registers, so no workspace and no
+ * allocation.  Asynchronous on `stream`. */
+int hbs_sgemm_bf16x3(const float *A, const float *B, float *C, int N, int niter, void *stream);
 /* Rectangular form: C[M x N] = A[M x K] . B[K x N], row-major with leading
- * dimensions lda >= K, ldb >= N, ldc >= N; workspace as above. */
-size_t hbs_gemm_bf16x3_workspace_size(int M, int N, int K);
+ * dimensions lda >= K, ldb >= N, ldc >= N. */
 int hbs_gemm_bf16x3(const float *A, const float *B, float *C, int M, int N, int K, int lda,
-                    int ldb, int ldc, void *workspace, size_t workspace_bytes, void *stream);
+                    int ldb, int ldc, void *stream);
 
 
 /* ------------------------------------------------------------------------ */

@@ -91,18 +91,20 @@ def test_python_wrapper_raises_hbs_error():
     assert ei.value.code == -1
 
 
-def test_bf16x3_workspace_contract():
+def test_bf16x3_argument_checks():
+    """The BF16x3 entries take no workspace; argument errors come before any
+    device call."""
     L = hb.lib()
-    assert L.hbs_sgemm_bf16x3_workspace_size(0, 1) == 0
-    need = L.hbs_sgemm_bf16x3_workspace_size(8192, 10)
-    assert need >= 10 * 4 * 8192 * 8192 * 2  # hi + lo of A and B per slice
-    assert L.hbs_sgemm_bf16x3_workspace_size(100, 1) > 0
-    assert L.hbs_sgemm_bf16x3(P, P, P, 64, 1, None, 1 << 30, None) == -4
-    small = L.hbs_sgemm_bf16x3_workspace_size(64, 1) - 1
-    assert L.hbs_sgemm_bf16x3(P, P, P, 64, 1, P, small, None) == -4
-    assert L.hbs_sgemm_bf16x3(P, P, P, 0, 1, P, 1 << 30, None) == -1
-    assert L.hbs_sgemm_bf16x3(None, P, P, 64, 1, P, 1 << 30, None) == -2
-    assert L.hbs_sgemm_bf16x3(P, P, P, 64, 1, ctypes.c_void_p(24), 1 << 30, None) == -3
+    assert L.hbs_sgemm_bf16x3(P, P, P, 0, 1, None) == -1
+    assert L.hbs_sgemm_bf16x3(P, P, P, 64, 0, None) == -1
+    assert L.hbs_sgemm_bf16x3(P, P, P, 1 << 30, 1 << 20, None) == -1  # N * N * niter overflows
+    assert L.hbs_sgemm_bf16x3(None, P, P, 64, 1, None) == -2
+    assert L.hbs_sgemm_bf16x3(P, P, None, 64, 1, None) == -2
+    assert L.hbs_gemm_bf16x3(P, P, P, 0, 8, 8, 8, 8, 8, None) == -1
+    assert L.hbs_gemm_bf16x3(P, P, P, 8, 8, 8, 7, 8, 8, None) == -1  # lda < K
+    assert L.hbs_gemm_bf16x3(P, P, P, 8, 8, 8, 8, 7, 8, None) == -1  # ldb < N
+    assert L.hbs_gemm_bf16x3(P, P, P, 8, 8, 8, 8, 8, 7, None) == -1  # ldc < N
+    assert L.hbs_gemm_bf16x3(P, None, P, 8, 8, 8, 8, 8, 8, None) == -2
 
 
 def test_grouped_plan_argument_checks():
@@ -133,7 +135,7 @@ def test_grouped_plan_argument_checks():
     assert L.hbs_graph_destroy(None) == 0
 
 
-def test_fp8res_and_rect_bf16x3_argument_checks():
+def test_fp8res_argument_checks():
     L = hb.lib()
     assert L.hbs_gemm_fp8res_workspace_size(0, 8, 8) == 0
     need = L.hbs_gemm_fp8res_workspace_size(16384, 16384, 4096)
@@ -144,6 +146,3 @@ def test_fp8res_and_rect_bf16x3_argument_checks():
     assert L.hbs_gemm_fp8res(P, P, P, 8, 8, 8, 8, 8, 8, None, 1 << 30, None) == -4
     assert L.hbs_gemm_fp8res(P, P, P, 8, 8, 8, 8, 8, 8, P, 16, None) == -4
     assert L.hbs_gemm_fp8res(P, P, P, 8, 8, 8, 8, 8, 8, ctypes.c_void_p(8), 1 << 30, None) == -3
-    assert L.hbs_gemm_bf16x3_workspace_size(8, 0, 8) == 0
-    assert L.hbs_gemm_bf16x3(P, P, P, 8, 8, 8, 8, 8, 7, P, 1 << 30, None) == -1
-    assert L.hbs_gemm_bf16x3(P, P, P, 8, 8, 8, 8, 8, 8, P, 16, None) == -4

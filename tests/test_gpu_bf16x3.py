@@ -66,22 +66,24 @@ def test_bf16x3_within_split_bound(N, niter, kind):
         assert e["e_cw"] <= 1e-5 * np.sqrt(N)
 
 
-def test_bf16x3_packed_planes_match_oracle_split():
-    """Workspace Ap[Mp][Kp/8][hi8|lo8] (A's plane, first in the workspace)
-    equals the oracle's split bit for bit; padding is zero."""
-    N = 300
-    a = dev_fill(N * N, hb.GEN_LOGUNIFORM, 41); b = dev_fill(N * N, hb.GEN_UNIFORM, 42)
-    c = dev_empty(N * N)
-    ws = torch.zeros(hb.bf16x3_workspace_size(N, 1), dtype=torch.uint8, device="cuda")
-    hb.sgemm_bf16x3(a, b, c, N, 1, workspace=ws)
+@pytest.mark.parametrize("N", [300, 512])
+def test_bf16x3_identity_reproduces_oracle_split(N):
+    """With B = I (and with A = I) each output is lo + 0 + hi of the other
+    operand's element, and hi + lo is exact in FP32, so C equals the oracle's
+    split summed in float32 bit for bit: this pins the in-register split of
+    the A path and of the B path (N = 300: ragged tiles; 512: interior)."""
+    x = dev_fill(N * N, hb.GEN_LOGUNIFORM, 41)
+    eye = torch.eye(N, dtype=torch.float32, device="cuda").reshape(-1).contiguous()
+    c1 = dev_empty(N * N); c2 = dev_empty(N * N)
+    hb.sgemm_bf16x3(x, eye, c1, N, 1)
+    hb.sgemm_bf16x3(eye, x, c2, N, 1)
     torch.cuda.synchronize()
-    Mp, Kp = 512, 320
-    Ap = ws[:Mp * Kp * 4].cpu().numpy().view(np.uint16).reshape(Mp, Kp // 8, 2, 8)
-    hi = Ap[:, :, 0, :].reshape(Mp, Kp)
-    lo = Ap[:, :, 1, :].reshape(Mp, Kp)
-    ohi, olo = oracle.split_bf16(a.cpu().numpy().reshape(N, N))
-    assert np.array_equal(hi[:N, :N], ohi) and np.array_equal(lo[:N, :N], olo)
-    assert not hi[N:].any() and not hi[:, N:].any() and not lo[N:].any()
+    xh = x.cpu().numpy().reshape(N, N)
+    hi, lo = oracle.split_bf16(xh)
+    want = oracle.bf16_bits_to_f32(hi) + oracle.bf16_bits_to_f32(lo)
+    assert want.dtype == np.float32 and (want != xh).any()  # the split drops bits
+    assert np.array_equal(c1.cpu().numpy().reshape(N, N), want)
+    assert np.array_equal(c2.cpu().numpy().reshape(N, N), want)
 
 
 def test_bf16x3_pattern_ragged_sizes():
@@ -93,16 +95,14 @@ def test_bf16x3_pattern_ragged_sizes():
         assert np.array_equal(c.cpu().numpy().reshape(N, N), oracle.pattern_slice(N, 0)), N
 
 
-def test_bf16x3_reuses_workspace_across_sizes():
-    big = hb.bf16x3_workspace_size(512, 1)
-    ws = torch.empty(big, dtype=torch.uint8, device="cuda")
+def test_bf16x3_deterministic_across_calls():
     for N in (512, 128):
         a = dev_fill(N * N, hb.GEN_UNIFORM, 1); b = dev_fill(N * N, hb.GEN_UNIFORM, 2)
         c1 = dev_empty(N * N); c2 = dev_empty(N * N)
-        hb.sgemm_bf16x3(a, b, c1, N, 1, workspace=ws)
+        hb.sgemm_bf16x3(a, b, c1, N, 1)
         hb.sgemm_bf16x3(a, b, c2, N, 1)
         torch.cuda.synchronize()
-        assert torch.equal(c1, c2)  # deterministic, independent of workspace history
+        assert torch.equal(c1, c2)
 
 
 def test_bf16x3_fp32_level_error_at_N8192_loguniform():
