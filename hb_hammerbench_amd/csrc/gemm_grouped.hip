@@ -17,10 +17,9 @@
 //  * split-K tiles write FP32 partials; the last of a tile's splits to finish
 //    (an agent-scope counter) adds them in split order -- deterministic -- and
 //    resets the counter to zero for the next run;
-//  * BF16x3 first runs one pre-pass launch that splits every operand into
-//    packed bf16 hi/lo rows in plan memory (B transposed, zero-padded), so the
-//    persistent GEMM streams 16-byte packed chunks with no split work; the two
-//    launches are captured together in the hipGraph.
+//  * BF16x3 tiles read the float32 operands directly and split them into bf16
+//    hi/lo planes in registers on the way to LDS, so both precisions are one
+//    launch with no pre-pass and no operand copies.
 #include <stdlib.h>
 #include <string.h>
 
@@ -36,8 +35,6 @@
 #include "hbs_internal.h"
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef unsigned short ushort4v __attribute__((ext_vector_type(4)));
-typedef unsigned short ushort8v __attribute__((ext_vector_type(8)));
 
 struct hbs_grouped_plan {
     int device;
@@ -54,8 +51,6 @@ struct hbs_grouped_plan {
     const int *d_boff;
     int *d_counters;
     float *d_partials;
-    const void *d_tasks;  // BF16x3 pack tasks
-    int ntasks;
 };
 
 namespace {
@@ -65,18 +60,6 @@ struct Prob {
     const float *B;
     float *C;
     int M, N, K, lda, ldb, ldc;
-    // BF16x3 only: packed hi/lo operands in plan memory, rows padded to 128,
-    // K padded to Kp (a multiple of 32): Ap[Mp][Kp/8][hi8|lo8], Bp[Np][Kp/8][hi8|lo8]
-    // with B transposed (row n holds column n).
-    const uint16_t *Ap;
-    const uint16_t *Bp;
-    int Kp;
-};
-
-// One 64 x 64 packing task of the BF16x3 pre-pass: op 0 packs rows r0..r0+63
-// of A over k0..k0+63; op 1 packs columns r0..r0+63 of B over k0..k0+63.
-struct PackTask {
-    int prob, op, r0, k0;
 };
 
 struct Item {
@@ -99,19 +82,16 @@ constexpr int kThreads = 256;
 // shared helpers
 // ---------------------------------------------------------------------------
 // x = hi + lo with hi = bf16_rne(x), lo = bf16_rne(x - hi); x - hi is exact.
-// v_cvt_pk_bf16_f32 rounds to nearest even, two floats per instruction.
-__device__ __forceinline__ void split2(float x0, float x1, uint16_t &h0, uint16_t &h1,
-                                       uint16_t &l0, uint16_t &l1) {
+// v_cvt_pk_bf16_f32 rounds to nearest even; hi and lo of two floats come back
+// as one dword each (x0 in the low half).
+__device__ __forceinline__ void split_pk(float x0, float x1, uint32_t &hi, uint32_t &lo) {
     typedef float f2 __attribute__((ext_vector_type(2)));
     typedef __bf16 b2 __attribute__((ext_vector_type(2)));
     const f2 v = {x0, x1};
-    const b2 h = __builtin_convertvector(v, b2);
-    const b2 l = __builtin_convertvector(v - __builtin_convertvector(h, f2), b2);
-    const uint32_t hu = __builtin_bit_cast(uint32_t, h), lu = __builtin_bit_cast(uint32_t, l);
-    h0 = (uint16_t)hu;
-    h1 = (uint16_t)(hu >> 16);
-    l0 = (uint16_t)lu;
-    l1 = (uint16_t)(lu >> 16);
+    const b2 hv = __builtin_convertvector(v, b2);
+    const b2 lv = __builtin_convertvector(v - __builtin_convertvector(hv, f2), b2);
+    hi = __builtin_bit_cast(uint32_t, hv);
+    lo = __builtin_bit_cast(uint32_t, lv);
 }
 
 // Epilogue for a 2x2-wave tile of FMxFN 32x32 accumulator blocks per wave.
@@ -289,93 +269,23 @@ __device__ void f32_tile(float *smem, const Prob &p, const Item &it, float *part
 }
 
 // ---------------------------------------------------------------------------
-// BF16x3 pre-pass: split every float into bf16 hi/lo and pack 8-k groups
-// [hi8|lo8], B transposed, zero-padded (one 64 x 64 task per workgroup).
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void grouped_pack_kernel(const Prob *__restrict__ probs,
-                                                           const PackTask *__restrict__ tasks) {
-    __shared__ float t[64][65];
-    const PackTask tk = tasks[blockIdx.x];
-    const Prob p = probs[tk.prob];
-    const int tid = threadIdx.x;
-    const int gpr = p.Kp / 8;  // 8-k groups per packed row
-    if (tk.op == 0) {
-        // A rows: thread -> (row, group) pairs, 8 consecutive k each
-        for (int e = tid; e < 64 * 8; e += 256) {
-            const int row = e >> 3, g = e & 7;
-            const int gr = tk.r0 + row, gk = tk.k0 + g * 8;
-            if (gk >= p.Kp) continue;
-            float v[8];
-            const float *src = p.A + (int64_t)gr * p.lda + gk;
-            if (gr < p.M && gk + 8 <= p.K) {
-                const floatx4 a0 = *(const floatx4 *)src, a1 = *(const floatx4 *)(src + 4);
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    v[j] = a0[j];
-                    v[j + 4] = a1[j];
-                }
-            } else {
-#pragma unroll
-                for (int j = 0; j < 8; ++j) v[j] = (gr < p.M && gk + j < p.K) ? src[j] : 0.0f;
-            }
-            ushort8v hi, lo;
-#pragma unroll
-            for (int j = 0; j < 8; j += 2) {
-                uint16_t h0, h1, l0, l1;
-                split2(v[j], v[j + 1], h0, h1, l0, l1);
-                hi[j] = h0; hi[j + 1] = h1;
-                lo[j] = l0; lo[j + 1] = l1;
-            }
-            uint16_t *dst = (uint16_t *)p.Ap + ((int64_t)gr * gpr + gk / 8) * 16;
-            *(ushort8v *)dst = hi;
-            *(ushort8v *)(dst + 8) = lo;
-        }
-    } else {
-        // B: stage B[k0..k0+63][r0..r0+63] and write columns as packed rows
-        for (int e = tid; e < 64 * 16; e += 256) {
-            const int kk = e >> 4, c4 = (e & 15) * 4;
-            const int gk = tk.k0 + kk, gn = tk.r0 + c4;
-            floatx4 v = {0, 0, 0, 0};
-            if (gk < p.K && gn + 4 <= p.N) {
-                v = *(const floatx4 *)(p.B + (int64_t)gk * p.ldb + gn);
-            } else if (gk < p.K) {
-#pragma unroll
-                for (int j = 0; j < 4; ++j) v[j] = (gn + j < p.N) ? p.B[(int64_t)gk * p.ldb + gn + j] : 0.0f;
-            }
-#pragma unroll
-            for (int j = 0; j < 4; ++j) t[kk][c4 + j] = v[j];
-        }
-        __syncthreads();
-        for (int e = tid; e < 64 * 8; e += 256) {
-            const int nn = e >> 3, g = e & 7;
-            const int gkg = tk.k0 / 8 + g;
-            if (gkg >= gpr) continue;
-            ushort8v hi, lo;
-#pragma unroll
-            for (int j = 0; j < 8; j += 2) {
-                uint16_t h0, h1, l0, l1;
-                split2(t[g * 8 + j][nn], t[g * 8 + j + 1][nn], h0, h1, l0, l1);
-                hi[j] = h0; hi[j + 1] = h1;
-                lo[j] = l0; lo[j + 1] = l1;
-            }
-            uint16_t *dst = (uint16_t *)p.Bp + ((int64_t)(tk.r0 + nn) * gpr + gkg) * 16;
-            *(ushort8v *)dst = hi;
-            *(ushort8v *)(dst + 8) = lo;
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// BF16x3 tile on packed operands: BK = 32, global -> registers -> LDS double
-// buffer, three v_mfma_f32_32x32x16_bf16 products per k step.
+// BF16x3 tile: BK = 32.  Each stage's float32 A (BM x 32) and B (32 x BN) tiles
+// are split in registers and stored to LDS rows that are K-contiguous for both
+// operands (per 8-k group: 8 hi then 8 lo bf16); three v_mfma_f32_32x32x16_bf16
+// products per 16 k.  A item: one row, 8 consecutive k (two 16-byte loads).  B
+// item: one column, 8 consecutive k (8 dword loads; the 64 lanes of a wave read
+// 64 adjacent columns, one 256-byte row segment per load).  The loads of a
+// stage are issued a full step before its LDS store, which runs between the
+// two 16-k substeps of the previous stage.
 // ---------------------------------------------------------------------------
 template <int BM, int BN>
 struct B3T {
-    static constexpr int BK = 32, ROW = 2 * BK + 8;  // packed bf16 per LDS row (+16 B pad)
-    static constexpr int STAGE = (BM + BN) * ROW;     // bf16 elements
-    static constexpr int LA = BM * 8 / kThreads;      // 16-byte chunks of A per thread
-    static constexpr int LB = BN * 8 / kThreads;
+    static constexpr int BK = 32, ROW = 2 * BK + 8;    // bf16 per LDS row (+16 B pad)
+    static constexpr int STAGE = (BM + BN) * ROW;       // bf16 elements
+    static constexpr int IA = BM * (BK / 8) / kThreads;  // A items per thread
+    static constexpr int IB = BN * (BK / 8) / kThreads;  // B items per thread
     static constexpr int LDS_BYTES = 2 * STAGE * 2;
+    static_assert(IA >= 1 && IB >= 1 && BN % 64 == 0, "items; one k group per wave for B");
 };
 
 template <int BM, int BN>
@@ -387,9 +297,6 @@ __device__ void b3_tile(uint16_t *smem, const Prob &p, const Item &it, float *pa
     const int wm = wave & 1, wn = wave >> 1;
     const int l32 = lane & 31, h = lane >> 5;
     const int m0 = it.tm * BM, n0 = it.tn * BN;
-    const int64_t rowe = (int64_t)p.Kp * 2;  // packed elements per row
-    const uint16_t *Ag = p.Ap + (int64_t)m0 * rowe;
-    const uint16_t *Bg = p.Bp + (int64_t)n0 * rowe;
 
     floatx16 acc[FM][FN];
 #pragma unroll
@@ -397,70 +304,110 @@ __device__ void b3_tile(uint16_t *smem, const Prob &p, const Item &it, float *pa
 #pragma unroll
         for (int c = 0; c < FN; ++c) acc[r][c] = floatx16{0};
 
-    floatx4 ra[T::LA], rb[T::LB];
-    // thread tid loads chunk (tid & 7) of rows (tid >> 3) + 32 i
-    const uint16_t *asrc = Ag + (tid >> 3) * rowe + (tid & 7) * 8;
-    const uint16_t *bsrc = Bg + (tid >> 3) * rowe + (tid & 7) * 8;
-    const int64_t step = 32 * rowe;
+    // K, lda and the split bounds are multiples of 4 and k0 of 32, so each
+    // 16-byte A load lies wholly inside or wholly outside [k0, k1)
+    float ra[T::IA][8], rb[T::IB][8];
     auto gload = [&](int k) {
 #pragma unroll
-        for (int i = 0; i < T::LA; ++i) ra[i] = *(const floatx4 *)(asrc + i * step + 2 * k);
+        for (int i = 0; i < T::IA; ++i) {
+            const int f = tid + i * kThreads;
+            const int row = f / (BK / 8), gk = k + (f % (BK / 8)) * 8;
+            const float *x = p.A + (int64_t)(m0 + row) * p.lda + gk;
+            const bool rok = m0 + row < p.M;
+            const floatx4 zero = {0, 0, 0, 0};
+            const floatx4 v0 = (rok && gk < it.k1) ? *(const floatx4 *)x : zero;
+            const floatx4 v1 = (rok && gk + 4 < it.k1) ? *(const floatx4 *)(x + 4) : zero;
 #pragma unroll
-        for (int i = 0; i < T::LB; ++i) rb[i] = *(const floatx4 *)(bsrc + i * step + 2 * k);
+            for (int j = 0; j < 4; ++j) {
+                ra[i][j] = v0[j];
+                ra[i][j + 4] = v1[j];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < T::IB; ++i) {
+            const int f = tid + i * kThreads;
+            const int col = f % BN;
+            const int gk = k + __builtin_amdgcn_readfirstlane(f / BN) * 8;
+            const bool cok = n0 + col < p.N;
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                rb[i][j] = (cok && gk + j < it.k1) ? p.B[(int64_t)(gk + j) * p.ldb + n0 + col] : 0.0f;
+        }
     };
     auto sstore = [&](int buf) {
         uint16_t *as = smem + buf * T::STAGE;
         uint16_t *bs = as + BM * ROW;
 #pragma unroll
-        for (int i = 0; i < T::LA; ++i) {
+        for (int i = 0; i < T::IA; ++i) {
             const int f = tid + i * kThreads;
-            *(floatx4 *)(as + (f >> 3) * ROW + (f & 7) * 8) = ra[i];
+            uint32_t hi[4], lo[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) split_pk(ra[i][2 * j], ra[i][2 * j + 1], hi[j], lo[j]);
+            uint16_t *d = as + (f / (BK / 8)) * ROW + (f % (BK / 8)) * 16;
+            *(uint4 *)d = uint4{hi[0], hi[1], hi[2], hi[3]};
+            *(uint4 *)(d + 8) = uint4{lo[0], lo[1], lo[2], lo[3]};
         }
 #pragma unroll
-        for (int i = 0; i < T::LB; ++i) {
+        for (int i = 0; i < T::IB; ++i) {
             const int f = tid + i * kThreads;
-            *(floatx4 *)(bs + (f >> 3) * ROW + (f & 7) * 8) = rb[i];
+            uint32_t hi[4], lo[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) split_pk(rb[i][2 * j], rb[i][2 * j + 1], hi[j], lo[j]);
+            uint16_t *d = bs + (f % BN) * ROW + (f / BN) * 16;
+            *(uint4 *)d = uint4{hi[0], hi[1], hi[2], hi[3]};
+            *(uint4 *)(d + 8) = uint4{lo[0], lo[1], lo[2], lo[3]};
         }
     };
-    auto compute = [&](int buf) {
+    // one 16-deep substep of the stage in buffer `buf`
+    auto compute = [&](int buf, int st) {
         const uint16_t *as = smem + buf * T::STAGE + (wm * (BM / 2) + l32) * ROW;
         const uint16_t *bs = smem + buf * T::STAGE + BM * ROW + (wn * (BN / 2) + l32) * ROW;
+        const int off = (2 * st + h) * 16;  // this lane half's 8-k group
+        bf16x8 ahi[FM], alo[FM], bhi[FN], blo[FN];
 #pragma unroll
-        for (int st = 0; st < BK / 16; ++st) {
-            const int off = (2 * st + h) * 16;
-            bf16x8 ahi[FM], alo[FM], bhi[FN], blo[FN];
+        for (int r = 0; r < FM; ++r) {
+            ahi[r] = *(const bf16x8 *)(as + r * 32 * ROW + off);
+            alo[r] = *(const bf16x8 *)(as + r * 32 * ROW + off + 8);
+        }
 #pragma unroll
-            for (int r = 0; r < FM; ++r) {
-                ahi[r] = *(const bf16x8 *)(as + r * 32 * ROW + off);
-                alo[r] = *(const bf16x8 *)(as + r * 32 * ROW + off + 8);
-            }
+        for (int c = 0; c < FN; ++c) {
+            bhi[c] = *(const bf16x8 *)(bs + c * 32 * ROW + off);
+            blo[c] = *(const bf16x8 *)(bs + c * 32 * ROW + off + 8);
+        }
+#pragma unroll
+        for (int r = 0; r < FM; ++r)
 #pragma unroll
             for (int c = 0; c < FN; ++c) {
-                bhi[c] = *(const bf16x8 *)(bs + c * 32 * ROW + off);
-                blo[c] = *(const bf16x8 *)(bs + c * 32 * ROW + off + 8);
+                acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[r], bhi[c], acc[r][c], 0, 0, 0);
+                acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[r], blo[c], acc[r][c], 0, 0, 0);
+                acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[r], bhi[c], acc[r][c], 0, 0, 0);
             }
-#pragma unroll
-            for (int r = 0; r < FM; ++r)
-#pragma unroll
-                for (int c = 0; c < FN; ++c) {
-                    acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[r], bhi[c], acc[r][c], 0, 0, 0);
-                    acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[r], blo[c], acc[r][c], 0, 0, 0);
-                    acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[r], bhi[c], acc[r][c], 0, 0, 0);
-                }
-        }
     };
 
     const int KT = (it.k1 - it.k0 + BK - 1) / BK;
     gload(it.k0);
     sstore(0);
+    if (KT > 1) gload(it.k0 + BK);
     __syncthreads();
-    for (int kt = 0; kt < KT; ++kt) {
+    int kt = 0;
+    for (; kt + 2 < KT; ++kt) {
         const int cur = kt & 1;
-        if (kt + 1 < KT) gload(it.k0 + (kt + 1) * BK);
-        compute(cur);
-        if (kt + 1 < KT) sstore(cur ^ 1);
+        compute(cur, 0);
+        sstore(cur ^ 1);
+        gload(it.k0 + (kt + 2) * BK);
+        compute(cur, 1);
         __syncthreads();
     }
+    if (kt + 1 < KT) {
+        const int cur = kt & 1;
+        compute(cur, 0);
+        sstore(cur ^ 1);
+        compute(cur, 1);
+        __syncthreads();
+        ++kt;
+    }
+    compute(kt & 1, 0);
+    compute(kt & 1, 1);
     finish_tile<BM, BN>(acc, p, it, partials, counters, s_flag);
 }
 
@@ -585,28 +532,12 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
     HBS_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
 
     std::vector<Prob> probs(count);
-    std::vector<PackTask> tasks;
-    std::vector<int64_t> pack_off_a(count, 0), pack_off_b(count, 0);
-    int64_t pack_elems = 0;  // bf16 elements of packed operands
     std::vector<Item> items;
     std::vector<double> cost;
     int64_t slots = 0, poff = 0, tiles_total = 0;
     for (int i = 0; i < count; ++i) {
         const hbs_gemm_problem &q = problems[i];
-        probs[i] = Prob{q.A, q.B, q.C, q.M, q.N, q.K, q.lda, q.ldb, q.ldc, nullptr, nullptr, 0};
-        if (precision == HBS_PREC_BF16X3) {
-            const int Kp = (q.K + 31) / 32 * 32, Mp = (q.M + 127) / 128 * 128,
-                      Np = (q.N + 127) / 128 * 128;
-            probs[i].Kp = Kp;
-            pack_off_a[i] = pack_elems;
-            pack_elems += (int64_t)Mp * Kp * 2;
-            pack_off_b[i] = pack_elems;
-            pack_elems += (int64_t)Np * Kp * 2;
-            for (int r0 = 0; r0 < Mp; r0 += 64)
-                for (int k0 = 0; k0 < Kp; k0 += 64) tasks.push_back(PackTask{i, 0, r0, k0});
-            for (int r0 = 0; r0 < Np; r0 += 64)
-                for (int k0 = 0; k0 < Kp; k0 += 64) tasks.push_back(PackTask{i, 1, r0, k0});
-        }
+        probs[i] = Prob{q.A, q.B, q.C, q.M, q.N, q.K, q.lda, q.ldb, q.ldc};
         Choice c = heuristic(precision, q.M, q.N, q.K);
         if (forced) {
             c = *forced;
@@ -654,7 +585,7 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
             }
         }
     }
-    if (items.size() > 0x7fffffffULL || tasks.size() > 0x7fffffffULL) return HBS_ERR_ARG;
+    if (items.size() > 0x7fffffffULL) return HBS_ERR_ARG;
 
     // LPT: items in decreasing cost, each to the least-loaded workgroup.
     // resident workgroups per CU: FP32 tiles use <= 168 VGPRs and 38 KB LDS
@@ -698,15 +629,8 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
     const size_t b_boff = align(boff.size() * sizeof(int));
     const size_t b_cnt = align(std::max<int64_t>(slots, 1) * sizeof(int));
     const size_t b_part = align(std::max<int64_t>(poff, 1) * sizeof(float));
-    const size_t b_tasks = align(std::max<size_t>(tasks.size(), 1) * sizeof(PackTask));
-    const size_t b_pack = align((size_t)std::max<int64_t>(pack_elems, 1) * sizeof(uint16_t));
     char *dmem = nullptr;
-    HBS_HIP_TRY(hipMalloc(&dmem, b_probs + b_items + b_boff + b_cnt + b_part + b_tasks + b_pack));
-    char *packed = dmem + b_probs + b_items + b_boff + b_cnt + b_part + b_tasks;
-    for (int i = 0; i < count && precision == HBS_PREC_BF16X3; ++i) {
-        probs[i].Ap = (const uint16_t *)packed + pack_off_a[i];
-        probs[i].Bp = (const uint16_t *)packed + pack_off_b[i];
-    }
+    HBS_HIP_TRY(hipMalloc(&dmem, b_probs + b_items + b_boff + b_cnt + b_part));
     hbs_grouped_plan *plan = new hbs_grouped_plan();
     plan->device = dev;
     plan->precision = precision;
@@ -722,16 +646,11 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
     plan->d_boff = (const int *)(dmem + b_probs + b_items);
     plan->d_counters = (int *)(dmem + b_probs + b_items + b_boff);
     plan->d_partials = (float *)(dmem + b_probs + b_items + b_boff + b_cnt);
-    plan->d_tasks = dmem + b_probs + b_items + b_boff + b_cnt + b_part;
-    plan->ntasks = (int)tasks.size();
     hipError_t e = hipMemcpy(dmem, probs.data(), probs.size() * sizeof(Prob), hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(dmem + b_probs, sorted.data(), sorted.size() * sizeof(Item), hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(dmem + b_probs + b_items, boff.data(), boff.size() * sizeof(int),
-                      hipMemcpyHostToDevice);
-    if (e == hipSuccess && !tasks.empty())
-        e = hipMemcpy((void *)plan->d_tasks, tasks.data(), tasks.size() * sizeof(PackTask),
                       hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(plan->d_counters, 0, b_cnt);
     if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -791,14 +710,10 @@ extern "C" int hbs_grouped_run(const hbs_grouped_plan *plan, void *stream) {
         grouped_kernel<HBS_PREC_FP32><<<plan->nblocks, kThreads, 0, s>>>(pr, it, plan->d_boff,
                                                                          plan->d_partials,
                                                                          plan->d_counters);
-    else {
-        grouped_pack_kernel<<<plan->ntasks, 256, 0, s>>>(pr, (const PackTask *)plan->d_tasks);
-        const int rc = hbs::launch_status();
-        if (rc) return rc;
+    else
         grouped_kernel<HBS_PREC_BF16X3><<<plan->nblocks, kThreads, 0, s>>>(pr, it, plan->d_boff,
                                                                            plan->d_partials,
                                                                            plan->d_counters);
-    }
     return hbs::launch_status();
 }
 

@@ -75,6 +75,38 @@ def test_grouped_integer_inputs_exact(prec):
         assert torch.equal(C, expect), shape
 
 
+@pytest.mark.parametrize("tile,split", [(0, 1), (1, 1), (2, 1), (3, 1), (0, 2)])
+def test_grouped_bf16x3_identity_reproduces_oracle_split(tile, split):
+    """BF16x3 with B = I (first problem) and with A = I (second): each output
+    is lo + 0 + hi of the other operand's element, and hi + lo is exact in
+    FP32, so C equals the oracle's split summed in float32 bit for bit.  Pins
+    the in-register split of the A path and of the B path for every tile
+    shape, with ragged M, N and K (132 is not a multiple of the 32-deep
+    stage); with split-K the other splits add exact zeros."""
+    M, K, N = 200, 132, 260
+
+    def loguniform(n, seed):
+        x = torch.empty(n, dtype=torch.float32, device="cuda")
+        hb.fill_random(x, n, hb.GEN_LOGUNIFORM, seed)
+        return x
+
+    a1 = loguniform(M * K, 71).view(M, K)
+    b2 = loguniform(K * N, 72).view(K, N)
+    eye = torch.eye(K, dtype=torch.float32, device="cuda").contiguous()
+    c1 = torch.full((M, K), float("nan"), device="cuda")
+    c2 = torch.full((K, N), float("nan"), device="cuda")
+    plan = hb.GroupedPlan([(a1, eye, c1, M, K, K), (eye, b2, c2, K, N, K)], hb.PREC_BF16X3,
+                          tile=tile, split_k=split)
+    plan.run()
+    torch.cuda.synchronize()
+    for x, c in ((a1, c1), (b2, c2)):
+        xh = x.cpu().numpy()
+        hi, lo = oracle.split_bf16(xh)
+        want = oracle.bf16_bits_to_f32(hi) + oracle.bf16_bits_to_f32(lo)
+        assert (want != xh).any()
+        assert np.array_equal(c.cpu().numpy(), want), (tile, split)
+
+
 @pytest.mark.parametrize("prec", [hb.PREC_FP32, hb.PREC_BF16X3])
 @pytest.mark.parametrize("tile,split", [(0, 1), (1, 1), (2, 1), (3, 1), (0, 3), (1, 4), (2, 2)])
 def test_grouped_forced_configs_within_bound(prec, tile, split):
