@@ -8,8 +8,10 @@ and resident in HBM before the timed region.
 
 Prints one JSON line: whole-job TFLOP/s, the roofline of the dominant kernel
 (timed with HIP events on the stream it is launched on), the FP64-verified
-error of the result, and the reference's CPU golden (host_mm restated in the
-oracle, single thread) timed on a bounded row sample on this host.
+error of the result, the BF16x3 kernel on the same workload with the FP32 and
+BF16x3 errors on log-uniform inputs (the metric's comparison), and the
+reference's CPU golden (host_mm restated in the oracle, single thread) timed
+on a bounded row sample on this host.  --variant selects the other workloads.
 """
 import argparse
 import json
@@ -44,6 +46,8 @@ def parse():
                    help="target duration of the CPU golden sample")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--no-verify", action="store_true", help="skip the FP64 error check")
+    p.add_argument("--no-compare", action="store_true",
+                   help="fp32: skip the BF16x3 comparison on the same workload")
     p.add_argument("--mode", choices=["gemm", "copy"], default="gemm",
                    help="copy: a plain device-to-device copy of --copy-mib MiB, the "
                         "reference kernel that HBM byte counters are calibrated on")
@@ -149,9 +153,57 @@ def setup_square(args, hb, torch):
     kind = "plain FP32 MFMA" if args.variant == "fp32" else "BF16x3 split"
     workload = (f"{kind} SGEMM, square N={N}, NITER={NITER} slices per launch, "
                 f"uniform[-1,1) seed {args.seed}")
-    return step, 2.0 * N * N * N * NITER, workload, {
-        "verify": verify, "config": {"N": N, "niter": NITER},
-        "traffic": measured_traffic(args.variant, f"N={N} NITER={NITER}")}
+    extra = {"verify": verify, "config": {"N": N, "niter": NITER},
+             "traffic": measured_traffic(args.variant, f"N={N} NITER={NITER}")}
+    if args.variant == "fp32" and not args.no_compare:
+        extra["compare"] = lambda fp32_tflops: bf16x3_comparison(args, hb, torch, a, b, c,
+                                                                 fp32_tflops)
+    return step, 2.0 * N * N * N * NITER, workload, extra
+
+
+def bf16x3_comparison(args, hb, torch, a, b, c, fp32_tflops):
+    """The metric's comparison at the same N and NITER: BF16x3 throughput on the
+    same uniform inputs (same step count, HIP events around each step), and the
+    FP32-level error claim -- both kernels on one log-uniform slice (|x| in
+    [1e-3, 1e3], random sign), max componentwise error against FP64 on the device."""
+    N, NITER = args.n, args.niter
+    flops = 2.0 * N * N * N * NITER
+    for _ in range(args.warmup):
+        hb.sgemm_bf16x3(a, b, c, N, NITER)
+    stream = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s, e in ev:
+        s.record(stream)
+        hb.sgemm_bf16x3(a, b, c, N, NITER)
+        e.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    value = flops * args.steps / wall / 1e12
+    kernel_ms = sum(s.elapsed_time(e) for s, e in ev) / len(ev)
+    out = {"value": round(value, 3), "unit": "TFLOP/s", "ms_per_step": round(wall / args.steps * 1e3, 4),
+           "kernel_ms_avg": round(kernel_ms, 4), "ratio_vs_fp32": round(value / fp32_tflops, 3),
+           "roofline_frac": round(flops / (kernel_ms / 1e3) / 1e12 / (PEAK_TFLOPS["bf16"] / 3.0), 4)}
+    if not args.no_verify:
+        nn = N * N
+        la, lb = a[:nn], b[:nn]
+        hb.fill_random(la, nn, hb.GEN_LOGUNIFORM, args.seed + 2)
+        hb.fill_random(lb, nn, hb.GEN_LOGUNIFORM, args.seed + 3)
+        c32 = c[:nn]
+        c3 = c[nn:2 * nn] if NITER > 1 else torch.empty_like(la)
+        hb.sgemm(la, lb, c32, N, 1)
+        hb.sgemm_bf16x3(la, lb, c3, N, 1)
+        e32 = hb.verify_f64(la, lb, c32, N, N, N)
+        e3 = hb.verify_f64(la, lb, c3, N, N, N)
+        out["loguniform_error"] = {
+            "fp32_e_cw": e32["e_cw"], "bf16x3_e_cw": e3["e_cw"],
+            "fp32_e_F": e32["e_F"], "bf16x3_e_F": e3["e_F"],
+            "e_cw_ratio": e3["e_cw"] / e32["e_cw"] if e32["e_cw"] > 0 else None,
+            "inputs": f"one {N}x{N} slice, |x| log-uniform in [1e-3, 1e3], random sign, "
+                      f"seeds {args.seed + 2}/{args.seed + 3}"}
+    return out
 
 
 def grouped_shapes(seed, problems):
@@ -324,6 +376,9 @@ def main():
     }
     if not args.no_verify:
         line["accuracy"] = extra["verify"]()
+    if "compare" in extra:
+        # after the FP32 result is verified: the comparison overwrites the inputs
+        line["comparison"] = {"bf16x3": extra["compare"](value)}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.n, args.cpu_seconds)
     print(json.dumps(line), flush=True)
