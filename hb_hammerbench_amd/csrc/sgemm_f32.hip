@@ -4,8 +4,9 @@
 // output blocks strided over a 16x8 tile group, DRAM->scratchpad block loads
 // at :36-109, a 4x4 register-blocked fmaf micro-kernel at :186-275).  The
 // MI355X design instead gives each workgroup one BM x BN output tile:
-//   * global -> registers -> LDS double buffer, 128-bit coalesced loads, the
-//     next K tile in flight while the current one is multiplied;
+//   * global -> registers -> LDS double buffer, 128-bit coalesced loads; each
+//     K stage is loaded a full step ahead and stored to LDS in the middle of
+//     the previous step;
 //   * each wave owns a (BM/WM) x (BN/WN) sub-tile held as 32x32 accumulator
 //     blocks of v_mfma_f32_32x32x2_f32 (FP32 in, FP32 accumulate);
 //   * an A fragment for four consecutive MFMAs comes from one 128-bit LDS read
@@ -43,7 +44,7 @@ struct Cfg {
     static_assert(kWTM % 32 == 0 && kWTN % 32 == 0, "wave tile must be a multiple of 32");
     static_assert((BM * BK / 4) % kThreads == 0, "A tile must split evenly over threads");
     static_assert((BK * BN / 4) % kThreads == 0, "B tile must split evenly over threads");
-    static_assert(BK % 8 == 0, "BK must be a multiple of 8");
+    static_assert(BK % 16 == 0, "BK must be a multiple of 16 (two or more 8-k chunks)");
 };
 
 template <int BM, int BN, int BK, int WM, int WN, bool EDGE>
@@ -159,34 +160,54 @@ __global__ __launch_bounds__(WM *WN * 64) void sgemm_f32_kernel(GemmArgs p) {
             __builtin_amdgcn_sched_group_barrier(0x008, kPer, 0);  // then MFMAs
         }
     };
-    auto compute = [&](int buf) {
+    // One stage (BK/8 chunks of 8 k) from LDS buffer `buf`, the fragments of
+    // chunk kc + 1 requested before the MFMAs of chunk kc.  `hook` runs after
+    // the MFMAs of chunk kHook - 1, once the fragments of chunk kHook are in
+    // flight, so the matrix core is never left waiting on a fresh LDS read.
+    constexpr int kChunks = BK / 8, kHook = kChunks / 2;
+    auto compute = [&](int buf, auto &&hook) {
         const float *as = As + buf * G::kStageA + (wm * G::kWTM + l32) * G::kLdA + 4 * h;
         const float *bs = Bs + buf * G::kStageB + 4 * h * G::kLdB + wn * G::kWTN + l32;
         floatx4 a0[G::kFM], a1[G::kFM];
         float b0[G::kFN][4], b1[G::kFN][4];
         load_frag(as, bs, 0, a0, b0);
 #pragma unroll
-        for (int kc = 0; kc < BK / 8; kc += 2) {
-            if (kc + 1 < BK / 8) load_frag(as, bs, kc + 1, a1, b1);
+        for (int kc = 0; kc < kChunks; kc += 2) {
+            if (kc + 1 < kChunks) load_frag(as, bs, kc + 1, a1, b1);
             mfma_chunk(a0, b0);
             interleave();
-            if (kc + 2 < BK / 8) load_frag(as, bs, kc + 2, a0, b0);
-            if (kc + 1 < BK / 8) mfma_chunk(a1, b1);
+            if (kc + 1 == kHook) hook();
+            if (kc + 2 < kChunks) load_frag(as, bs, kc + 2, a0, b0);
+            if (kc + 1 < kChunks) mfma_chunk(a1, b1);
             interleave();
+            if (kc + 2 == kHook) hook();
         }
     };
 
+    // Pipeline: stage kt + 1 is stored to the other LDS buffer in the middle
+    // of step kt and the global loads of stage kt + 2 are issued right after,
+    // so every load has a whole step to arrive and the store overlaps MFMAs.
     const int KT = (p.K + BK - 1) / BK;
     gload(0);
     sstore(0);
+    if (KT > 1) gload(BK);
     __syncthreads();
-    for (int kt = 0; kt < KT; ++kt) {
+    int kt = 0;
+    for (; kt + 2 < KT; ++kt) {
         const int cur = kt & 1;
-        if (kt + 1 < KT) gload((kt + 1) * BK);
-        compute(cur);
-        if (kt + 1 < KT) sstore(cur ^ 1);
+        compute(cur, [&]() {
+            sstore(cur ^ 1);
+            gload((kt + 2) * BK);
+        });
         __syncthreads();
     }
+    if (kt + 1 < KT) {
+        const int cur = kt & 1;
+        compute(cur, [&]() { sstore(cur ^ 1); });
+        __syncthreads();
+        ++kt;
+    }
+    compute(kt & 1, [&]() {});
 
     // Epilogue: 32x32 accumulator layout -- register q of lane l holds
     // row 8*(q/4) + 4*(l/32) + q%4, col l%32 of its block.
@@ -224,8 +245,8 @@ int run_f32(GemmArgs p, hipStream_t stream) {
     // Largest tile whose grid still covers the 256 CUs.
     const int64_t big_tiles = (int64_t)((p.M + 255) / 256) * ((p.N + 255) / 256) * p.batch;
     if (big_tiles >= 256) {
-        const bool edge = !(vec_ok && p.M % 256 == 0 && p.N % 256 == 0 && p.K % 16 == 0);
-        return launch_cfg<256, 256, 16, 2, 4>(p, edge, stream);
+        const bool edge = !(vec_ok && p.M % 256 == 0 && p.N % 256 == 0 && p.K % 32 == 0);
+        return launch_cfg<256, 256, 32, 4, 2>(p, edge, stream);
     }
     const int64_t mid_tiles = (int64_t)((p.M + 127) / 128) * ((p.N + 127) / 128) * p.batch;
     if (mid_tiles >= 128) {
