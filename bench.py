@@ -8,10 +8,11 @@ and resident in HBM before the timed region.
 
 Prints one JSON line: whole-job TFLOP/s, the roofline of the dominant kernel
 (timed with HIP events on the stream it is launched on), the FP64-verified
-error of the result, the BF16x3 kernel on the same workload with the FP32 and
-BF16x3 errors on log-uniform inputs (the metric's comparison), and the
-reference's CPU golden (host_mm restated in the oracle, single thread) timed
-on a bounded row sample on this host.  --variant selects the other workloads.
+error of the result, the metric's comparison (the BF16x3 kernel on the same
+inputs, and both kernels on one log-uniform batch with their errors against
+FP64), and the reference's CPU golden (host_mm restated in the oracle, single
+thread) timed on a bounded row sample on this host.  --variant selects the
+other workloads.
 """
 import argparse
 import json
@@ -133,8 +134,10 @@ def setup_square(args, hb, torch):
     a = torch.empty(n_total, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)
     c = torch.empty_like(a)
-    hb.fill_random(a, n_total, hb.GEN_UNIFORM, args.seed)
-    hb.fill_random(b, n_total, hb.GEN_UNIFORM, args.seed + 1)
+    # FP32 (C2): uniform[-1,1); BF16x3 (C3): |x| log-uniform in [1e-3, 1e3], random sign
+    kind = hb.GEN_UNIFORM if args.variant == "fp32" else hb.GEN_LOGUNIFORM
+    hb.fill_random(a, n_total, kind, args.seed)
+    hb.fill_random(b, n_total, kind, args.seed + 1)
     torch.cuda.synchronize()
 
     def step():
@@ -150,9 +153,12 @@ def setup_square(args, hb, torch):
         return {"slice": it, "e_cw": st["e_cw"], "e_nw": st["e_nw"], "e_F": st["e_F"],
                 "gate_e_cw": 1e-5 * N ** 0.5, "oracle": "FP64 on device (hbs_verify_f64)"}
 
-    kind = "plain FP32 MFMA" if args.variant == "fp32" else "BF16x3 split"
-    workload = (f"{kind} SGEMM, square N={N}, NITER={NITER} slices per launch, "
-                f"uniform[-1,1) seed {args.seed}")
+    if args.variant == "fp32":
+        workload = (f"plain FP32 MFMA SGEMM, square N={N}, NITER={NITER} slices per launch, "
+                    f"uniform[-1,1) seed {args.seed}")
+    else:
+        workload = (f"BF16x3 split SGEMM, square N={N}, NITER={NITER} slices per launch, "
+                    f"|x| log-uniform in [1e-3, 1e3] with random sign, seed {args.seed}")
     extra = {"verify": verify, "config": {"N": N, "niter": NITER},
              "traffic": measured_traffic(args.variant, f"N={N} NITER={NITER}")}
     if args.variant == "fp32" and not args.no_compare:
@@ -161,49 +167,63 @@ def setup_square(args, hb, torch):
     return step, 2.0 * N * N * N * NITER, workload, extra
 
 
-def bf16x3_comparison(args, hb, torch, a, b, c, fp32_tflops):
-    """The metric's comparison at the same N and NITER: BF16x3 throughput on the
-    same uniform inputs (same step count, HIP events around each step), and the
-    FP32-level error claim -- both kernels on one log-uniform slice (|x| in
-    [1e-3, 1e3], random sign), max componentwise error against FP64 on the device."""
-    N, NITER = args.n, args.niter
-    flops = 2.0 * N * N * N * NITER
-    for _ in range(args.warmup):
-        hb.sgemm_bf16x3(a, b, c, N, NITER)
+def time_steps(torch, steps, warmup, fn):
+    """Mean wall time per step (s) over `steps` calls after `warmup`, and the
+    mean HIP-event time per call (ms) on the current stream."""
+    for _ in range(warmup):
+        fn()
     stream = torch.cuda.current_stream()
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-          for _ in range(args.steps)]
+          for _ in range(steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for s, e in ev:
         s.record(stream)
-        hb.sgemm_bf16x3(a, b, c, N, NITER)
+        fn()
         e.record(stream)
     torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    value = flops * args.steps / wall / 1e12
-    kernel_ms = sum(s.elapsed_time(e) for s, e in ev) / len(ev)
-    out = {"value": round(value, 3), "unit": "TFLOP/s", "ms_per_step": round(wall / args.steps * 1e3, 4),
-           "kernel_ms_avg": round(kernel_ms, 4), "ratio_vs_fp32": round(value / fp32_tflops, 3),
-           "roofline_frac": round(flops / (kernel_ms / 1e3) / 1e12 / (PEAK_TFLOPS["bf16"] / 3.0), 4)}
+    wall = (time.perf_counter() - t0) / steps
+    return wall, sum(s.elapsed_time(e) for s, e in ev) / len(ev)
+
+
+def bf16x3_comparison(args, hb, torch, a, b, c, fp32_tflops):
+    """The metric's comparison at the same N and NITER.
+      same_inputs: BF16x3 on the uniform inputs `value` was measured on;
+      loguniform:  both kernels on one log-uniform batch (|x| in [1e-3, 1e3],
+                   random sign: the BF16x3 configuration's inputs), their
+                   throughput ratio, and both errors against FP64 on one slice
+                   (the FP32-level error claim: e_cw ratio <= 2)."""
+    N, NITER = args.n, args.niter
+    flops = 2.0 * N * N * N * NITER
+    bf16x3_peak = PEAK_TFLOPS["bf16"] / 3.0
+    wall, kms = time_steps(torch, args.steps, args.warmup,
+                           lambda: hb.sgemm_bf16x3(a, b, c, N, NITER))
+    same = {"value": round(flops / wall / 1e12, 3), "ms_per_step": round(wall * 1e3, 4),
+            "kernel_ms_avg": round(kms, 4),
+            "ratio_vs_fp32": round(flops / wall / 1e12 / fp32_tflops, 3),
+            "roofline_frac": round(flops / (kms / 1e3) / 1e12 / bf16x3_peak, 4)}
+    total = N * N * NITER
+    hb.fill_random(a, total, hb.GEN_LOGUNIFORM, args.seed + 2)
+    hb.fill_random(b, total, hb.GEN_LOGUNIFORM, args.seed + 3)
+    w32, _ = time_steps(torch, args.steps, args.warmup, lambda: hb.sgemm(a, b, c, N, NITER))
+    w3, _ = time_steps(torch, args.steps, args.warmup, lambda: hb.sgemm_bf16x3(a, b, c, N, NITER))
+    logu = {"fp32_value": round(flops / w32 / 1e12, 3), "bf16x3_value": round(flops / w3 / 1e12, 3),
+            "ratio": round(w32 / w3, 3),
+            "inputs": f"{NITER} slices of {N}x{N}, |x| log-uniform in [1e-3, 1e3], random sign, "
+                      f"seeds {args.seed + 2}/{args.seed + 3}"}
     if not args.no_verify:
         nn = N * N
-        la, lb = a[:nn], b[:nn]
-        hb.fill_random(la, nn, hb.GEN_LOGUNIFORM, args.seed + 2)
-        hb.fill_random(lb, nn, hb.GEN_LOGUNIFORM, args.seed + 3)
         c32 = c[:nn]
-        c3 = c[nn:2 * nn] if NITER > 1 else torch.empty_like(la)
-        hb.sgemm(la, lb, c32, N, 1)
-        hb.sgemm_bf16x3(la, lb, c3, N, 1)
-        e32 = hb.verify_f64(la, lb, c32, N, N, N)
-        e3 = hb.verify_f64(la, lb, c3, N, N, N)
-        out["loguniform_error"] = {
+        c3 = c[nn:2 * nn] if NITER > 1 else torch.empty_like(a[:nn])
+        hb.sgemm(a[:nn], b[:nn], c32, N, 1)
+        hb.sgemm_bf16x3(a[:nn], b[:nn], c3, N, 1)
+        e32 = hb.verify_f64(a[:nn], b[:nn], c32, N, N, N)
+        e3 = hb.verify_f64(a[:nn], b[:nn], c3, N, N, N)
+        logu["error_slice0"] = {
             "fp32_e_cw": e32["e_cw"], "bf16x3_e_cw": e3["e_cw"],
             "fp32_e_F": e32["e_F"], "bf16x3_e_F": e3["e_F"],
-            "e_cw_ratio": e3["e_cw"] / e32["e_cw"] if e32["e_cw"] > 0 else None,
-            "inputs": f"one {N}x{N} slice, |x| log-uniform in [1e-3, 1e3], random sign, "
-                      f"seeds {args.seed + 2}/{args.seed + 3}"}
-    return out
+            "e_cw_ratio": e3["e_cw"] / e32["e_cw"] if e32["e_cw"] > 0 else None}
+    return {"same_inputs": same, "loguniform": logu}
 
 
 def grouped_shapes(seed, problems):
