@@ -25,7 +25,7 @@ _LIB_NAME = "libhbsgemm.so"
 
 GEN_UNIFORM, GEN_LOGUNIFORM, GEN_NORMAL_OUTLIER = 0, 1, 2
 PREC_FP32, PREC_BF16X3 = 0, 1
-TILE_128x128, TILE_64x64, TILE_128x64, TILE_64x128 = 0, 1, 2, 3
+TILE_128x128, TILE_64x64, TILE_128x64, TILE_64x128, TILE_256x256 = 0, 1, 2, 3, 4
 
 _lib = None
 

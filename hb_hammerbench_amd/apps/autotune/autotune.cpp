@@ -182,9 +182,9 @@ static int group_mode(const char *shapes_path, const char *out_path, const std::
             const int mc = std::get<0>(cl.second), nc = std::get<1>(cl.second),
                       kc = std::get<2>(cl.second);
             Override keep{mc, nc, kc, -1, 0};
-            for (int tile = 0; tile < 4; ++tile) {
+            for (int tile = 0; tile <= HBS_TILE_256x256; ++tile) {
                 for (int split : {1, 2, 4}) {
-                    if (split > 1 && kc < 256) continue;
+                    if (split > 1 && (kc < 256 || tile == HBS_TILE_256x256)) continue;
                     std::vector<Override> trial = ov;
                     trial.push_back(Override{mc, nc, kc, tile, split});
                     write_table(tmp.c_str(), pname, trial, nullptr);
@@ -288,7 +288,7 @@ int main(int argc, char **argv) {
     fprintf(out, "# hbsgemm grouped tuning file: prec Mclass Nclass Kclass tile split ns_per_item\n");
     for (int prec : precs) {
         const char *pname = prec == HBS_PREC_FP32 ? "fp32" : "bf16x3";
-        const int ntiles = 4;
+        const int ntiles = HBS_TILE_256x256 + 1;
         for (int mc : classes) {
             for (int nc : classes) {
                 for (int kc : classes) {
@@ -304,8 +304,10 @@ int main(int argc, char **argv) {
                     }
                     Probe best{-1, 0, 1e30, 0.0};
                     for (int tile = 0; tile < ntiles; ++tile) {
+                        // the large-tile path needs a 256 x 256 core (no split-K)
+                        if (tile == HBS_TILE_256x256 && (mc < 256 || nc < 256)) continue;
                         for (int split : {1, 2, 4}) {
-                            if (split > 1 && kc < 512) continue;
+                            if (split > 1 && (kc < 512 || tile == HBS_TILE_256x256)) continue;
                             hbs_grouped_plan *plan = nullptr;
                             int rc = hbs_grouped_plan_create_forced(probs.data(), count, prec, tile,
                                                                     split, &plan);

@@ -32,9 +32,8 @@
 #include <tuple>
 #include <vector>
 
-#include "hbs_internal.h"
-
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+#include "tile_bf16x3.h"
+#include "tile_f32.h"
 
 struct hbs_grouped_plan {
     int device;
@@ -51,6 +50,11 @@ struct hbs_grouped_plan {
     const int *d_boff;
     int *d_counters;
     float *d_partials;
+    // large-tile launch (256 x 256 tiles over the 256-aligned cores)
+    int nblocks_big;
+    int64_t items_big;
+    const void *d_items_big;
+    const int *d_boff_big;
 };
 
 namespace {
@@ -73,27 +77,16 @@ struct Item {
     int64_t poff;  // float offset of the tile's partial slots (split tiles only)
 };
 
-// Tile shapes: id -> (BM, BN).  All use 256 threads as 2 x 2 waves.
-constexpr int kTileBM[4] = {128, 64, 128, 64};
-constexpr int kTileBN[4] = {128, 64, 64, 128};
+// Tile shapes: id -> (BM, BN).  Ids 0..3 run on the 256-thread launch as
+// 2 x 2 waves; id 4 (kTileBig) is the 512-thread large-tile launch.
+constexpr int kTileBM[5] = {128, 64, 128, 64, 256};
+constexpr int kTileBN[5] = {128, 64, 64, 128, 256};
 constexpr int kThreads = 256;
+constexpr int kTileBig = 4, kBig = 256, kBigThreads = 512;
 
 // ---------------------------------------------------------------------------
 // shared helpers
 // ---------------------------------------------------------------------------
-// x = hi + lo with hi = bf16_rne(x), lo = bf16_rne(x - hi); x - hi is exact.
-// v_cvt_pk_bf16_f32 rounds to nearest even; hi and lo of two floats come back
-// as one dword each (x0 in the low half).
-__device__ __forceinline__ void split_pk(float x0, float x1, uint32_t &hi, uint32_t &lo) {
-    typedef float f2 __attribute__((ext_vector_type(2)));
-    typedef __bf16 b2 __attribute__((ext_vector_type(2)));
-    const f2 v = {x0, x1};
-    const b2 hv = __builtin_convertvector(v, b2);
-    const b2 lv = __builtin_convertvector(v - __builtin_convertvector(hv, f2), b2);
-    hi = __builtin_bit_cast(uint32_t, hv);
-    lo = __builtin_bit_cast(uint32_t, lv);
-}
-
 // Epilogue for a 2x2-wave tile of FMxFN 32x32 accumulator blocks per wave.
 // Direct tiles store C; split tiles store a partial, and the last split to
 // arrive sums all partials in split order and stores C.
@@ -342,7 +335,7 @@ __device__ void b3_tile(uint16_t *smem, const Prob &p, const Item &it, float *pa
             const int f = tid + i * kThreads;
             uint32_t hi[4], lo[4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) split_pk(ra[i][2 * j], ra[i][2 * j + 1], hi[j], lo[j]);
+            for (int j = 0; j < 4; ++j) hbs::split_pk(ra[i][2 * j], ra[i][2 * j + 1], hi[j], lo[j]);
             uint16_t *d = as + (f / (BK / 8)) * ROW + (f % (BK / 8)) * 16;
             *(uint4 *)d = uint4{hi[0], hi[1], hi[2], hi[3]};
             *(uint4 *)(d + 8) = uint4{lo[0], lo[1], lo[2], lo[3]};
@@ -352,7 +345,7 @@ __device__ void b3_tile(uint16_t *smem, const Prob &p, const Item &it, float *pa
             const int f = tid + i * kThreads;
             uint32_t hi[4], lo[4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) split_pk(rb[i][2 * j], rb[i][2 * j + 1], hi[j], lo[j]);
+            for (int j = 0; j < 4; ++j) hbs::split_pk(rb[i][2 * j], rb[i][2 * j + 1], hi[j], lo[j]);
             uint16_t *d = bs + (f % BN) * ROW + (f / BN) * 16;
             *(uint4 *)d = uint4{hi[0], hi[1], hi[2], hi[3]};
             *(uint4 *)(d + 8) = uint4{lo[0], lo[1], lo[2], lo[3]};
@@ -449,6 +442,40 @@ __global__ __launch_bounds__(kThreads, PREC == HBS_PREC_FP32 ? 3 : 2) void group
 }
 
 // ---------------------------------------------------------------------------
+// Large-tile launch: one 512-thread workgroup per CU walks its list of
+// 256 x 256 tiles with the square kernels' tile code.  A problem on this path
+// is covered whole: tiles at its ragged bottom and right edges read clamped
+// rows and columns (kTileClamp) and store only what lies inside M x N.  No
+// split-K; K must be a multiple of the 32-deep stage.
+// ---------------------------------------------------------------------------
+using BigF32 = hbs::F32Cfg<256, 256, 32, 4, 2>;
+using BigB3 = hbs::B3Main;
+static_assert(BigF32::kThreads == kBigThreads && BigB3::kThreads == kBigThreads, "big tiles");
+
+template <int PREC>
+__global__ __launch_bounds__(kBigThreads) void grouped_big_kernel(const Prob *__restrict__ probs,
+                                                                  const Item *__restrict__ items,
+                                                                  const int *__restrict__ boff) {
+    constexpr int LDS = PREC == HBS_PREC_FP32 ? BigF32::kLdsFloats * 4 : 2 * BigB3::kStage * 2;
+    __shared__ __attribute__((aligned(16))) char smem[LDS];
+    const int b = blockIdx.x;
+    for (int i = boff[b]; i < boff[b + 1]; ++i) {
+        const Item it = items[i];
+        const Prob p = probs[it.prob];
+        const int m0 = it.tm * kBig, n0 = it.tn * kBig;
+        const float *Ag = p.A + (int64_t)m0 * p.lda;
+        const float *Bg = p.B + n0;
+        if constexpr (PREC == HBS_PREC_FP32)
+            hbs::f32_block_tile<256, 256, 32, 4, 2, hbs::kTileClamp>(
+                (float *)smem, Ag, Bg, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc, m0, n0);
+        else
+            hbs::bf16x3_block_tile<BigB3, hbs::kTileClamp>((uint16_t *)smem, Ag, Bg, p.C, p.M, p.N,
+                                                           p.K, p.lda, p.ldb, p.ldc, m0, n0);
+        __syncthreads();  // LDS is reused by the next item
+    }
+}
+
+// ---------------------------------------------------------------------------
 // host side: tuning table, plan construction
 // ---------------------------------------------------------------------------
 // Size class of a dimension: the power of two nearest to it in log scale,
@@ -490,7 +517,18 @@ bool load_tuning(const char *path, std::map<TuneKey, Choice> &table) {
 
 bool tile_supported(int precision, int tile) {
     (void)precision;
-    return tile >= 0 && tile <= 3;
+    return tile >= 0 && tile <= kTileBig;
+}
+
+// The large-tile path needs K a multiple of its 32-deep stage.  By default
+// BF16x3 problems take it when M, N >= 256 and K >= 256: the 256 x 256 BF16x3
+// tile runs at about twice the 128 x 128 one's rate, which outweighs the
+// padding of ragged edges.  FP32 stays on the 256-thread launch by default
+// (there the gain is smaller than the padding); tuning files and forced plans
+// can select the path for either precision.
+bool big_possible(int M, int N, int K) { (void)M; (void)N; return K % 32 == 0; }
+bool big_default(int precision, int M, int N, int K) {
+    return precision == HBS_PREC_BF16X3 && M >= kBig && N >= kBig && K >= 256 && K % 32 == 0;
 }
 
 Choice heuristic(int precision, int M, int N, int K) {
@@ -531,20 +569,54 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
     HBS_HIP_TRY(hipGetDevice(&dev));
     HBS_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
 
+    // Problems on the 256-thread launch, each with its tile shape and split-K.
+    struct Small {
+        int prob;
+        Choice c;
+    };
     std::vector<Prob> probs(count);
-    std::vector<Item> items;
-    std::vector<double> cost;
-    int64_t slots = 0, poff = 0, tiles_total = 0;
+    std::vector<Small> small;
+    std::vector<Item> items_big;
+    std::vector<double> cost_big;
+    auto lookup = [&](int M, int N, int K, bool &big) {
+        Choice c = heuristic(precision, M, N, K);
+        big = big_default(precision, M, N, K);
+        if (forced) {
+            big = forced->tile == kTileBig && big_possible(M, N, K);
+            if (forced->tile != kTileBig) c = *forced;
+            else c.split = forced->split;  // for problems that cannot take the path
+        } else if (table) {
+            auto f = table->find(TuneKey(precision, size_class(M), size_class(N), size_class(K)));
+            if (f != table->end() && tile_supported(precision, f->second.tile)) {
+                big = f->second.tile == kTileBig && big_possible(M, N, K);
+                if (f->second.tile != kTileBig) c = f->second;
+            }
+        }
+        return c;
+    };
     for (int i = 0; i < count; ++i) {
         const hbs_gemm_problem &q = problems[i];
         probs[i] = Prob{q.A, q.B, q.C, q.M, q.N, q.K, q.lda, q.ldb, q.ldc};
-        Choice c = heuristic(precision, q.M, q.N, q.K);
-        if (forced) {
-            c = *forced;
-        } else if (table) {
-            auto f = table->find(TuneKey(precision, size_class(q.M), size_class(q.N), size_class(q.K)));
-            if (f != table->end() && tile_supported(precision, f->second.tile)) c = f->second;
+        bool big = false;
+        const Choice c = lookup(q.M, q.N, q.K, big);
+        if (!big) {
+            small.push_back(Small{i, c});
+            continue;
         }
+        // the whole problem in 256 x 256 tiles; edge tiles clamp their loads
+        for (int tm = 0; tm < (q.M + kBig - 1) / kBig; ++tm)
+            for (int tn = 0; tn < (q.N + kBig - 1) / kBig; ++tn) {
+                items_big.push_back(Item{i, tm, tn, 0, q.K, kTileBig, 1, 0, -1, 0});
+                cost_big.push_back((double)q.K);
+            }
+    }
+
+    std::vector<Item> items;
+    std::vector<double> cost;
+    int64_t slots = 0, poff = 0, tiles_total = (int64_t)items_big.size();
+    for (const Small &sp : small) {
+        const Prob &q = probs[sp.prob];
+        const Choice &c = sp.c;
         const int BM = kTileBM[c.tile], BN = kTileBN[c.tile];
         const int BK = precision == HBS_PREC_FP32 ? 16 : 32;
         const int tm_n = (q.M + BM - 1) / BM, tn_n = (q.N + BN - 1) / BN;
@@ -563,7 +635,7 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
                 ++tiles_total;
                 for (int s = 0; s < ns; ++s) {
                     Item it;
-                    it.prob = i;
+                    it.prob = sp.prob;
                     it.tm = tm;
                     it.tn = tn;
                     it.k0 = s * chunk;
@@ -587,50 +659,70 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
     }
     if (items.size() > 0x7fffffffULL) return HBS_ERR_ARG;
 
-    // LPT: items in decreasing cost, each to the least-loaded workgroup.
-    // resident workgroups per CU: FP32 tiles use <= 168 VGPRs and 38 KB LDS
-    // (three fit), BF16x3 tiles <= 256 VGPRs and 74 KB LDS (two fit)
+    if (items.size() > 0x7fffffffULL || items_big.size() > 0x7fffffffULL) return HBS_ERR_ARG;
+
+    // LPT: items in decreasing cost, each to the least-loaded workgroup; the
+    // item lists of the workgroups are concatenated with offsets boff.
+    auto lpt = [](const std::vector<Item> &its, const std::vector<double> &cst, int nb,
+                  std::vector<Item> &sorted, std::vector<int> &boff, double &imbalance) {
+        std::vector<int64_t> order(its.size());
+        for (size_t i = 0; i < order.size(); ++i) order[i] = (int64_t)i;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int64_t x, int64_t y) { return cst[x] > cst[y]; });
+        using Load = std::pair<double, int>;
+        std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
+        for (int b = 0; b < nb; ++b) heap.push(Load(0.0, b));
+        std::vector<std::vector<int64_t>> lists(nb);
+        for (int64_t idx : order) {
+            Load l = heap.top();
+            heap.pop();
+            lists[l.second].push_back(idx);
+            l.first += cst[idx];
+            heap.push(l);
+        }
+        double maxl = 0.0, suml = 0.0;
+        while (!heap.empty()) {
+            maxl = std::max(maxl, heap.top().first);
+            suml += heap.top().first;
+            heap.pop();
+        }
+        imbalance = suml > 0.0 ? maxl / (suml / nb) : 1.0;
+        sorted.clear();
+        sorted.reserve(its.size());
+        boff.assign(nb + 1, 0);
+        for (int b = 0; b < nb; ++b) {
+            boff[b] = (int)sorted.size();
+            for (int64_t idx : lists[b]) sorted.push_back(its[idx]);
+        }
+        boff[nb] = (int)sorted.size();
+    };
+    // resident workgroups per CU on the 256-thread launch: FP32 tiles use
+    // <= 168 VGPRs and 38 KB LDS (three fit), BF16x3 tiles <= 256 VGPRs and
+    // 74 KB LDS (two fit); the large-tile launch holds one per CU
     const int per_cu = precision == HBS_PREC_FP32 ? 3 : 2;
-    const int nblocks = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)per_cu * cus, (int64_t)items.size()));
-    std::vector<int64_t> order(items.size());
-    for (size_t i = 0; i < order.size(); ++i) order[i] = (int64_t)i;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int64_t a, int64_t b) { return cost[a] > cost[b]; });
-    using Load = std::pair<double, int>;
-    std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
-    for (int b = 0; b < nblocks; ++b) heap.push(Load(0.0, b));
-    std::vector<std::vector<int64_t>> lists(nblocks);
-    for (int64_t idx : order) {
-        Load l = heap.top();
-        heap.pop();
-        lists[l.second].push_back(idx);
-        l.first += cost[idx];
-        heap.push(l);
-    }
-    double maxl = 0.0, suml = 0.0;
-    while (!heap.empty()) {
-        maxl = std::max(maxl, heap.top().first);
-        suml += heap.top().first;
-        heap.pop();
-    }
-    std::vector<Item> sorted;
-    sorted.reserve(items.size());
-    std::vector<int> boff(nblocks + 1, 0);
-    for (int b = 0; b < nblocks; ++b) {
-        boff[b] = (int)sorted.size();
-        for (int64_t idx : lists[b]) sorted.push_back(items[idx]);
-    }
-    boff[nblocks] = (int)sorted.size();
+    const int nblocks = (int)std::min<int64_t>((int64_t)per_cu * cus, (int64_t)items.size());
+    const int nblocks_big = (int)std::min<int64_t>(cus, (int64_t)items_big.size());
+    std::vector<Item> sorted, sorted_big;
+    std::vector<int> boff, boff_big;
+    double imb = 1.0, imb_big = 1.0;
+    lpt(items, cost, nblocks, sorted, boff, imb);
+    lpt(items_big, cost_big, nblocks_big, sorted_big, boff_big, imb_big);
 
     // one device allocation: problems | items | offsets | counters | partials
+    // | large-tile items | large-tile offsets
     auto align = [](size_t x) { return (x + 255) / 256 * 256; };
     const size_t b_probs = align(probs.size() * sizeof(Prob));
-    const size_t b_items = align(sorted.size() * sizeof(Item));
+    const size_t b_items = align(std::max<size_t>(sorted.size(), 1) * sizeof(Item));
     const size_t b_boff = align(boff.size() * sizeof(int));
     const size_t b_cnt = align(std::max<int64_t>(slots, 1) * sizeof(int));
     const size_t b_part = align(std::max<int64_t>(poff, 1) * sizeof(float));
+    const size_t b_items_big = align(std::max<size_t>(sorted_big.size(), 1) * sizeof(Item));
+    const size_t b_boff_big = align(boff_big.size() * sizeof(int));
+    const size_t o_items = b_probs, o_boff = o_items + b_items, o_cnt = o_boff + b_boff;
+    const size_t o_part = o_cnt + b_cnt, o_items_big = o_part + b_part;
+    const size_t o_boff_big = o_items_big + b_items_big, total = o_boff_big + b_boff_big;
     char *dmem = nullptr;
-    HBS_HIP_TRY(hipMalloc(&dmem, b_probs + b_items + b_boff + b_cnt + b_part));
+    HBS_HIP_TRY(hipMalloc(&dmem, total));
     hbs_grouped_plan *plan = new hbs_grouped_plan();
     plan->device = dev;
     plan->precision = precision;
@@ -639,18 +731,28 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
     plan->items = (int64_t)sorted.size();
     plan->tiles = tiles_total;
     plan->split_tiles = slots;
-    plan->est_imbalance = suml > 0.0 ? maxl / (suml / nblocks) : 1.0;
+    plan->est_imbalance = std::max(nblocks > 0 ? imb : 1.0, nblocks_big > 0 ? imb_big : 1.0);
     plan->dmem = dmem;
     plan->d_probs = dmem;
-    plan->d_items = dmem + b_probs;
-    plan->d_boff = (const int *)(dmem + b_probs + b_items);
-    plan->d_counters = (int *)(dmem + b_probs + b_items + b_boff);
-    plan->d_partials = (float *)(dmem + b_probs + b_items + b_boff + b_cnt);
+    plan->d_items = dmem + o_items;
+    plan->d_boff = (const int *)(dmem + o_boff);
+    plan->d_counters = (int *)(dmem + o_cnt);
+    plan->d_partials = (float *)(dmem + o_part);
+    plan->nblocks_big = nblocks_big;
+    plan->items_big = (int64_t)sorted_big.size();
+    plan->d_items_big = dmem + o_items_big;
+    plan->d_boff_big = (const int *)(dmem + o_boff_big);
     hipError_t e = hipMemcpy(dmem, probs.data(), probs.size() * sizeof(Prob), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !sorted.empty())
+        e = hipMemcpy(dmem + o_items, sorted.data(), sorted.size() * sizeof(Item),
+                      hipMemcpyHostToDevice);
     if (e == hipSuccess)
-        e = hipMemcpy(dmem + b_probs, sorted.data(), sorted.size() * sizeof(Item), hipMemcpyHostToDevice);
+        e = hipMemcpy(dmem + o_boff, boff.data(), boff.size() * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !sorted_big.empty())
+        e = hipMemcpy(dmem + o_items_big, sorted_big.data(), sorted_big.size() * sizeof(Item),
+                      hipMemcpyHostToDevice);
     if (e == hipSuccess)
-        e = hipMemcpy(dmem + b_probs + b_items, boff.data(), boff.size() * sizeof(int),
+        e = hipMemcpy(dmem + o_boff_big, boff_big.data(), boff_big.size() * sizeof(int),
                       hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(plan->d_counters, 0, b_cnt);
     if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -690,9 +792,9 @@ extern "C" int hbs_grouped_plan_create_forced(const hbs_gemm_problem *problems, 
 extern "C" int hbs_grouped_plan_info(const hbs_grouped_plan *plan, hbs_grouped_info *info) {
     if (!plan || !info) return HBS_ERR_NULL;
     info->problems = plan->count;
-    info->workgroups = plan->nblocks;
+    info->workgroups = plan->nblocks + plan->nblocks_big;
     info->tiles = plan->tiles;
-    info->work_items = plan->items;
+    info->work_items = plan->items + plan->items_big;
     info->split_tiles = plan->split_tiles;
     info->est_imbalance = plan->est_imbalance;
     return HBS_SUCCESS;
@@ -706,15 +808,28 @@ extern "C" int hbs_grouped_run(const hbs_grouped_plan *plan, void *stream) {
     hipStream_t s = (hipStream_t)stream;
     const Prob *pr = (const Prob *)plan->d_probs;
     const Item *it = (const Item *)plan->d_items;
-    if (plan->precision == HBS_PREC_FP32)
-        grouped_kernel<HBS_PREC_FP32><<<plan->nblocks, kThreads, 0, s>>>(pr, it, plan->d_boff,
-                                                                         plan->d_partials,
-                                                                         plan->d_counters);
-    else
-        grouped_kernel<HBS_PREC_BF16X3><<<plan->nblocks, kThreads, 0, s>>>(pr, it, plan->d_boff,
-                                                                           plan->d_partials,
-                                                                           plan->d_counters);
-    return hbs::launch_status();
+    const Item *itb = (const Item *)plan->d_items_big;
+    const bool fp32 = plan->precision == HBS_PREC_FP32;
+    if (plan->nblocks_big > 0) {
+        if (fp32)
+            grouped_big_kernel<HBS_PREC_FP32><<<plan->nblocks_big, kBigThreads, 0, s>>>(
+                pr, itb, plan->d_boff_big);
+        else
+            grouped_big_kernel<HBS_PREC_BF16X3><<<plan->nblocks_big, kBigThreads, 0, s>>>(
+                pr, itb, plan->d_boff_big);
+        const int rc = hbs::launch_status();
+        if (rc) return rc;
+    }
+    if (plan->nblocks > 0) {
+        if (fp32)
+            grouped_kernel<HBS_PREC_FP32><<<plan->nblocks, kThreads, 0, s>>>(
+                pr, it, plan->d_boff, plan->d_partials, plan->d_counters);
+        else
+            grouped_kernel<HBS_PREC_BF16X3><<<plan->nblocks, kThreads, 0, s>>>(
+                pr, it, plan->d_boff, plan->d_partials, plan->d_counters);
+        return hbs::launch_status();
+    }
+    return HBS_SUCCESS;
 }
 
 extern "C" int hbs_grouped_plan_destroy(hbs_grouped_plan *plan) {

@@ -17,6 +17,16 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 namespace hbs {
 
+// Load modes of a block tile (tile_f32.h, tile_bf16x3.h):
+//   kTileInterior  the tile lies inside M x N, K is a multiple of the stage
+//                  depth and the operands allow 16-byte loads: no predicates;
+//   kTileEdge      any shape and alignment: per-element predicates, zero fill;
+//   kTileClamp     ragged M or N, with K a multiple of the stage depth and
+//                  16-byte-loadable operands: rows and columns beyond M or N
+//                  are read from the last valid one (clamped addresses at full
+//                  load width) and only the stores are predicated.
+enum TileMode { kTileInterior = 0, kTileEdge = 1, kTileClamp = 2 };
+
 // Checks that the current device is a gfx950 GPU (cached per device).
 int check_device();
 

@@ -1,0 +1,216 @@
+// tile_bf16x3.h -- one BF16x3 output tile, shared by the square / rectangular
+// BF16x3 kernel (sgemm_bf16x3.hip) and the grouped kernel's large-tile launch.
+//
+// Each float is split x = hi + lo, hi = bf16_rne(x), lo = bf16_rne(x - hi)
+// (x - hi is exact), and C = Alo.Bhi + Ahi.Blo + Ahi.Bhi is accumulated in
+// FP32 by v_mfma_f32_32x32x16_bf16; the lo.lo term (relative size 2^-16) is
+// dropped.  The split happens in registers on the way to LDS.
+#pragma once
+
+#include "hbs_internal.h"
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+namespace hbs {
+
+template <int BM_, int BN_, int BK_, int WM_, int WN_>
+struct B3Cfg {
+    static constexpr int BM = BM_, BN = BN_, BK = BK_, WM = WM_, WN = WN_;
+    static constexpr int kThreads = WM * WN * 64;
+    static constexpr int kWTM = BM / WM, kWTN = BN / WN;
+    static constexpr int kFM = kWTM / 32, kFN = kWTN / 32;
+    // LDS row of one tile row (A) or column (B): per 8-k group, 8 hi then 8 lo
+    // bf16, plus 16 bytes of padding (conflict-free 128-bit fragment reads)
+    static constexpr int kLdsRow = BK * 2 + 8;
+    static constexpr int kStage = (BM + BN) * kLdsRow;    // bf16 elements per stage
+    static_assert(kWTM % 32 == 0 && kWTN % 32 == 0, "wave tile");
+    static_assert(2 * kStage * 2 <= 160 * 1024, "LDS");
+};
+// Production configuration: 256 x 256 tile, BK = 32, 8 waves of 64 x 128 (the
+// fastest of the shapes measured at N = 8192, NITER = 10).
+using B3Main = B3Cfg<256, 256, 32, 4, 2>;
+
+// hi and lo planes of two floats, each as one dword holding two bf16 (x0 low).
+__device__ __forceinline__ void split_pk(float x0, float x1, uint32_t &hi, uint32_t &lo) {
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    typedef __bf16 b2 __attribute__((ext_vector_type(2)));
+    const f2 v = {x0, x1};
+    const b2 h = __builtin_convertvector(v, b2);
+    const b2 l = __builtin_convertvector(v - __builtin_convertvector(h, f2), b2);
+    hi = __builtin_bit_cast(uint32_t, h);
+    lo = __builtin_bit_cast(uint32_t, l);
+}
+
+// Each workgroup loads its A (rows x 32 k) and B (32 k x cols) float32 tiles,
+// splits every element in registers and stores the planes to LDS rows that
+// are K-contiguous for both operands, as the MFMA fragments read them.
+//   A item: one row, 8 consecutive k (two 16-byte loads).
+//   B item: one column, 8 consecutive k (8 dword loads; the 64 lanes of a wave
+//           read 64 adjacent columns, so each load is one 256-byte row segment).
+// MODE (TileMode): kTileInterior, kTileEdge (ragged M, N or K, or A not
+// 16-byte loadable; out-of-range elements are zero) or kTileClamp (ragged M or
+// N; K a multiple of 32 and A 16-byte loadable).
+// Ag = A + m0 * lda (first row of the tile), Bg = B + n0 (first column),
+// Cg = C (the matrix); M, N, K bound the predicates.  smem holds
+// 2 * G::kStage bf16.  Every thread of the workgroup must call it.
+template <class G, int MODE>
+__device__ __forceinline__ void bf16x3_block_tile(uint16_t *smem, const float *__restrict__ Ag,
+                                                  const float *__restrict__ Bg,
+                                                  float *__restrict__ Cg, int M, int N, int K,
+                                                  int lda, int ldb, int ldc, int m0, int n0) {
+    constexpr int kBM = G::BM, kBN = G::BN, kBK = G::BK, kWM = G::WM, kThreads = G::kThreads;
+    constexpr int kWTM = G::kWTM, kWTN = G::kWTN, kFM = G::kFM, kFN = G::kFN;
+    constexpr int kLdsRow = G::kLdsRow, kStage = G::kStage;
+    constexpr int kGroups = kBK / 8;                       // 8-k groups per tile row
+    constexpr int kItemsA = kBM * kGroups / kThreads;
+    constexpr int kItemsB = kBN * kGroups / kThreads;
+    static_assert(kBM * kGroups % kThreads == 0 && kBN * kGroups % kThreads == 0, "items");
+    static_assert(kBN % 64 == 0 && kThreads % kBN == 0, "B items: one k group per wave");
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm = wave % kWM, wn = wave / kWM;
+    const int l32 = lane & 31, h = lane >> 5;
+
+    floatx16 acc[kFM][kFN];
+#pragma unroll
+    for (int r = 0; r < kFM; ++r)
+#pragma unroll
+        for (int c = 0; c < kFN; ++c) acc[r][c] = floatx16{0};
+
+    float ra[kItemsA][8], rb[kItemsB][8];
+
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < kItemsA; ++i) {
+            const int f = tid + i * kThreads;
+            const int row = f / kGroups, gk = k0 + (f % kGroups) * 8;
+            const int srow = MODE == kTileClamp ? min(row, M - m0 - 1) : row;
+            const float *x = Ag + (int64_t)srow * lda + gk;
+            if (MODE != kTileEdge) {
+                const floatx4 v0 = *(const floatx4 *)x, v1 = *(const floatx4 *)(x + 4);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    ra[i][j] = v0[j];
+                    ra[i][j + 4] = v1[j];
+                }
+            } else {
+                const bool rok = m0 + row < M;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) ra[i][j] = (rok && gk + j < K) ? x[j] : 0.0f;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < kItemsB; ++i) {
+            const int f = tid + i * kThreads;
+            const unsigned col =
+                MODE == kTileClamp ? (unsigned)min(f % kBN, N - n0 - 1) : (unsigned)(f % kBN);
+            // the k group is the same for the whole wave (kBN is a multiple of 64)
+            // and each k row pointer is uniform: scalar base + 32-bit lane offset
+            const int gk = k0 + __builtin_amdgcn_readfirstlane(f / kBN) * 8;
+            const bool cok = MODE != kTileEdge || n0 + (int)col < N;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const float *rowp = Bg + (int64_t)(gk + j) * ldb;
+                rb[i][j] = (MODE != kTileEdge || (cok && gk + j < K)) ? rowp[col] : 0.0f;
+            }
+        }
+    };
+    auto sstore = [&](int buf) {
+        uint16_t *as = smem + buf * kStage;
+        uint16_t *bsm = as + kBM * kLdsRow;
+#pragma unroll
+        for (int i = 0; i < kItemsA; ++i) {
+            const int f = tid + i * kThreads;
+            const int row = f / kGroups, g = f % kGroups;
+            uint32_t hi[4], lo[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) split_pk(ra[i][2 * j], ra[i][2 * j + 1], hi[j], lo[j]);
+            uint16_t *d = as + row * kLdsRow + g * 16;
+            *(uint4 *)d = uint4{hi[0], hi[1], hi[2], hi[3]};
+            *(uint4 *)(d + 8) = uint4{lo[0], lo[1], lo[2], lo[3]};
+        }
+#pragma unroll
+        for (int i = 0; i < kItemsB; ++i) {
+            const int f = tid + i * kThreads;
+            const int col = f % kBN, g = f / kBN;
+            uint32_t hi[4], lo[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) split_pk(rb[i][2 * j], rb[i][2 * j + 1], hi[j], lo[j]);
+            uint16_t *d = bsm + col * kLdsRow + g * 16;
+            *(uint4 *)d = uint4{hi[0], hi[1], hi[2], hi[3]};
+            *(uint4 *)(d + 8) = uint4{lo[0], lo[1], lo[2], lo[3]};
+        }
+    };
+    // one 16-deep substep of the stage in buffer `buf`
+    auto compute = [&](int buf, int st) {
+        const uint16_t *as = smem + buf * kStage + (wm * kWTM + l32) * kLdsRow;
+        const uint16_t *bsm = smem + buf * kStage + kBM * kLdsRow + (wn * kWTN + l32) * kLdsRow;
+        const int off = (2 * st + h) * 16;  // this lane half's 8-k group
+        bf16x8 ahi[kFM], alo[kFM], bhi[kFN], blo[kFN];
+#pragma unroll
+        for (int r = 0; r < kFM; ++r) {
+            ahi[r] = *(const bf16x8 *)(as + r * 32 * kLdsRow + off);
+            alo[r] = *(const bf16x8 *)(as + r * 32 * kLdsRow + off + 8);
+        }
+#pragma unroll
+        for (int c = 0; c < kFN; ++c) {
+            bhi[c] = *(const bf16x8 *)(bsm + c * 32 * kLdsRow + off);
+            blo[c] = *(const bf16x8 *)(bsm + c * 32 * kLdsRow + off + 8);
+        }
+#pragma unroll
+        for (int r = 0; r < kFM; ++r)
+#pragma unroll
+            for (int c = 0; c < kFN; ++c) {
+                acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[r], bhi[c], acc[r][c], 0, 0, 0);
+                acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[r], blo[c], acc[r][c], 0, 0, 0);
+                acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[r], bhi[c], acc[r][c], 0, 0, 0);
+            }
+    };
+    static_assert(kBK == 32, "two substeps per stage");
+
+    // Pipeline: the global loads of stage kt + 2 are issued in the middle of
+    // step kt, right after stage kt + 1 has been split into the other LDS
+    // buffer, so each load has a whole step to arrive; the split of one
+    // stage runs between the two substeps of the previous one.
+    const int KT = (K + kBK - 1) / kBK;
+    gload(0);
+    sstore(0);
+    if (KT > 1) gload(kBK);
+    __syncthreads();
+    int kt = 0;
+    for (; kt + 2 < KT; ++kt) {
+        const int cur = kt & 1;
+        compute(cur, 0);
+        sstore(cur ^ 1);
+        gload((kt + 2) * kBK);
+        compute(cur, 1);
+        __syncthreads();
+    }
+    if (kt + 1 < KT) {
+        const int cur = kt & 1;
+        compute(cur, 0);
+        sstore(cur ^ 1);
+        compute(cur, 1);
+        __syncthreads();
+        ++kt;
+    }
+    compute(kt & 1, 0);
+    compute(kt & 1, 1);
+
+#pragma unroll
+    for (int r = 0; r < kFM; ++r) {
+#pragma unroll
+        for (int c = 0; c < kFN; ++c) {
+            const int col = n0 + wn * kWTN + c * 32 + l32;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int row = m0 + wm * kWTM + r * 32 + 8 * (q >> 2) + 4 * h + (q & 3);
+                if (MODE == kTileInterior || (row < M && col < N))
+                    Cg[(int64_t)row * ldc + col] = acc[r][c][q];
+            }
+        }
+    }
+}
+
+}  // namespace hbs

@@ -1,0 +1,218 @@
+// tile_f32.h -- one FP32-MFMA output tile, shared by the square / strided
+// SGEMM kernel (sgemm_f32.hip) and the grouped kernel's large-tile launch.
+//
+// A workgroup of WM x WN waves computes the BM x BN tile at (m0, n0) of
+// C = A . B over the whole K:
+//   * global -> registers -> LDS double buffer, 128-bit coalesced loads; each
+//     K stage is loaded a full step ahead and stored to LDS in the middle of
+//     the previous step;
+//   * each wave owns a (BM/WM) x (BN/WN) sub-tile held as 32x32 accumulator
+//     blocks of v_mfma_f32_32x32x2_f32 (FP32 in, FP32 accumulate);
+//   * an A fragment for four consecutive MFMAs comes from one 128-bit LDS read
+//     (lane half h takes k = 4h..4h+3 of each 8-wide k chunk).
+#pragma once
+
+#include "hbs_internal.h"
+
+namespace hbs {
+
+template <int BM, int BN, int BK, int WM, int WN>
+struct F32Cfg {
+    static constexpr int kThreads = WM * WN * 64;
+    static constexpr int kWTM = BM / WM;  // wave tile rows
+    static constexpr int kWTN = BN / WN;  // wave tile cols
+    static constexpr int kFM = kWTM / 32; // 32x32 blocks per wave (rows)
+    static constexpr int kFN = kWTN / 32; // 32x32 blocks per wave (cols)
+    static constexpr int kLdA = BK + 4;   // LDS row stride of the [BM][BK] A tile
+    static constexpr int kLdB = BN + 8;   // LDS row stride of the [BK][BN] B tile
+    static constexpr int kAF4 = (BM * BK / 4) / kThreads;
+    static constexpr int kBF4 = (BK * BN / 4) / kThreads;
+    static constexpr int kStageA = BM * kLdA;
+    static constexpr int kStageB = BK * kLdB;
+    static constexpr int kLdsFloats = 2 * (kStageA + kStageB);
+    static_assert(kWTM % 32 == 0 && kWTN % 32 == 0, "wave tile must be a multiple of 32");
+    static_assert((BM * BK / 4) % kThreads == 0, "A tile must split evenly over threads");
+    static_assert((BK * BN / 4) % kThreads == 0, "B tile must split evenly over threads");
+    static_assert(BK % 16 == 0, "BK must be a multiple of 16 (two or more 8-k chunks)");
+};
+
+// Ag = A + m0 * lda (first row of the tile), Bg = B + n0 (first column),
+// Cg = C (the matrix); M, N, K bound the predicates.  smem holds
+// F32Cfg::kLdsFloats floats.  Every thread of the workgroup must call it.
+// MODE (TileMode): kTileInterior, kTileEdge (any shape or alignment) or
+// kTileClamp (ragged M or N; K a multiple of BK, N a multiple of 4, 16-byte
+// loadable operands).
+template <int BM, int BN, int BK, int WM, int WN, int MODE>
+__device__ __forceinline__ void f32_block_tile(float *smem, const float *__restrict__ Ag,
+                                               const float *__restrict__ Bg,
+                                               float *__restrict__ Cg, int M, int N, int K,
+                                               int lda, int ldb, int ldc, int m0, int n0) {
+    using G = F32Cfg<BM, BN, BK, WM, WN>;
+    float *const As = smem;
+    float *const Bs = smem + 2 * G::kStageA;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm = wave % WM, wn = wave / WM;
+    const int l32 = lane & 31, h = lane >> 5;
+
+    floatx16 acc[G::kFM][G::kFN];
+#pragma unroll
+    for (int r = 0; r < G::kFM; ++r)
+#pragma unroll
+        for (int c = 0; c < G::kFN; ++c) acc[r][c] = floatx16{0};
+
+    floatx4 ra[G::kAF4], rb[G::kBF4];
+
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < G::kAF4; ++i) {
+            const int f = tid + i * G::kThreads;
+            const int row = f / (BK / 4), c4 = f % (BK / 4);
+            if constexpr (MODE != kTileEdge) {
+                const int srow = MODE == kTileClamp ? min(row, M - m0 - 1) : row;
+                ra[i] = *(const floatx4 *)(Ag + (int64_t)srow * lda + k0 + c4 * 4);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int gk = k0 + c4 * 4 + j;
+                    ra[i][j] = (m0 + row < M && gk < K) ? Ag[(int64_t)row * lda + gk] : 0.0f;
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < G::kBF4; ++i) {
+            const int f = tid + i * G::kThreads;
+            const int kr = f / (BN / 4), c4 = f % (BN / 4);
+            if constexpr (MODE != kTileEdge) {
+                const int sc = MODE == kTileClamp ? min(c4 * 4, N - n0 - 4) : c4 * 4;
+                rb[i] = *(const floatx4 *)(Bg + (int64_t)(k0 + kr) * ldb + sc);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int gn = n0 + c4 * 4 + j;
+                    rb[i][j] = (k0 + kr < K && gn < N) ? Bg[(int64_t)(k0 + kr) * ldb + c4 * 4 + j]
+                                                          : 0.0f;
+                }
+            }
+        }
+    };
+
+    auto sstore = [&](int buf) {
+        float *as = As + buf * G::kStageA;
+        float *bs = Bs + buf * G::kStageB;
+#pragma unroll
+        for (int i = 0; i < G::kAF4; ++i) {
+            const int f = tid + i * G::kThreads;
+            const int row = f / (BK / 4), c4 = f % (BK / 4);
+            *(floatx4 *)(as + row * G::kLdA + c4 * 4) = ra[i];
+        }
+#pragma unroll
+        for (int i = 0; i < G::kBF4; ++i) {
+            const int f = tid + i * G::kThreads;
+            const int kr = f / (BN / 4), c4 = f % (BN / 4);
+            *(floatx4 *)(bs + kr * G::kLdB + c4 * 4) = rb[i];
+        }
+    };
+
+    // Fragments for one 8-wide k chunk: A from one 128-bit read per 32-row
+    // block, B as four 32-bit reads per 32-col block.  The reads for chunk
+    // kc + 1 are issued before the MFMAs of chunk kc so LDS latency overlaps
+    // matrix-core work.
+    auto load_frag = [&](const float *as, const float *bs, int kc, floatx4 (&a)[G::kFM],
+                         float (&b)[G::kFN][4]) {
+#pragma unroll
+        for (int r = 0; r < G::kFM; ++r) a[r] = *(const floatx4 *)(as + r * 32 * G::kLdA + kc * 8);
+#pragma unroll
+        for (int c = 0; c < G::kFN; ++c)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) b[c][q] = bs[(kc * 8 + q) * G::kLdB + c * 32];
+    };
+    auto mfma_chunk = [&](const floatx4 (&a)[G::kFM], const float (&b)[G::kFN][4]) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int r = 0; r < G::kFM; ++r)
+#pragma unroll
+                for (int c = 0; c < G::kFN; ++c)
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[r][q], b[c][q], acc[r][c], 0, 0, 0);
+    };
+    // Scheduling hint: spread the next chunk's LDS reads (kFM 128-bit and
+    // 2 kFN paired 32-bit reads) between this chunk's 4 kFM kFN MFMAs.
+    auto interleave = [&]() {
+        constexpr int kReads = G::kFM + 2 * G::kFN;
+        constexpr int kMfma = 4 * G::kFM * G::kFN;
+        constexpr int kPer = kMfma / kReads > 0 ? kMfma / kReads : 1;
+#pragma unroll
+        for (int i = 0; i < kReads; ++i) {
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);     // one LDS read
+            __builtin_amdgcn_sched_group_barrier(0x008, kPer, 0);  // then MFMAs
+        }
+    };
+    // One stage (BK/8 chunks of 8 k) from LDS buffer `buf`, the fragments of
+    // chunk kc + 1 requested before the MFMAs of chunk kc.  `hook` runs after
+    // the MFMAs of chunk kHook - 1, once the fragments of chunk kHook are in
+    // flight, so the matrix core is never left waiting on a fresh LDS read.
+    constexpr int kChunks = BK / 8, kHook = kChunks / 2;
+    auto compute = [&](int buf, auto &&hook) {
+        const float *as = As + buf * G::kStageA + (wm * G::kWTM + l32) * G::kLdA + 4 * h;
+        const float *bs = Bs + buf * G::kStageB + 4 * h * G::kLdB + wn * G::kWTN + l32;
+        floatx4 a0[G::kFM], a1[G::kFM];
+        float b0[G::kFN][4], b1[G::kFN][4];
+        load_frag(as, bs, 0, a0, b0);
+#pragma unroll
+        for (int kc = 0; kc < kChunks; kc += 2) {
+            if (kc + 1 < kChunks) load_frag(as, bs, kc + 1, a1, b1);
+            mfma_chunk(a0, b0);
+            interleave();
+            if (kc + 1 == kHook) hook();
+            if (kc + 2 < kChunks) load_frag(as, bs, kc + 2, a0, b0);
+            if (kc + 1 < kChunks) mfma_chunk(a1, b1);
+            interleave();
+            if (kc + 2 == kHook) hook();
+        }
+    };
+
+    // Pipeline: stage kt + 1 is stored to the other LDS buffer in the middle
+    // of step kt and the global loads of stage kt + 2 are issued right after,
+    // so every load has a whole step to arrive and the store overlaps MFMAs.
+    const int KT = (K + BK - 1) / BK;
+    gload(0);
+    sstore(0);
+    if (KT > 1) gload(BK);
+    __syncthreads();
+    int kt = 0;
+    for (; kt + 2 < KT; ++kt) {
+        const int cur = kt & 1;
+        compute(cur, [&]() {
+            sstore(cur ^ 1);
+            gload((kt + 2) * BK);
+        });
+        __syncthreads();
+    }
+    if (kt + 1 < KT) {
+        const int cur = kt & 1;
+        compute(cur, [&]() { sstore(cur ^ 1); });
+        __syncthreads();
+        ++kt;
+    }
+    compute(kt & 1, [&]() {});
+
+    // Epilogue: 32x32 accumulator layout -- register q of lane l holds
+    // row 8*(q/4) + 4*(l/32) + q%4, col l%32 of its block.
+#pragma unroll
+    for (int r = 0; r < G::kFM; ++r) {
+#pragma unroll
+        for (int c = 0; c < G::kFN; ++c) {
+            const int col = n0 + wn * G::kWTN + c * 32 + l32;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int row = m0 + wm * G::kWTM + r * 32 + 8 * (q >> 2) + 4 * h + (q & 3);
+                if (MODE == kTileInterior || (row < M && col < N))
+                    Cg[(int64_t)row * ldc + col] = acc[r][c][q];
+            }
+        }
+    }
+}
+
+}  // namespace hbs

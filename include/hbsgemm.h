@@ -100,14 +100,23 @@ typedef struct {
 } hbs_gemm_problem;
 
 enum { HBS_PREC_FP32 = 0, HBS_PREC_BF16X3 = 1 };
-/* Tile shapes of the grouped kernel (BM x BN). */
-enum { HBS_TILE_128x128 = 0, HBS_TILE_64x64 = 1, HBS_TILE_128x64 = 2, HBS_TILE_64x128 = 3 };
+/* Tile shapes of the grouped kernel (BM x BN).  0..3 run on the 256-thread
+ * persistent launch (with optional split-K).  HBS_TILE_256x256 runs the whole
+ * problem on the 512-thread large-tile launch with the square kernels' tile
+ * (K a multiple of 32; no split-K; edge tiles clamp their loads). */
+enum {
+    HBS_TILE_128x128 = 0,
+    HBS_TILE_64x64 = 1,
+    HBS_TILE_128x64 = 2,
+    HBS_TILE_64x128 = 3,
+    HBS_TILE_256x256 = 4
+};
 
 typedef struct hbs_grouped_plan hbs_grouped_plan;
 
 typedef struct {
     int problems;
-    int workgroups;       /* persistent workgroups of one run               */
+    int workgroups;       /* persistent workgroups of one run (both launches) */
     int64_t tiles;        /* output tiles over all problems                 */
     int64_t work_items;   /* tiles x K splits                               */
     int64_t split_tiles;  /* tiles computed with split-K                    */
@@ -119,19 +128,22 @@ typedef struct {
  * problem: M, N, K >= 1, lda >= K, ldb >= N, ldc >= N; A and B 16-byte aligned
  * with N, K, lda, ldb multiples of 4 (HBS_ERR_ALIGN otherwise).  Tile shape
  * and split-K come from `tuning_file` (written by the autotuner) when it is
- * non-NULL -- HBS_ERR_ARG if it cannot be read -- else from a size heuristic.
- * Allocates and uploads the plan's device buffers (synchronous). */
+ * non-NULL -- HBS_ERR_ARG if it cannot be read -- else from a size heuristic
+ * (for BF16x3, HBS_TILE_256x256 when M, N >= 256, K >= 256 and K is a multiple
+ * of 32).  Allocates and uploads the plan's device buffers (synchronous). */
 int hbs_grouped_plan_create(const hbs_gemm_problem *problems, int count, int precision,
                             const char *tuning_file, hbs_grouped_plan **plan);
 /* Same with one tile shape and split-K factor (1..64) for every problem (the
- * autotuner's probe). */
+ * autotuner's probe).  With HBS_TILE_256x256, problems that cannot take the
+ * large-tile path (K not a multiple of 32) use the size heuristic's tile with
+ * this split-K. */
 int hbs_grouped_plan_create_forced(const hbs_gemm_problem *problems, int count, int precision,
                                    int tile, int split_k, hbs_grouped_plan **plan);
 int hbs_grouped_plan_info(const hbs_grouped_plan *plan, hbs_grouped_info *info);
-/* Computes every problem of the plan on `stream`: one persistent kernel launch
- * (FP32), or a packing pre-pass plus the persistent launch (BF16x3; the packed
- * hi/lo operands live in plan memory).  Deterministic; runs of one plan must
- * not overlap (they share counters and packed buffers). */
+/* Computes every problem of the plan on `stream`: the large-tile launch (when
+ * the plan has 256 x 256 tiles), then the 256-thread persistent launch.  Both
+ * precisions read the operands in place.  Deterministic; runs of one plan must
+ * not overlap (they share split-K counters and partial buffers). */
 int hbs_grouped_run(const hbs_grouped_plan *plan, void *stream);
 int hbs_grouped_plan_destroy(hbs_grouped_plan *plan);
 

@@ -86,6 +86,27 @@ def test_bf16x3_identity_reproduces_oracle_split(N):
     assert np.array_equal(c2.cpu().numpy().reshape(N, N), want)
 
 
+@pytest.mark.parametrize("M,N,K,lda,ldb,ldc", [(300, 260, 256, 256, 260, 264),
+                                                (1000, 772, 64, 68, 776, 772),
+                                                (257, 513, 96, 97, 513, 515)])
+def test_gemm_bf16x3_rectangular_within_split_bound(M, N, K, lda, ldb, ldc):
+    """Rectangular entry with leading dimensions: ragged M and N with K a
+    multiple of 32 and A 16-byte loadable take the clamped-load tiles (the
+    first two shapes), an odd lda the predicated ones; elements outside M x N
+    stay untouched."""
+    a = dev_fill(M * lda, hb.GEN_LOGUNIFORM, 51); b = dev_fill(K * ldb, hb.GEN_UNIFORM, 52)
+    c = torch.full((M * ldc,), -3.0, dtype=torch.float32, device="cuda")
+    hb.gemm_bf16x3(a, b, c, M, N, K, lda=lda, ldb=ldb, ldc=ldc)
+    torch.cuda.synchronize()
+    A = a.cpu().numpy().reshape(M, lda)[:, :K]
+    B = b.cpu().numpy().reshape(K, ldb)[:, :N]
+    Cf = c.cpu().numpy().reshape(M, ldc)
+    R, S = oracle.gemm_f64(np.ascontiguousarray(A), np.ascontiguousarray(B))
+    bound = (3.1 * 2.0 ** -16 + gamma(3 * K + 2)) * S
+    assert np.all(np.abs(Cf[:, :N] - R) <= bound)
+    assert np.all(Cf[:, N:] == -3.0)
+
+
 def test_bf16x3_pattern_ragged_sizes():
     for N in (3, 100, 257):
         a = dev_empty(N * N); b = dev_empty(N * N); c = dev_empty(N * N)

@@ -140,21 +140,116 @@ def test_grouped_plan_info_counts_tiles():
     assert info["problems"] == 3 and info["workgroups"] >= 1
 
 
+# Shapes for the large-tile path: whole problems in 256 x 256 tiles, ragged at
+# the bottom and right (clamped loads) or not, plus one K that is not a
+# multiple of 32, which must stay on the 256-thread launch.
+BIG = [(300, 260, 256), (512, 512, 320), (257, 1024, 256), (256, 256, 512), (1000, 1028, 768),
+       (1024, 300, 1024), (512, 512, 100), (64, 2048, 256)]
+
+
+def big_tiles(shapes, prec, forced):
+    """256 x 256 tiles the plan puts on the large-tile launch."""
+    n = 0
+    for M, N, K in shapes:
+        if forced:
+            big = K % 32 == 0
+        else:  # the default: BF16x3 with M, N, K >= 256
+            big = prec == hb.PREC_BF16X3 and M >= 256 and N >= 256 and K >= 256 and K % 32 == 0
+        if big:
+            n += ((M + 255) // 256) * ((N + 255) // 256)
+    return n
+
+
 @pytest.mark.parametrize("prec", [hb.PREC_FP32, hb.PREC_BF16X3])
-def test_grouped_graph_replays_match_eager(prec):
-    shapes = [(256, 192, 512), (64, 64, 2048), (96, 320, 64), (1024, 64, 1024)]
-    probs, views, bufs = make_group(shapes, "uniform", seed=3)
-    plan = hb.GroupedPlan(probs, prec)
+def test_grouped_large_tiles_integer_exact(prec):
+    probs, views, _ = make_group(BIG, "pattern")
+    plan = hb.GroupedPlan(probs, prec, tile=hb.TILE_256x256, split_k=1)
     plan.run()
     torch.cuda.synchronize()
-    eager = bufs[2].clone()
+    for (A, B, C), shape in zip(views, BIG):
+        assert torch.equal(C, (A.double() @ B.double()).float()), shape
+
+
+@pytest.mark.parametrize("prec", [hb.PREC_FP32, hb.PREC_BF16X3])
+@pytest.mark.parametrize("mode", ["forced", "forced-split2", "heuristic"])
+def test_grouped_large_tiles_within_bound(prec, mode):
+    """Large-tile cores plus remainders on the 256-thread launch: every
+    problem within its FP64 bound, a second run identical, and the plan's
+    tile count includes the 256 x 256 tiles."""
+    probs, views, _ = make_group(BIG, "uniform", seed=29)
+    if mode == "heuristic":
+        plan = hb.GroupedPlan(probs, prec)
+    else:
+        plan = hb.GroupedPlan(probs, prec, tile=hb.TILE_256x256, split_k=2 if "split" in mode else 1)
+    info = plan.info()
+    nbig = big_tiles(BIG, prec, mode != "heuristic")
+    assert info["tiles"] > nbig  # (512, 512, 100) is never on the large-tile launch
+    if mode != "heuristic" or prec == hb.PREC_BF16X3:
+        assert nbig > 0
+    if mode == "forced-split2":
+        assert info["split_tiles"] > 0
+    plan.run()
+    torch.cuda.synchronize()
+    first = [C.clone() for (_, _, C) in views]
+    plan.run()
+    torch.cuda.synchronize()
+    for (A, B, C), c0, (M, N, K) in zip(views, first, BIG):
+        assert torch.equal(C, c0), (M, N, K)
+        R, S = oracle.gemm_f64(A.cpu().numpy(), B.cpu().numpy())
+        if prec == hb.PREC_FP32:
+            bound = gamma(K + 2) * S
+        else:
+            bound = (3.1 * 2.0 ** -16 + gamma(3 * K + 2)) * S
+        assert np.all(np.abs(C.cpu().numpy() - R) <= bound), (M, N, K)
+
+
+def test_grouped_large_tiles_identity_reproduces_oracle_split():
+    """BF16x3 through the large-tile launch with ragged edges: with B = I the
+    output is the oracle's hi + lo of A bit for bit (rows 256..299 come from a
+    tile that clamps its loads), and with A = I that of B (columns 256..259
+    likewise)."""
+    M, K, N = 300, 512, 260
+
+    def loguniform(n, seed):
+        x = torch.empty(n, dtype=torch.float32, device="cuda")
+        hb.fill_random(x, n, hb.GEN_LOGUNIFORM, seed)
+        return x
+
+    a1 = loguniform(M * K, 81).view(M, K)
+    b2 = loguniform(K * N, 82).view(K, N)
+    eye = torch.eye(K, dtype=torch.float32, device="cuda").contiguous()
+    c1 = torch.full((M, K), float("nan"), device="cuda")
+    c2 = torch.full((K, N), float("nan"), device="cuda")
+    plan = hb.GroupedPlan([(a1, eye, c1, M, K, K), (eye, b2, c2, K, N, K)], hb.PREC_BF16X3,
+                          tile=hb.TILE_256x256, split_k=1)
+    assert plan.info()["tiles"] == 2 * 2 + 2 * 2  # both problems whole, in 256 x 256 tiles
+    plan.run()
+    torch.cuda.synchronize()
+    for x, c in ((a1, c1), (b2, c2)):
+        xh = x.cpu().numpy()
+        hi, lo = oracle.split_bf16(xh)
+        want = oracle.bf16_bits_to_f32(hi) + oracle.bf16_bits_to_f32(lo)
+        assert np.array_equal(c.cpu().numpy(), want)
+
+
+@pytest.mark.parametrize("prec", [hb.PREC_FP32, hb.PREC_BF16X3])
+def test_grouped_graph_replays_match_eager(prec):
+    shapes = [(256, 192, 512), (64, 64, 2048), (96, 320, 64), (1024, 64, 1024), (520, 772, 512)]
+    probs, views, bufs = make_group(shapes, "uniform", seed=3)
+    plan = hb.GroupedPlan(probs, prec)
+    assert plan.info()["tiles"] > sum(((M + 127) // 128) * ((N + 127) // 128)
+                                      for M, N, _ in shapes[:4])  # the last one has a core
+    plan.run()
+    torch.cuda.synchronize()
+    eager = [C.clone() for (_, _, C) in views]
     g = plan.graph()
     for rep in range(3):
         bufs[2].zero_()             # eager work between replays
         torch.cuda.synchronize()
         g.launch()
         torch.cuda.synchronize()
-        assert torch.equal(bufs[2], eager), rep
+        for (_, _, C), e in zip(views, eager):
+            assert torch.equal(C, e), rep
     g.close()
 
 
