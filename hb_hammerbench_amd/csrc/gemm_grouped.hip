@@ -509,7 +509,7 @@ bool load_tuning(const char *path, std::map<TuneKey, Choice> &table) {
         if (!(ss >> prec >> mc >> nc >> kc >> tile >> split)) continue;
         ss >> ns;
         const int pr = prec == "fp32" ? HBS_PREC_FP32 : prec == "bf16x3" ? HBS_PREC_BF16X3 : -1;
-        if (pr < 0 || tile < 0 || tile > 3 || split < 1 || split > 64) continue;
+        if (pr < 0 || tile < 0 || tile > kTileBig || split < 1 || split > 64) continue;
         table[TuneKey(pr, mc, nc, kc)] = Choice{tile, split, ns};
     }
     return true;

@@ -268,6 +268,18 @@ def test_grouped_tuning_file_is_applied(tmp_path):
         assert np.all(np.abs(C.cpu().numpy() - R) <= gamma(2048 + 2) * S)
     with pytest.raises(hb.HbsError):
         hb.GroupedPlan(probs, hb.PREC_FP32, tuning_file=str(tmp_path / "missing.txt"))
+    # a large-tile entry (tile id 4) moves its class to the 512-thread launch
+    shapes = [(512, 512, 512), (64, 64, 64)]
+    probs, views, _ = make_group(shapes, "uniform", seed=12)
+    path.write_text("fp32 512 512 512 4 1 0\n")
+    plan = hb.GroupedPlan(probs, hb.PREC_FP32, tuning_file=str(path))
+    info = plan.info()
+    assert info["tiles"] == 4 + 1 and info["work_items"] == 4 + 1
+    plan.run()
+    torch.cuda.synchronize()
+    for (A, B, C), (M, N, K) in zip(views, shapes):
+        R, S = oracle.gemm_f64(A.cpu().numpy(), B.cpu().numpy())
+        assert np.all(np.abs(C.cpu().numpy() - R) <= gamma(K + 2) * S)
 
 
 @pytest.mark.parametrize("prec", [hb.PREC_FP32, hb.PREC_BF16X3])
