@@ -113,9 +113,10 @@ def cpu_baseline(N, target_s):
 
 
 def measured_traffic(variant, workload_key):
-    """HBM bytes per launch of the dominant kernel, from the calibrated
-    rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in profiles/traffic.json
-    (None when that file has no entry for this exact workload)."""
+    """HBM bytes per step (all launches of one step, as `roofline.achieved` is
+    timed), from the calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+    summarised in profiles/traffic.json (None when that file has no entry for
+    this exact workload)."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
