@@ -696,6 +696,51 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
         }
         boff[nb] = (int)sorted.size();
     };
+    // Large-tile launch: one workgroup per CU, workgroup b on XCD b % 8.  The
+    // tiles go out in waves of one per workgroup, in the order problems by K
+    // (longest first), tiles of a problem in a raster of 4 tile rows, and
+    // within a wave 32 consecutive tiles to the 32 workgroups of one XCD: the
+    // tiles an XCD runs at the same time are a compact block of one problem
+    // and share A and B panels in that XCD's L2.  Tiles of one wave have about
+    // the same K, so every workgroup gets a similar load.
+    auto xcd_waves = [&](const std::vector<Item> &its, const std::vector<double> &cst, int nb,
+                         std::vector<Item> &sorted, std::vector<int> &boff, double &imbalance) {
+        std::vector<int64_t> order(its.size());
+        for (size_t i = 0; i < order.size(); ++i) order[i] = (int64_t)i;
+        auto key = [&](int64_t x) {
+            const Item &t = its[x];
+            const int gm = 4, rows = (probs[t.prob].M + kBig - 1) / kBig;
+            const int g = t.tm / gm, first = g * gm, h = std::min(gm, rows - first);
+            const int64_t pos = ((int64_t)g * ((probs[t.prob].N + kBig - 1) / kBig) + t.tn) * h +
+                                (t.tm - first);
+            return std::make_tuple(-cst[x], t.prob, pos);
+        };
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int64_t x, int64_t y) { return key(x) < key(y); });
+        constexpr int kXcd = 8;
+        const int per = nb % kXcd == 0 ? nb / kXcd : nb;  // workgroups per XCD
+        std::vector<std::vector<int64_t>> lists(nb);
+        std::vector<double> load(nb, 0.0);
+        for (size_t t = 0; t < order.size(); ++t) {
+            const int j = (int)(t % nb);
+            const int b = nb % kXcd == 0 ? (j % per) * kXcd + j / per : j;
+            lists[b].push_back(order[t]);
+            load[b] += cst[order[t]];
+        }
+        double maxl = 0.0, suml = 0.0;
+        for (double l : load) {
+            maxl = std::max(maxl, l);
+            suml += l;
+        }
+        imbalance = suml > 0.0 ? maxl / (suml / nb) : 1.0;
+        sorted.clear();
+        boff.assign(nb + 1, 0);
+        for (int b = 0; b < nb; ++b) {
+            boff[b] = (int)sorted.size();
+            for (int64_t idx : lists[b]) sorted.push_back(its[idx]);
+        }
+        boff[nb] = (int)sorted.size();
+    };
     // resident workgroups per CU on the 256-thread launch: FP32 tiles use
     // <= 168 VGPRs and 38 KB LDS (three fit), BF16x3 tiles <= 256 VGPRs and
     // 74 KB LDS (two fit); the large-tile launch holds one per CU
@@ -706,7 +751,7 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
     std::vector<int> boff, boff_big;
     double imb = 1.0, imb_big = 1.0;
     lpt(items, cost, nblocks, sorted, boff, imb);
-    lpt(items_big, cost_big, nblocks_big, sorted_big, boff_big, imb_big);
+    xcd_waves(items_big, cost_big, nblocks_big, sorted_big, boff_big, imb_big);
 
     // one device allocation: problems | items | offsets | counters | partials
     // | large-tile items | large-tile offsets
