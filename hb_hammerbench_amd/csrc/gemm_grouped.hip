@@ -709,11 +709,12 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
         for (size_t i = 0; i < order.size(); ++i) order[i] = (int64_t)i;
         auto key = [&](int64_t x) {
             const Item &t = its[x];
-            const int gm = 4, rows = (probs[t.prob].M + kBig - 1) / kBig;
+            const int bm = kTileBM[t.cfg], bn = kTileBN[t.cfg];
+            const int gm = 4, rows = (probs[t.prob].M + bm - 1) / bm;
             const int g = t.tm / gm, first = g * gm, h = std::min(gm, rows - first);
-            const int64_t pos = ((int64_t)g * ((probs[t.prob].N + kBig - 1) / kBig) + t.tn) * h +
+            const int64_t pos = ((int64_t)g * ((probs[t.prob].N + bn - 1) / bn) + t.tn) * h +
                                 (t.tm - first);
-            return std::make_tuple(-cst[x], t.prob, pos);
+            return std::make_tuple(-cst[x], t.prob, pos, t.split);
         };
         std::stable_sort(order.begin(), order.end(),
                          [&](int64_t x, int64_t y) { return key(x) < key(y); });
