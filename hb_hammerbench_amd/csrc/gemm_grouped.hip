@@ -166,250 +166,43 @@ __device__ __forceinline__ void finish_tile(floatx16 (&acc)[BM / 64][BN / 64], c
 }
 
 // ---------------------------------------------------------------------------
-// FP32-MFMA tile: BK = 16, v_mfma_f32_32x32x2_f32, LDS double buffer.
+// Tiles of the 256-thread launch (2 x 2 waves): the square kernels' main loops
+// (tile_f32.h, BK = 16; tile_bf16x3.h, BK = 32) over the item's K range
+// [k0, k1), then finish_tile, with clamped loads (the API requires
+// 16-byte-loadable A and B and N, K multiples of 4; split ranges start on a
+// stage).
 // ---------------------------------------------------------------------------
-template <int BM, int BN>
-struct F32T {
-    static constexpr int BK = 16, LDA = BK + 4, LDB = BN + 8;
-    static constexpr int STAGE = BM * LDA + BK * LDB;  // floats
-    static constexpr int AF4 = BM * BK / 4 / kThreads, BF4 = BK * BN / 4 / kThreads;
-    static constexpr int LDS_BYTES = 2 * STAGE * 4;
-};
-
 template <int BM, int BN>
 __device__ void f32_tile(float *smem, const Prob &p, const Item &it, float *partials,
                          int *counters, int *s_flag) {
-    using T = F32T<BM, BN>;
-    constexpr int FM = BM / 64, FN = BN / 64, BK = T::BK;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
-    const int l32 = lane & 31, h = lane >> 5;
-    const int m0 = it.tm * BM, n0 = it.tn * BN;
-
-    floatx16 acc[FM][FN];
-#pragma unroll
-    for (int r = 0; r < FM; ++r)
-#pragma unroll
-        for (int c = 0; c < FN; ++c) acc[r][c] = floatx16{0};
-
-    floatx4 ra[T::AF4], rb[T::BF4];
-    auto gload = [&](int k) {
-#pragma unroll
-        for (int i = 0; i < T::AF4; ++i) {
-            const int f = tid + i * kThreads;
-            const int row = f / (BK / 4), gk = k + (f % (BK / 4)) * 4;
-            ra[i] = (m0 + row < p.M && gk < it.k1)
-                        ? *(const floatx4 *)(p.A + (int64_t)(m0 + row) * p.lda + gk)
-                        : floatx4{0, 0, 0, 0};
-        }
-#pragma unroll
-        for (int i = 0; i < T::BF4; ++i) {
-            const int f = tid + i * kThreads;
-            const int gk = k + f / (BN / 4), gn = n0 + (f % (BN / 4)) * 4;
-            rb[i] = (gk < it.k1 && gn < p.N) ? *(const floatx4 *)(p.B + (int64_t)gk * p.ldb + gn)
-                                              : floatx4{0, 0, 0, 0};
-        }
-    };
-    auto sstore = [&](int buf) {
-        float *as = smem + buf * T::STAGE;
-        float *bs = as + BM * T::LDA;
-#pragma unroll
-        for (int i = 0; i < T::AF4; ++i) {
-            const int f = tid + i * kThreads;
-            *(floatx4 *)(as + (f / (BK / 4)) * T::LDA + (f % (BK / 4)) * 4) = ra[i];
-        }
-#pragma unroll
-        for (int i = 0; i < T::BF4; ++i) {
-            const int f = tid + i * kThreads;
-            *(floatx4 *)(bs + (f / (BN / 4)) * T::LDB + (f % (BN / 4)) * 4) = rb[i];
-        }
-    };
-    auto compute = [&](int buf) {
-        const float *as = smem + buf * T::STAGE + (wm * (BM / 2) + l32) * T::LDA + 4 * h;
-        const float *bs = smem + buf * T::STAGE + BM * T::LDA + 4 * h * T::LDB + wn * (BN / 2) + l32;
-#pragma unroll
-        for (int kc = 0; kc < BK / 8; ++kc) {
-            floatx4 a[FM];
-            float b[FN][4];
-#pragma unroll
-            for (int r = 0; r < FM; ++r) a[r] = *(const floatx4 *)(as + r * 32 * T::LDA + kc * 8);
-#pragma unroll
-            for (int c = 0; c < FN; ++c)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) b[c][q] = bs[(kc * 8 + q) * T::LDB + c * 32];
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-#pragma unroll
-                for (int r = 0; r < FM; ++r)
-#pragma unroll
-                    for (int c = 0; c < FN; ++c)
-                        acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[r][q], b[c][q], acc[r][c], 0, 0, 0);
-        }
-    };
-
-    const int KT = (it.k1 - it.k0 + BK - 1) / BK;
-    gload(it.k0);
-    sstore(0);
-    __syncthreads();
-    for (int kt = 0; kt < KT; ++kt) {
-        const int cur = kt & 1;
-        if (kt + 1 < KT) gload(it.k0 + (kt + 1) * BK);
-        compute(cur);
-        if (kt + 1 < KT) sstore(cur ^ 1);
-        __syncthreads();
-    }
+    using G = hbs::F32Cfg<BM, BN, 16, 2, 2>;
+    floatx16 acc[G::kFM][G::kFN];
+    const int m0 = it.tm * BM, n0 = it.tn * BN, K = it.k1 - it.k0;
+    const float *Ag = p.A + (int64_t)m0 * p.lda + it.k0;
+    const float *Bg = p.B + (int64_t)it.k0 * p.ldb + n0;
+    hbs::f32_block_mma<BM, BN, 16, 2, 2, hbs::kTileClampK>(smem, Ag, Bg, p.M, p.N, K, p.lda,
+                                                            p.ldb, m0, n0, acc);
     finish_tile<BM, BN>(acc, p, it, partials, counters, s_flag);
 }
-
-// ---------------------------------------------------------------------------
-// BF16x3 tile: BK = 32.  Each stage's float32 A (BM x 32) and B (32 x BN) tiles
-// are split in registers and stored to LDS rows that are K-contiguous for both
-// operands (per 8-k group: 8 hi then 8 lo bf16); three v_mfma_f32_32x32x16_bf16
-// products per 16 k.  A item: one row, 8 consecutive k (two 16-byte loads).  B
-// item: one column, 8 consecutive k (8 dword loads; the 64 lanes of a wave read
-// 64 adjacent columns, one 256-byte row segment per load).  The loads of a
-// stage are issued a full step before its LDS store, which runs between the
-// two 16-k substeps of the previous stage.
-// ---------------------------------------------------------------------------
-template <int BM, int BN>
-struct B3T {
-    static constexpr int BK = 32, ROW = 2 * BK + 8;    // bf16 per LDS row (+16 B pad)
-    static constexpr int STAGE = (BM + BN) * ROW;       // bf16 elements
-    static constexpr int IA = BM * (BK / 8) / kThreads;  // A items per thread
-    static constexpr int IB = BN * (BK / 8) / kThreads;  // B items per thread
-    static constexpr int LDS_BYTES = 2 * STAGE * 2;
-    static_assert(IA >= 1 && IB >= 1 && BN % 64 == 0, "items; one k group per wave for B");
-};
 
 template <int BM, int BN>
 __device__ void b3_tile(uint16_t *smem, const Prob &p, const Item &it, float *partials,
                         int *counters, int *s_flag) {
-    using T = B3T<BM, BN>;
-    constexpr int FM = BM / 64, FN = BN / 64, BK = T::BK, ROW = T::ROW;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
-    const int l32 = lane & 31, h = lane >> 5;
-    const int m0 = it.tm * BM, n0 = it.tn * BN;
-
-    floatx16 acc[FM][FN];
-#pragma unroll
-    for (int r = 0; r < FM; ++r)
-#pragma unroll
-        for (int c = 0; c < FN; ++c) acc[r][c] = floatx16{0};
-
-    // K, lda and the split bounds are multiples of 4 and k0 of 32, so each
-    // 16-byte A load lies wholly inside or wholly outside [k0, k1)
-    float ra[T::IA][8], rb[T::IB][8];
-    auto gload = [&](int k) {
-#pragma unroll
-        for (int i = 0; i < T::IA; ++i) {
-            const int f = tid + i * kThreads;
-            const int row = f / (BK / 8), gk = k + (f % (BK / 8)) * 8;
-            const float *x = p.A + (int64_t)(m0 + row) * p.lda + gk;
-            const bool rok = m0 + row < p.M;
-            const floatx4 zero = {0, 0, 0, 0};
-            const floatx4 v0 = (rok && gk < it.k1) ? *(const floatx4 *)x : zero;
-            const floatx4 v1 = (rok && gk + 4 < it.k1) ? *(const floatx4 *)(x + 4) : zero;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                ra[i][j] = v0[j];
-                ra[i][j + 4] = v1[j];
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < T::IB; ++i) {
-            const int f = tid + i * kThreads;
-            const int col = f % BN;
-            const int gk = k + __builtin_amdgcn_readfirstlane(f / BN) * 8;
-            const bool cok = n0 + col < p.N;
-#pragma unroll
-            for (int j = 0; j < 8; ++j)
-                rb[i][j] = (cok && gk + j < it.k1) ? p.B[(int64_t)(gk + j) * p.ldb + n0 + col] : 0.0f;
-        }
-    };
-    auto sstore = [&](int buf) {
-        uint16_t *as = smem + buf * T::STAGE;
-        uint16_t *bs = as + BM * ROW;
-#pragma unroll
-        for (int i = 0; i < T::IA; ++i) {
-            const int f = tid + i * kThreads;
-            uint32_t hi[4], lo[4];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) hbs::split_pk(ra[i][2 * j], ra[i][2 * j + 1], hi[j], lo[j]);
-            uint16_t *d = as + (f / (BK / 8)) * ROW + (f % (BK / 8)) * 16;
-            *(uint4 *)d = uint4{hi[0], hi[1], hi[2], hi[3]};
-            *(uint4 *)(d + 8) = uint4{lo[0], lo[1], lo[2], lo[3]};
-        }
-#pragma unroll
-        for (int i = 0; i < T::IB; ++i) {
-            const int f = tid + i * kThreads;
-            uint32_t hi[4], lo[4];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) hbs::split_pk(rb[i][2 * j], rb[i][2 * j + 1], hi[j], lo[j]);
-            uint16_t *d = bs + (f % BN) * ROW + (f / BN) * 16;
-            *(uint4 *)d = uint4{hi[0], hi[1], hi[2], hi[3]};
-            *(uint4 *)(d + 8) = uint4{lo[0], lo[1], lo[2], lo[3]};
-        }
-    };
-    // one 16-deep substep of the stage in buffer `buf`
-    auto compute = [&](int buf, int st) {
-        const uint16_t *as = smem + buf * T::STAGE + (wm * (BM / 2) + l32) * ROW;
-        const uint16_t *bs = smem + buf * T::STAGE + BM * ROW + (wn * (BN / 2) + l32) * ROW;
-        const int off = (2 * st + h) * 16;  // this lane half's 8-k group
-        bf16x8 ahi[FM], alo[FM], bhi[FN], blo[FN];
-#pragma unroll
-        for (int r = 0; r < FM; ++r) {
-            ahi[r] = *(const bf16x8 *)(as + r * 32 * ROW + off);
-            alo[r] = *(const bf16x8 *)(as + r * 32 * ROW + off + 8);
-        }
-#pragma unroll
-        for (int c = 0; c < FN; ++c) {
-            bhi[c] = *(const bf16x8 *)(bs + c * 32 * ROW + off);
-            blo[c] = *(const bf16x8 *)(bs + c * 32 * ROW + off + 8);
-        }
-#pragma unroll
-        for (int r = 0; r < FM; ++r)
-#pragma unroll
-            for (int c = 0; c < FN; ++c) {
-                acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[r], bhi[c], acc[r][c], 0, 0, 0);
-                acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[r], blo[c], acc[r][c], 0, 0, 0);
-                acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[r], bhi[c], acc[r][c], 0, 0, 0);
-            }
-    };
-
-    const int KT = (it.k1 - it.k0 + BK - 1) / BK;
-    gload(it.k0);
-    sstore(0);
-    if (KT > 1) gload(it.k0 + BK);
-    __syncthreads();
-    int kt = 0;
-    for (; kt + 2 < KT; ++kt) {
-        const int cur = kt & 1;
-        compute(cur, 0);
-        sstore(cur ^ 1);
-        gload(it.k0 + (kt + 2) * BK);
-        compute(cur, 1);
-        __syncthreads();
-    }
-    if (kt + 1 < KT) {
-        const int cur = kt & 1;
-        compute(cur, 0);
-        sstore(cur ^ 1);
-        compute(cur, 1);
-        __syncthreads();
-        ++kt;
-    }
-    compute(kt & 1, 0);
-    compute(kt & 1, 1);
+    using G = hbs::B3Cfg<BM, BN, 32, 2, 2>;
+    floatx16 acc[G::kFM][G::kFN];
+    const int m0 = it.tm * BM, n0 = it.tn * BN, K = it.k1 - it.k0;
+    const float *Ag = p.A + (int64_t)m0 * p.lda + it.k0;
+    const float *Bg = p.B + (int64_t)it.k0 * p.ldb + n0;
+    hbs::bf16x3_block_mma<G, hbs::kTileClampK>(smem, Ag, Bg, p.M, p.N, K, p.lda, p.ldb, m0, n0,
+                                               acc);
     finish_tile<BM, BN>(acc, p, it, partials, counters, s_flag);
 }
 
-constexpr int kLdsF32 = F32T<128, 128>::LDS_BYTES;
-constexpr int kLdsB3 = B3T<128, 128>::LDS_BYTES;
-static_assert(F32T<64, 128>::LDS_BYTES <= kLdsF32 && F32T<128, 64>::LDS_BYTES <= kLdsF32, "lds");
+constexpr int kLdsF32 = hbs::F32Cfg<128, 128, 16, 2, 2>::kLdsFloats * 4;
+constexpr int kLdsB3 = 2 * hbs::B3Cfg<128, 128, 32, 2, 2>::kStage * 2;
 
 template <int PREC>
-__global__ __launch_bounds__(kThreads, PREC == HBS_PREC_FP32 ? 3 : 2) void grouped_kernel(const Prob *__restrict__ probs,
+__global__ __launch_bounds__(kThreads, 2) void grouped_kernel(const Prob *__restrict__ probs,
                                                               const Item *__restrict__ items,
                                                               const int *__restrict__ boff,
                                                               float *partials, int *counters) {
@@ -625,9 +418,9 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
         const int ns = (q.K + chunk - 1) / chunk;
         // Work-item cost in ns of one workgroup slot: the autotuner's measured
         // time when the tuning file has this class, else a throughput model
-        // (FP32: 157.3 TFLOP/s shared by 3 x CUs slots; BF16x3: 2516.6 / 3
-        // TFLOP/s-equivalent by 2 x CUs slots) with a tile-shape efficiency.
-        const double slots_all = (precision == HBS_PREC_FP32 ? 3.0 : 2.0) * cus;
+        // (FP32: 157.3 TFLOP/s, BF16x3: 2516.6 / 3 TFLOP/s-equivalent, shared
+        // by 2 x CUs slots) with a tile-shape efficiency.
+        const double slots_all = 2.0 * cus;
         const double peak = precision == HBS_PREC_FP32 ? 157.3e12 : 2516.6e12 / 3.0;
         const double eff = (c.tile == 0) ? 0.7 : (c.tile == 1 ? 0.35 : 0.5);
         for (int tm = 0; tm < tm_n; ++tm) {
@@ -742,10 +535,10 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
         }
         boff[nb] = (int)sorted.size();
     };
-    // resident workgroups per CU on the 256-thread launch: FP32 tiles use
-    // <= 168 VGPRs and 38 KB LDS (three fit), BF16x3 tiles <= 256 VGPRs and
-    // 74 KB LDS (two fit); the large-tile launch holds one per CU
-    const int per_cu = precision == HBS_PREC_FP32 ? 3 : 2;
+    // resident workgroups per CU on the 256-thread launch: two (up to 256
+    // VGPRs each; 38 KB LDS for FP32, 74 KB for BF16x3); the large-tile launch
+    // holds one per CU
+    const int per_cu = 2;
     const int nblocks = (int)std::min<int64_t>((int64_t)per_cu * cus, (int64_t)items.size());
     const int nblocks_big = (int)std::min<int64_t>(cus, (int64_t)items_big.size());
     std::vector<Item> sorted, sorted_big;

@@ -24,8 +24,10 @@ namespace hbs {
 //   kTileClamp     ragged M or N, with K a multiple of the stage depth and
 //                  16-byte-loadable operands: rows and columns beyond M or N
 //                  are read from the last valid one (clamped addresses at full
-//                  load width) and only the stores are predicated.
-enum TileMode { kTileInterior = 0, kTileEdge = 1, kTileClamp = 2 };
+//                  load width) and only the stores are predicated;
+//   kTileClampK    kTileClamp with K any multiple of 4: whole 4-k groups
+//                  beyond K are also zero.
+enum TileMode { kTileInterior = 0, kTileEdge = 1, kTileClamp = 2, kTileClampK = 3 };
 
 // Checks that the current device is a gfx950 GPU (cached per device).
 int check_device();

@@ -44,15 +44,17 @@ int launch_gemm(const float *A, const float *B, float *C, int M, int N, int K, i
                (M + G::BM - 1) / G::BM, (N + G::BN - 1) / G::BN, group_m};
     const int64_t gblocks = (int64_t)g.tiles_m * g.tiles_n * batch;
     if (gblocks > 0x7fffffffLL) return HBS_ERR_ARG;
-    // interior tiles when the shape divides evenly, clamped loads for ragged M
-    // or N (K a multiple of the stage depth, A 16-byte loadable), else
-    // predicated loads
+    // interior tiles when the shape divides evenly, clamped loads for ragged
+    // M or N (with K predicates when K is not a multiple of the stage; K a
+    // multiple of 4, A 16-byte loadable), else predicated loads
     const bool vec_a = hbs::aligned16(A) && lda % 4 == 0 && sA % 4 == 0;
     const unsigned nb = (unsigned)gblocks;
     if (vec_a && K % G::BK == 0 && M % G::BM == 0 && N % G::BN == 0)
         gemm_bf16x3_kernel<G, hbs::kTileInterior><<<nb, G::kThreads, 0, stream>>>(g);
     else if (vec_a && K % G::BK == 0)
         gemm_bf16x3_kernel<G, hbs::kTileClamp><<<nb, G::kThreads, 0, stream>>>(g);
+    else if (vec_a && K % 4 == 0)
+        gemm_bf16x3_kernel<G, hbs::kTileClampK><<<nb, G::kThreads, 0, stream>>>(g);
     else
         gemm_bf16x3_kernel<G, hbs::kTileEdge><<<nb, G::kThreads, 0, stream>>>(g);
     return hbs::launch_status();

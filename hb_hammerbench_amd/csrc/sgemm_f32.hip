@@ -37,8 +37,8 @@ __global__ __launch_bounds__(WM *WN * 64) void sgemm_f32_kernel(GemmArgs p) {
 }
 
 // Interior tiles when the shape divides evenly; ragged M or N with aligned
-// operands, K a multiple of BK and N of 4 take the clamped loads; anything
-// else the predicated ones.
+// operands take the clamped loads (with K predicates when K is not a multiple
+// of BK, K and N multiples of 4); anything else the predicated ones.
 template <int BM, int BN, int BK, int WM, int WN>
 int launch_cfg(GemmArgs p, bool vec_ok, hipStream_t stream) {
     using G = hbs::F32Cfg<BM, BN, BK, WM, WN>;
@@ -51,6 +51,8 @@ int launch_cfg(GemmArgs p, bool vec_ok, hipStream_t stream) {
         sgemm_f32_kernel<BM, BN, BK, WM, WN, hbs::kTileInterior><<<nb, G::kThreads, 0, stream>>>(p);
     else if (vec_ok && p.K % BK == 0 && p.N % 4 == 0)
         sgemm_f32_kernel<BM, BN, BK, WM, WN, hbs::kTileClamp><<<nb, G::kThreads, 0, stream>>>(p);
+    else if (vec_ok && p.K % 4 == 0 && p.N % 4 == 0)
+        sgemm_f32_kernel<BM, BN, BK, WM, WN, hbs::kTileClampK><<<nb, G::kThreads, 0, stream>>>(p);
     else
         sgemm_f32_kernel<BM, BN, BK, WM, WN, hbs::kTileEdge><<<nb, G::kThreads, 0, stream>>>(p);
     return hbs::launch_status();

@@ -47,17 +47,20 @@ __device__ __forceinline__ void split_pk(float x0, float x1, uint32_t &hi, uint3
 //   A item: one row, 8 consecutive k (two 16-byte loads).
 //   B item: one column, 8 consecutive k (8 dword loads; the 64 lanes of a wave
 //           read 64 adjacent columns, so each load is one 256-byte row segment).
-// MODE (TileMode): kTileInterior, kTileEdge (ragged M, N or K, or A not
-// 16-byte loadable; out-of-range elements are zero) or kTileClamp (ragged M or
-// N; K a multiple of 32 and A 16-byte loadable).
-// Ag = A + m0 * lda (first row of the tile), Bg = B + n0 (first column),
-// Cg = C (the matrix); M, N, K bound the predicates.  smem holds
-// 2 * G::kStage bf16.  Every thread of the workgroup must call it.
+// MODE (TileMode): kTileInterior, kTileEdge (any shape, A of any alignment;
+// out-of-range elements are zero), kTileClamp (ragged M or N; K a multiple of
+// 32 and A 16-byte loadable) or kTileClampK (the same with K a multiple of 4).
+// Main loop: acc (this lane's kFM x kFN 32x32 blocks of the wave's sub-tile)
+// = A[m0.., 0..K) . B[0..K, n0..), with Ag = A + m0 * lda (first row of the
+// tile) and Bg = B + n0 (first column); a K range [k0, k1) of a larger
+// product is Ag + k0, Bg + k0 * ldb and K = k1 - k0.  M, N, K bound the
+// predicates.  smem holds 2 * G::kStage bf16.  Every thread of the workgroup
+// must call it.
 template <class G, int MODE>
-__device__ __forceinline__ void bf16x3_block_tile(uint16_t *smem, const float *__restrict__ Ag,
-                                                  const float *__restrict__ Bg,
-                                                  float *__restrict__ Cg, int M, int N, int K,
-                                                  int lda, int ldb, int ldc, int m0, int n0) {
+__device__ __forceinline__ void bf16x3_block_mma(uint16_t *smem, const float *__restrict__ Ag,
+                                                 const float *__restrict__ Bg, int M, int N, int K,
+                                                 int lda, int ldb, int m0, int n0,
+                                                 floatx16 (&acc)[G::kFM][G::kFN]) {
     constexpr int kBM = G::BM, kBN = G::BN, kBK = G::BK, kWM = G::WM, kThreads = G::kThreads;
     constexpr int kWTM = G::kWTM, kWTN = G::kWTN, kFM = G::kFM, kFN = G::kFN;
     constexpr int kLdsRow = G::kLdsRow, kStage = G::kStage;
@@ -72,7 +75,6 @@ __device__ __forceinline__ void bf16x3_block_tile(uint16_t *smem, const float *_
     const int wm = wave % kWM, wn = wave / kWM;
     const int l32 = lane & 31, h = lane >> 5;
 
-    floatx16 acc[kFM][kFN];
 #pragma unroll
     for (int r = 0; r < kFM; ++r)
 #pragma unroll
@@ -85,10 +87,14 @@ __device__ __forceinline__ void bf16x3_block_tile(uint16_t *smem, const float *_
         for (int i = 0; i < kItemsA; ++i) {
             const int f = tid + i * kThreads;
             const int row = f / kGroups, gk = k0 + (f % kGroups) * 8;
-            const int srow = MODE == kTileClamp ? min(row, M - m0 - 1) : row;
+            const int srow = MODE >= kTileClamp ? min(row, M - m0 - 1) : row;
             const float *x = Ag + (int64_t)srow * lda + gk;
             if (MODE != kTileEdge) {
-                const floatx4 v0 = *(const floatx4 *)x, v1 = *(const floatx4 *)(x + 4);
+                const floatx4 zero = {0, 0, 0, 0};
+                const bool k0ok = MODE != kTileClampK || gk < K;
+                const bool k1ok = MODE != kTileClampK || gk + 4 < K;
+                const floatx4 v0 = k0ok ? *(const floatx4 *)x : zero;
+                const floatx4 v1 = k1ok ? *(const floatx4 *)(x + 4) : zero;
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     ra[i][j] = v0[j];
@@ -104,7 +110,7 @@ __device__ __forceinline__ void bf16x3_block_tile(uint16_t *smem, const float *_
         for (int i = 0; i < kItemsB; ++i) {
             const int f = tid + i * kThreads;
             const unsigned col =
-                MODE == kTileClamp ? (unsigned)min(f % kBN, N - n0 - 1) : (unsigned)(f % kBN);
+                MODE >= kTileClamp ? (unsigned)min(f % kBN, N - n0 - 1) : (unsigned)(f % kBN);
             // the k group is the same for the whole wave (kBN is a multiple of 64)
             // and each k row pointer is uniform: scalar base + 32-bit lane offset
             const int gk = k0 + __builtin_amdgcn_readfirstlane(f / kBN) * 8;
@@ -112,7 +118,9 @@ __device__ __forceinline__ void bf16x3_block_tile(uint16_t *smem, const float *_
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 const float *rowp = Bg + (int64_t)(gk + j) * ldb;
-                rb[i][j] = (MODE != kTileEdge || (cok && gk + j < K)) ? rowp[col] : 0.0f;
+                const bool ok = MODE == kTileInterior || MODE == kTileClamp ||
+                                (MODE == kTileClampK ? gk + j < K : (cok && gk + j < K));
+                rb[i][j] = ok ? rowp[col] : 0.0f;
             }
         }
     };
@@ -197,7 +205,18 @@ __device__ __forceinline__ void bf16x3_block_tile(uint16_t *smem, const float *_
     }
     compute(kt & 1, 0);
     compute(kt & 1, 1);
+}
 
+// Epilogue: stores acc to C (Cg = the matrix) at the tile (m0, n0); outside
+// kTileInterior the stores are predicated on M x N.
+template <class G, int MODE>
+__device__ __forceinline__ void bf16x3_block_store(float *__restrict__ Cg, int M, int N, int ldc,
+                                                   int m0, int n0,
+                                                   const floatx16 (&acc)[G::kFM][G::kFN]) {
+    constexpr int kWM = G::WM, kWTM = G::kWTM, kWTN = G::kWTN, kFM = G::kFM, kFN = G::kFN;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave % kWM, wn = wave / kWM;
+    const int l32 = lane & 31, h = lane >> 5;
 #pragma unroll
     for (int r = 0; r < kFM; ++r) {
 #pragma unroll
@@ -211,6 +230,17 @@ __device__ __forceinline__ void bf16x3_block_tile(uint16_t *smem, const float *_
             }
         }
     }
+}
+
+// One whole output tile: main loop and store.
+template <class G, int MODE>
+__device__ __forceinline__ void bf16x3_block_tile(uint16_t *smem, const float *__restrict__ Ag,
+                                                  const float *__restrict__ Bg,
+                                                  float *__restrict__ Cg, int M, int N, int K,
+                                                  int lda, int ldb, int ldc, int m0, int n0) {
+    floatx16 acc[G::kFM][G::kFN];
+    bf16x3_block_mma<G, MODE>(smem, Ag, Bg, M, N, K, lda, ldb, m0, n0, acc);
+    bf16x3_block_store<G, MODE>(Cg, M, N, ldc, m0, n0, acc);
 }
 
 }  // namespace hbs

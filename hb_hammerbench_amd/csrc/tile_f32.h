@@ -36,17 +36,19 @@ struct F32Cfg {
     static_assert(BK % 16 == 0, "BK must be a multiple of 16 (two or more 8-k chunks)");
 };
 
-// Ag = A + m0 * lda (first row of the tile), Bg = B + n0 (first column),
-// Cg = C (the matrix); M, N, K bound the predicates.  smem holds
-// F32Cfg::kLdsFloats floats.  Every thread of the workgroup must call it.
-// MODE (TileMode): kTileInterior, kTileEdge (any shape or alignment) or
-// kTileClamp (ragged M or N; K a multiple of BK, N a multiple of 4, 16-byte
-// loadable operands).
+// Main loop: acc (this lane's kFM x kFN 32x32 blocks of the wave's sub-tile)
+// = A[m0.., 0..K) . B[0..K, n0..), with Ag = A + m0 * lda (first row of the
+// tile) and Bg = B + n0 (first column); a K range [k0, k1) of a larger
+// product is Ag + k0, Bg + k0 * ldb and K = k1 - k0.  M, N, K bound the
+// predicates.  smem holds F32Cfg::kLdsFloats floats.  Every thread of the
+// workgroup must call it.  MODE (TileMode): kTileInterior, kTileEdge (any
+// shape or alignment), kTileClamp (ragged M or N; K a multiple of BK, N of 4,
+// 16-byte loadable operands) or kTileClampK (the same with K a multiple of 4).
 template <int BM, int BN, int BK, int WM, int WN, int MODE>
-__device__ __forceinline__ void f32_block_tile(float *smem, const float *__restrict__ Ag,
-                                               const float *__restrict__ Bg,
-                                               float *__restrict__ Cg, int M, int N, int K,
-                                               int lda, int ldb, int ldc, int m0, int n0) {
+__device__ __forceinline__ void f32_block_mma(
+    float *smem, const float *__restrict__ Ag, const float *__restrict__ Bg, int M, int N, int K,
+    int lda, int ldb, int m0, int n0,
+    floatx16 (&acc)[F32Cfg<BM, BN, BK, WM, WN>::kFM][F32Cfg<BM, BN, BK, WM, WN>::kFN]) {
     using G = F32Cfg<BM, BN, BK, WM, WN>;
     float *const As = smem;
     float *const Bs = smem + 2 * G::kStageA;
@@ -56,7 +58,6 @@ __device__ __forceinline__ void f32_block_tile(float *smem, const float *__restr
     const int wm = wave % WM, wn = wave / WM;
     const int l32 = lane & 31, h = lane >> 5;
 
-    floatx16 acc[G::kFM][G::kFN];
 #pragma unroll
     for (int r = 0; r < G::kFM; ++r)
 #pragma unroll
@@ -70,8 +71,11 @@ __device__ __forceinline__ void f32_block_tile(float *smem, const float *__restr
             const int f = tid + i * G::kThreads;
             const int row = f / (BK / 4), c4 = f % (BK / 4);
             if constexpr (MODE != kTileEdge) {
-                const int srow = MODE == kTileClamp ? min(row, M - m0 - 1) : row;
-                ra[i] = *(const floatx4 *)(Ag + (int64_t)srow * lda + k0 + c4 * 4);
+                const int srow = MODE >= kTileClamp ? min(row, M - m0 - 1) : row;
+                const floatx4 zero = {0, 0, 0, 0};
+                ra[i] = (MODE != kTileClampK || k0 + c4 * 4 < K)
+                            ? *(const floatx4 *)(Ag + (int64_t)srow * lda + k0 + c4 * 4)
+                            : zero;
             } else {
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
@@ -85,8 +89,11 @@ __device__ __forceinline__ void f32_block_tile(float *smem, const float *__restr
             const int f = tid + i * G::kThreads;
             const int kr = f / (BN / 4), c4 = f % (BN / 4);
             if constexpr (MODE != kTileEdge) {
-                const int sc = MODE == kTileClamp ? min(c4 * 4, N - n0 - 4) : c4 * 4;
-                rb[i] = *(const floatx4 *)(Bg + (int64_t)(k0 + kr) * ldb + sc);
+                const int sc = MODE >= kTileClamp ? min(c4 * 4, N - n0 - 4) : c4 * 4;
+                const floatx4 zero = {0, 0, 0, 0};
+                rb[i] = (MODE != kTileClampK || k0 + kr < K)
+                            ? *(const floatx4 *)(Bg + (int64_t)(k0 + kr) * ldb + sc)
+                            : zero;
             } else {
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
@@ -198,8 +205,20 @@ __device__ __forceinline__ void f32_block_tile(float *smem, const float *__restr
     }
     compute(kt & 1, [&]() {});
 
-    // Epilogue: 32x32 accumulator layout -- register q of lane l holds
-    // row 8*(q/4) + 4*(l/32) + q%4, col l%32 of its block.
+}
+
+// Epilogue: stores acc to C (Cg = the matrix) at the tile (m0, n0).  32x32
+// accumulator layout: register q of lane l holds row 8*(q/4) + 4*(l/32) + q%4,
+// col l%32 of its block.  Outside kTileInterior the stores are predicated on
+// M x N.
+template <int BM, int BN, int BK, int WM, int WN, int MODE>
+__device__ __forceinline__ void f32_block_store(
+    float *__restrict__ Cg, int M, int N, int ldc, int m0, int n0,
+    const floatx16 (&acc)[F32Cfg<BM, BN, BK, WM, WN>::kFM][F32Cfg<BM, BN, BK, WM, WN>::kFN]) {
+    using G = F32Cfg<BM, BN, BK, WM, WN>;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave % WM, wn = wave / WM;
+    const int l32 = lane & 31, h = lane >> 5;
 #pragma unroll
     for (int r = 0; r < G::kFM; ++r) {
 #pragma unroll
@@ -213,6 +232,18 @@ __device__ __forceinline__ void f32_block_tile(float *smem, const float *__restr
             }
         }
     }
+}
+
+// One whole output tile: main loop and store.
+template <int BM, int BN, int BK, int WM, int WN, int MODE>
+__device__ __forceinline__ void f32_block_tile(float *smem, const float *__restrict__ Ag,
+                                               const float *__restrict__ Bg,
+                                               float *__restrict__ Cg, int M, int N, int K,
+                                               int lda, int ldb, int ldc, int m0, int n0) {
+    using G = F32Cfg<BM, BN, BK, WM, WN>;
+    floatx16 acc[G::kFM][G::kFN];
+    f32_block_mma<BM, BN, BK, WM, WN, MODE>(smem, Ag, Bg, M, N, K, lda, ldb, m0, n0, acc);
+    f32_block_store<BM, BN, BK, WM, WN, MODE>(Cg, M, N, ldc, m0, n0, acc);
 }
 
 }  // namespace hbs
