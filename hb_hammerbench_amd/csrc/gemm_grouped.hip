@@ -8,18 +8,24 @@
 // Design:
 //  * a plan (host side, built once) chooses a tile shape and split-K factor per
 //    problem -- from the autotuner's tuning file when one is given, else from a
-//    size heuristic -- expands every problem into work items (one output tile
-//    times one K range) and assigns them to a fixed number of persistent
-//    workgroups by longest-processing-time-first bin packing;
-//  * the kernel is one launch of (3 or 2 x CUs) workgroups, each walking its own
-//    item list, so the launch is capturable in a hipGraph and needs no host
-//    work or memset between replays;
+//    size heuristic -- and expands every problem into work items (one output
+//    tile times one K range);
+//  * two persistent launches, captured together in a hipGraph, each with a
+//    fixed number of workgroups walking their own item lists (no host work or
+//    memset between replays):
+//      - the large-tile launch, one 512-thread workgroup per CU, runs the
+//        square kernels' 256 x 256 tile (tile_f32.h, tile_bf16x3.h) over whole
+//        problems, edge tiles with clamped loads; its tiles are dealt in
+//        XCD-local raster blocks;
+//      - the 256-thread launch, two workgroups per CU, runs 128 x 128 to
+//        64 x 64 tiles with the same main loops over each item's K range,
+//        balanced by longest-processing-time-first bin packing;
 //  * split-K tiles write FP32 partials; the last of a tile's splits to finish
 //    (an agent-scope counter) adds them in split order -- deterministic -- and
 //    resets the counter to zero for the next run;
 //  * BF16x3 tiles read the float32 operands directly and split them into bf16
-//    hi/lo planes in registers on the way to LDS, so both precisions are one
-//    launch with no pre-pass and no operand copies.
+//    hi/lo planes in registers on the way to LDS: no pre-pass and no operand
+//    copies.
 #include <stdlib.h>
 #include <string.h>
 
