@@ -320,14 +320,16 @@ bool tile_supported(int precision, int tile) {
 }
 
 // The large-tile path needs K a multiple of its 32-deep stage.  By default
-// BF16x3 problems take it when M, N >= 256 and K >= 256: the 256 x 256 BF16x3
-// tile runs at about twice the 128 x 128 one's rate, which outweighs the
-// padding of ragged edges.  FP32 stays on the 256-thread launch by default
-// (there the gain is smaller than the padding); tuning files and forced plans
-// can select the path for either precision.
+// every BF16x3 problem that allows it takes it, whatever its M, N and K: the
+// 256 x 256 BF16x3 tile runs at two to five times the per-CU rate of the
+// 128 x 128 tiles of two workgroups per CU, which outweighs the padding of
+// thin problems (M or N of 64 to 192) and the fixed cost of short K, and one
+// launch leaves no second tail.  FP32 stays on the 256-thread launch by
+// default (there the two launches measure the same); tuning files and forced
+// plans can select the path for either precision.
 bool big_possible(int M, int N, int K) { (void)M; (void)N; return K % 32 == 0; }
 bool big_default(int precision, int M, int N, int K) {
-    return precision == HBS_PREC_BF16X3 && M >= kBig && N >= kBig && K >= 256 && K % 32 == 0;
+    return precision == HBS_PREC_BF16X3 && big_possible(M, N, K);
 }
 
 Choice heuristic(int precision, int M, int N, int K) {
