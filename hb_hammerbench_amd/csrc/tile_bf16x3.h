@@ -25,6 +25,23 @@ struct B3Cfg {
     static constexpr int kStage = (BM + BN) * kLdsRow;    // bf16 elements per stage
     static_assert(kWTM % 32 == 0 && kWTN % 32 == 0, "wave tile");
     static_assert(2 * kStage * 2 <= 160 * 1024, "LDS");
+    // Sub-tile (wm, wn) of wave `wave`.  With 8 waves as 4 x 2, the waves of
+    // each pair (2s, 2s + 1) take different column halves and row blocks two
+    // apart, so that in an edge tile with only its top rows or left columns
+    // inside M x N both waves of a pair are rarely idle together.  With the
+    // MFMAs of such idle waves skipped, this pairing measured 1% faster on a
+    // group of ragged problems, where pairing w with w + 4 measured no gain:
+    // consistent with waves 2s, 2s + 1 sharing a SIMD.  Other shapes use
+    // wm = wave % WM, wn = wave / WM.
+    __device__ static void wave_pos(int wave, int &wm, int &wn) {
+        if (WM == 4 && WN == 2) {
+            wn = wave & 1;
+            wm = ((wave >> 1) + 2 * wn) & 3;
+        } else {
+            wn = wave / WM;
+            wm = wave % WM;
+        }
+    }
 };
 // Production configuration: 256 x 256 tile, BK = 32, 8 waves of 64 x 128 (the
 // fastest of the shapes measured at N = 8192, NITER = 10).
@@ -56,12 +73,13 @@ __device__ __forceinline__ void split_pk(float x0, float x1, uint32_t &hi, uint3
 // product is Ag + k0, Bg + k0 * ldb and K = k1 - k0.  M, N, K bound the
 // predicates.  smem holds 2 * G::kStage bf16.  Every thread of the workgroup
 // must call it.
-template <class G, int MODE>
-__device__ __forceinline__ void bf16x3_block_mma(uint16_t *smem, const float *__restrict__ Ag,
-                                                 const float *__restrict__ Bg, int M, int N, int K,
-                                                 int lda, int ldb, int m0, int n0,
-                                                 floatx16 (&acc)[G::kFM][G::kFN]) {
-    constexpr int kBM = G::BM, kBN = G::BN, kBK = G::BK, kWM = G::WM, kThreads = G::kThreads;
+template <class G, int MODE, bool COMPUTE>
+__device__ __forceinline__ void bf16x3_block_mma_impl(uint16_t *smem, const float *__restrict__ Ag,
+                                                      const float *__restrict__ Bg, int M, int N,
+                                                      int K, int lda, int ldb, int m0, int n0,
+                                                      int wm, int wn,
+                                                      floatx16 (&acc)[G::kFM][G::kFN]) {
+    constexpr int kBM = G::BM, kBN = G::BN, kBK = G::BK, kThreads = G::kThreads;
     constexpr int kWTM = G::kWTM, kWTN = G::kWTN, kFM = G::kFM, kFN = G::kFN;
     constexpr int kLdsRow = G::kLdsRow, kStage = G::kStage;
     constexpr int kGroups = kBK / 8;                       // 8-k groups per tile row
@@ -71,8 +89,7 @@ __device__ __forceinline__ void bf16x3_block_mma(uint16_t *smem, const float *__
     static_assert(kBN % 64 == 0 && kThreads % kBN == 0, "B items: one k group per wave");
 
     const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
-    const int wm = wave % kWM, wn = wave / kWM;
+    const int lane = tid & 63;
     const int l32 = lane & 31, h = lane >> 5;
 
 #pragma unroll
@@ -152,6 +169,7 @@ __device__ __forceinline__ void bf16x3_block_mma(uint16_t *smem, const float *__
     };
     // one 16-deep substep of the stage in buffer `buf`
     auto compute = [&](int buf, int st) {
+        if (!COMPUTE) return;
         const uint16_t *as = smem + buf * kStage + (wm * kWTM + l32) * kLdsRow;
         const uint16_t *bsm = smem + buf * kStage + kBM * kLdsRow + (wn * kWTN + l32) * kLdsRow;
         const int off = (2 * st + h) * 16;  // this lane half's 8-k group
@@ -207,15 +225,34 @@ __device__ __forceinline__ void bf16x3_block_mma(uint16_t *smem, const float *__
     compute(kt & 1, 1);
 }
 
+// With 8 waves as 4 x 2, a wave whose sub-tile lies wholly outside M x N
+// (edge tiles outside kTileInterior) runs the same loop without fragment reads
+// and MFMAs: it still loads and splits its share of each stage and meets every
+// barrier.
+template <class G, int MODE>
+__device__ __forceinline__ void bf16x3_block_mma(uint16_t *smem, const float *__restrict__ Ag,
+                                                 const float *__restrict__ Bg, int M, int N, int K,
+                                                 int lda, int ldb, int m0, int n0,
+                                                 floatx16 (&acc)[G::kFM][G::kFN]) {
+    int wm, wn;
+    G::wave_pos(__builtin_amdgcn_readfirstlane(threadIdx.x >> 6), wm, wn);
+    constexpr bool kSkip = MODE != kTileInterior && G::WM == 4 && G::WN == 2;
+    if (!kSkip || (m0 + wm * G::kWTM < M && n0 + wn * G::kWTN < N))
+        bf16x3_block_mma_impl<G, MODE, true>(smem, Ag, Bg, M, N, K, lda, ldb, m0, n0, wm, wn, acc);
+    else
+        bf16x3_block_mma_impl<G, MODE, false>(smem, Ag, Bg, M, N, K, lda, ldb, m0, n0, wm, wn, acc);
+}
+
 // Epilogue: stores acc to C (Cg = the matrix) at the tile (m0, n0); outside
 // kTileInterior the stores are predicated on M x N.
 template <class G, int MODE>
 __device__ __forceinline__ void bf16x3_block_store(float *__restrict__ Cg, int M, int N, int ldc,
                                                    int m0, int n0,
                                                    const floatx16 (&acc)[G::kFM][G::kFN]) {
-    constexpr int kWM = G::WM, kWTM = G::kWTM, kWTN = G::kWTN, kFM = G::kFM, kFN = G::kFN;
+    constexpr int kWTM = G::kWTM, kWTN = G::kWTN, kFM = G::kFM, kFN = G::kFN;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wm = wave % kWM, wn = wave / kWM;
+    int wm, wn;
+    G::wave_pos(wave, wm, wn);
     const int l32 = lane & 31, h = lane >> 5;
 #pragma unroll
     for (int r = 0; r < kFM; ++r) {
