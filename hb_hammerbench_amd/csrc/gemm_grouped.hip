@@ -18,8 +18,11 @@
 //        problems, edge tiles with clamped loads; its tiles are dealt in
 //        XCD-local raster blocks;
 //      - the 256-thread launch, two workgroups per CU, runs 128 x 128 to
-//        64 x 64 tiles with the same main loops over each item's K range,
-//        balanced by longest-processing-time-first bin packing;
+//        64 x 64 tiles with the same main loops over each item's K range; its
+//        workgroups take items from one list in decreasing cost order through
+//        an atomic queue head (greedy list scheduling, which adapts to the
+//        measured rather than the modelled item times), and the last
+//        workgroup to leave resets the queue for the next run;
 //  * split-K tiles write FP32 partials; the last of a tile's splits to finish
 //    (an agent-scope counter) adds them in split order -- deterministic -- and
 //    resets the counter to zero for the next run;
@@ -50,11 +53,11 @@ struct hbs_grouped_plan {
     int64_t tiles;
     int64_t split_tiles;
     double est_imbalance;
-    void *dmem;  // problems | items | block offsets | counters | partials
+    void *dmem;  // problems | items | counters + queue | partials | large-tile items, offsets
     const void *d_probs;
-    const void *d_items;
-    const int *d_boff;
-    int *d_counters;
+    const void *d_items;  // 256-thread launch: one list, decreasing cost
+    int *d_counters;      // split-K counters
+    int *d_queue;         // queue head and finished-workgroup count (zero at rest)
     float *d_partials;
     // large-tile launch (256 x 256 tiles over the 256-aligned cores)
     int nblocks_big;
@@ -210,13 +213,22 @@ constexpr int kLdsB3 = 2 * hbs::B3Cfg<128, 128, 32, 2, 2>::kStage * 2;
 template <int PREC>
 __global__ __launch_bounds__(kThreads, 2) void grouped_kernel(const Prob *__restrict__ probs,
                                                               const Item *__restrict__ items,
-                                                              const int *__restrict__ boff,
-                                                              float *partials, int *counters) {
+                                                              int nitems, float *partials,
+                                                              int *counters, int *queue) {
     constexpr int LDS = PREC == HBS_PREC_FP32 ? kLdsF32 : kLdsB3;
     __shared__ __attribute__((aligned(16))) char smem[LDS];
-    __shared__ int s_flag;
-    const int b = blockIdx.x;
-    for (int i = boff[b]; i < boff[b + 1]; ++i) {
+    __shared__ int s_flag, s_next;
+    // queue[0]: next item to hand out; queue[1]: workgroups that have left.
+    // Thread 0 fetches the next index when an item starts and publishes it in
+    // LDS when the item ends, so the fetch is hidden behind the item.
+    if (threadIdx.x == 0)
+        s_next = __hip_atomic_fetch_add(&queue[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    int i = s_next;
+    while (i < nitems) {
+        int next = 0;
+        if (threadIdx.x == 0)
+            next = __hip_atomic_fetch_add(&queue[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const Item it = items[i];
         const Prob p = probs[it.prob];
         if constexpr (PREC == HBS_PREC_FP32) {
@@ -236,7 +248,18 @@ __global__ __launch_bounds__(kThreads, 2) void grouped_kernel(const Prob *__rest
                 default: b3_tile<64, 128>(s, p, it, partials, counters, &s_flag); break;
             }
         }
-        __syncthreads();  // s_flag and LDS are reused by the next item
+        if (threadIdx.x == 0) s_next = next;
+        __syncthreads();  // s_flag and LDS are reused by the next item; s_next is set
+        i = s_next;
+        __syncthreads();  // every thread has read s_next
+    }
+    // every fetch of this workgroup has returned; the last workgroup to get
+    // here resets the queue (no fetch can follow: all others have left)
+    if (threadIdx.x == 0 &&
+        __hip_atomic_fetch_add(&queue[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+            (int)gridDim.x - 1) {
+        __hip_atomic_store(&queue[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&queue[1], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -462,10 +485,12 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
 
     if (items.size() > 0x7fffffffULL || items_big.size() > 0x7fffffffULL) return HBS_ERR_ARG;
 
-    // LPT: items in decreasing cost, each to the least-loaded workgroup; the
-    // item lists of the workgroups are concatenated with offsets boff.
+    // 256-thread launch: items in decreasing cost, handed out in that order to
+    // whichever workgroup is free first.  With the modelled costs this is
+    // longest-processing-time-first list scheduling; the same simulation
+    // gives the imbalance estimate.
     auto lpt = [](const std::vector<Item> &its, const std::vector<double> &cst, int nb,
-                  std::vector<Item> &sorted, std::vector<int> &boff, double &imbalance) {
+                  std::vector<Item> &sorted, double &imbalance) {
         std::vector<int64_t> order(its.size());
         for (size_t i = 0; i < order.size(); ++i) order[i] = (int64_t)i;
         std::stable_sort(order.begin(), order.end(),
@@ -473,11 +498,9 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
         using Load = std::pair<double, int>;
         std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
         for (int b = 0; b < nb; ++b) heap.push(Load(0.0, b));
-        std::vector<std::vector<int64_t>> lists(nb);
         for (int64_t idx : order) {
             Load l = heap.top();
             heap.pop();
-            lists[l.second].push_back(idx);
             l.first += cst[idx];
             heap.push(l);
         }
@@ -490,12 +513,7 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
         imbalance = suml > 0.0 ? maxl / (suml / nb) : 1.0;
         sorted.clear();
         sorted.reserve(its.size());
-        boff.assign(nb + 1, 0);
-        for (int b = 0; b < nb; ++b) {
-            boff[b] = (int)sorted.size();
-            for (int64_t idx : lists[b]) sorted.push_back(its[idx]);
-        }
-        boff[nb] = (int)sorted.size();
+        for (int64_t idx : order) sorted.push_back(its[idx]);
     };
     // Large-tile launch: one workgroup per CU, workgroup b on XCD b % 8.  The
     // tiles go out in waves of one per workgroup, in the order problems by K
@@ -550,22 +568,21 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
     const int nblocks = (int)std::min<int64_t>((int64_t)per_cu * cus, (int64_t)items.size());
     const int nblocks_big = (int)std::min<int64_t>(cus, (int64_t)items_big.size());
     std::vector<Item> sorted, sorted_big;
-    std::vector<int> boff, boff_big;
+    std::vector<int> boff_big;
     double imb = 1.0, imb_big = 1.0;
-    lpt(items, cost, nblocks, sorted, boff, imb);
+    lpt(items, cost, nblocks, sorted, imb);
     xcd_waves(items_big, cost_big, nblocks_big, sorted_big, boff_big, imb_big);
 
-    // one device allocation: problems | items | offsets | counters | partials
-    // | large-tile items | large-tile offsets
+    // one device allocation: problems | items | split counters + queue |
+    // partials | large-tile items | large-tile offsets
     auto align = [](size_t x) { return (x + 255) / 256 * 256; };
     const size_t b_probs = align(probs.size() * sizeof(Prob));
     const size_t b_items = align(std::max<size_t>(sorted.size(), 1) * sizeof(Item));
-    const size_t b_boff = align(boff.size() * sizeof(int));
-    const size_t b_cnt = align(std::max<int64_t>(slots, 1) * sizeof(int));
+    const size_t b_cnt = align((slots + 2) * sizeof(int));  // split counters, queue
     const size_t b_part = align(std::max<int64_t>(poff, 1) * sizeof(float));
     const size_t b_items_big = align(std::max<size_t>(sorted_big.size(), 1) * sizeof(Item));
     const size_t b_boff_big = align(boff_big.size() * sizeof(int));
-    const size_t o_items = b_probs, o_boff = o_items + b_items, o_cnt = o_boff + b_boff;
+    const size_t o_items = b_probs, o_cnt = o_items + b_items;
     const size_t o_part = o_cnt + b_cnt, o_items_big = o_part + b_part;
     const size_t o_boff_big = o_items_big + b_items_big, total = o_boff_big + b_boff_big;
     char *dmem = nullptr;
@@ -582,8 +599,8 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
     plan->dmem = dmem;
     plan->d_probs = dmem;
     plan->d_items = dmem + o_items;
-    plan->d_boff = (const int *)(dmem + o_boff);
     plan->d_counters = (int *)(dmem + o_cnt);
+    plan->d_queue = plan->d_counters + slots;
     plan->d_partials = (float *)(dmem + o_part);
     plan->nblocks_big = nblocks_big;
     plan->items_big = (int64_t)sorted_big.size();
@@ -593,8 +610,6 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
     if (e == hipSuccess && !sorted.empty())
         e = hipMemcpy(dmem + o_items, sorted.data(), sorted.size() * sizeof(Item),
                       hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = hipMemcpy(dmem + o_boff, boff.data(), boff.size() * sizeof(int), hipMemcpyHostToDevice);
     if (e == hipSuccess && !sorted_big.empty())
         e = hipMemcpy(dmem + o_items_big, sorted_big.data(), sorted_big.size() * sizeof(Item),
                       hipMemcpyHostToDevice);
@@ -670,10 +685,10 @@ extern "C" int hbs_grouped_run(const hbs_grouped_plan *plan, void *stream) {
     if (plan->nblocks > 0) {
         if (fp32)
             grouped_kernel<HBS_PREC_FP32><<<plan->nblocks, kThreads, 0, s>>>(
-                pr, it, plan->d_boff, plan->d_partials, plan->d_counters);
+                pr, it, (int)plan->items, plan->d_partials, plan->d_counters, plan->d_queue);
         else
             grouped_kernel<HBS_PREC_BF16X3><<<plan->nblocks, kThreads, 0, s>>>(
-                pr, it, plan->d_boff, plan->d_partials, plan->d_counters);
+                pr, it, (int)plan->items, plan->d_partials, plan->d_counters, plan->d_queue);
         return hbs::launch_status();
     }
     return HBS_SUCCESS;
