@@ -16,7 +16,7 @@
 //      - the large-tile launch, one 512-thread workgroup per CU, runs the
 //        square kernels' 256 x 256 tile (tile_f32.h, tile_bf16x3.h) over whole
 //        problems, edge tiles with clamped loads; its tiles are dealt in
-//        XCD-local raster blocks;
+//        XCD-local raster blocks to one item queue per XCD;
 //      - the 256-thread launch, two workgroups per CU, runs 128 x 128 to
 //        64 x 64 tiles with the same main loops over each item's K range; its
 //        workgroups take items from one list in decreasing cost order through
@@ -53,7 +53,7 @@ struct hbs_grouped_plan {
     int64_t tiles;
     int64_t split_tiles;
     double est_imbalance;
-    void *dmem;  // problems | items | counters + queue | partials | large-tile items, offsets
+    void *dmem;  // problems | items | counters + queues | partials | large-tile items, offsets
     const void *d_probs;
     const void *d_items;  // 256-thread launch: one list, decreasing cost
     int *d_counters;      // split-K counters
@@ -63,7 +63,9 @@ struct hbs_grouped_plan {
     int nblocks_big;
     int64_t items_big;
     const void *d_items_big;
-    const int *d_boff_big;
+    const int *d_qoff_big;  // item range of each of its nq_big queues
+    int nq_big;
+    int *d_queue_big;       // nq_big heads and a finished-workgroup count
 };
 
 namespace {
@@ -271,17 +273,32 @@ __global__ __launch_bounds__(kThreads, 2) void grouped_kernel(const Prob *__rest
 // split-K; K must be a multiple of the 32-deep stage.
 // ---------------------------------------------------------------------------
 using BigF32 = hbs::F32Cfg<256, 256, 32, 4, 2>;
+constexpr int kXcdQueues = 8;  // one item queue per XCD on the large-tile launch
 using BigB3 = hbs::B3Main;
 static_assert(BigF32::kThreads == kBigThreads && BigB3::kThreads == kBigThreads, "big tiles");
 
 template <int PREC>
 __global__ __launch_bounds__(kBigThreads) void grouped_big_kernel(const Prob *__restrict__ probs,
                                                                   const Item *__restrict__ items,
-                                                                  const int *__restrict__ boff) {
+                                                                  const int *__restrict__ qoff,
+                                                                  int nq, int *queue) {
     constexpr int LDS = PREC == HBS_PREC_FP32 ? BigF32::kLdsFloats * 4 : 2 * BigB3::kStage * 2;
     __shared__ __attribute__((aligned(16))) char smem[LDS];
-    const int b = blockIdx.x;
-    for (int i = boff[b]; i < boff[b + 1]; ++i) {
+    __shared__ int s_next;
+    // queue q = blockIdx.x % nq (the workgroup's XCD when nq = 8) hands out
+    // items[qoff[q] .. qoff[q + 1]) in order; queue[q] is its head and
+    // queue[kXcdQueues] counts the workgroups that have left
+    const int q = blockIdx.x % nq, end = qoff[q + 1];
+    if (threadIdx.x == 0)
+        s_next = qoff[q] +
+                 __hip_atomic_fetch_add(&queue[q], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    int i = s_next;
+    while (i < end) {
+        int next = 0;
+        if (threadIdx.x == 0)
+            next = qoff[q] + __hip_atomic_fetch_add(&queue[q], 1, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
         const Item it = items[i];
         const Prob p = probs[it.prob];
         const int m0 = it.tm * kBig, n0 = it.tn * kBig;
@@ -293,7 +310,16 @@ __global__ __launch_bounds__(kBigThreads) void grouped_big_kernel(const Prob *__
         else
             hbs::bf16x3_block_tile<BigB3, hbs::kTileClamp>((uint16_t *)smem, Ag, Bg, p.C, p.M, p.N,
                                                            p.K, p.lda, p.ldb, p.ldc, m0, n0);
-        __syncthreads();  // LDS is reused by the next item
+        if (threadIdx.x == 0) s_next = next;
+        __syncthreads();  // LDS is reused by the next item; s_next is set
+        i = s_next;
+        __syncthreads();  // every thread has read s_next
+    }
+    if (threadIdx.x == 0 &&
+        __hip_atomic_fetch_add(&queue[kXcdQueues], 1, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT) == (int)gridDim.x - 1) {
+        for (int j = 0; j <= kXcdQueues; ++j)
+            __hip_atomic_store(&queue[j], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -343,16 +369,16 @@ bool tile_supported(int precision, int tile) {
 }
 
 // The large-tile path needs K a multiple of its 32-deep stage.  By default
-// every BF16x3 problem that allows it takes it, whatever its M, N and K: the
-// 256 x 256 BF16x3 tile runs at two to five times the per-CU rate of the
-// 128 x 128 tiles of two workgroups per CU, which outweighs the padding of
-// thin problems (M or N of 64 to 192) and the fixed cost of short K, and one
-// launch leaves no second tail.  FP32 stays on the 256-thread launch by
-// default (there the two launches measure the same); tuning files and forced
-// plans can select the path for either precision.
+// every problem that allows it takes it, whatever its M, N and K, in both
+// precisions: the 256 x 256 tile runs at a higher per-CU rate than the 128 x
+// 128 tiles of two workgroups per CU (for BF16x3 two to five times), its edge
+// tiles skip the MFMAs of waves outside M x N and its per-XCD queues even out
+// their shorter times, and one launch leaves no second tail.  Tuning files and
+// forced plans can still select the 256-thread tiles per size class.
 bool big_possible(int M, int N, int K) { (void)M; (void)N; return K % 32 == 0; }
 bool big_default(int precision, int M, int N, int K) {
-    return precision == HBS_PREC_BF16X3 && big_possible(M, N, K);
+    (void)precision;
+    return big_possible(M, N, K);
 }
 
 Choice heuristic(int precision, int M, int N, int K) {
@@ -515,15 +541,17 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
         sorted.reserve(its.size());
         for (int64_t idx : order) sorted.push_back(its[idx]);
     };
-    // Large-tile launch: one workgroup per CU, workgroup b on XCD b % 8.  The
-    // tiles go out in waves of one per workgroup, in the order problems by K
-    // (longest first), tiles of a problem in a raster of 4 tile rows, and
-    // within a wave 32 consecutive tiles to the 32 workgroups of one XCD: the
-    // tiles an XCD runs at the same time are a compact block of one problem
-    // and share A and B panels in that XCD's L2.  Tiles of one wave have about
-    // the same K, so every workgroup gets a similar load.
+    // Large-tile launch: one workgroup per CU, workgroup b on XCD b % 8, and
+    // one item queue per XCD.  The tiles are ordered by problem K (longest
+    // first), then by problem, then in a raster of 4 tile rows, and dealt to
+    // the queues in runs of 32 (one per workgroup of an XCD): the tiles an XCD
+    // runs at the same time are a compact block of one problem and share A and
+    // B panels in that XCD's L2.  Within an XCD the workgroups take tiles in
+    // queue order as they finish, so edge tiles that run short (their idle
+    // waves skip the MFMAs) and tiles that run long even out.
     auto xcd_waves = [&](const std::vector<Item> &its, const std::vector<double> &cst, int nb,
-                         std::vector<Item> &sorted, std::vector<int> &boff, double &imbalance) {
+                         std::vector<Item> &sorted, std::vector<int> &qoff, int &nq,
+                         double &imbalance) {
         std::vector<int64_t> order(its.size());
         for (size_t i = 0; i < order.size(); ++i) order[i] = (int64_t)i;
         auto key = [&](int64_t x) {
@@ -537,29 +565,37 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
         };
         std::stable_sort(order.begin(), order.end(),
                          [&](int64_t x, int64_t y) { return key(x) < key(y); });
-        constexpr int kXcd = 8;
-        const int per = nb % kXcd == 0 ? nb / kXcd : nb;  // workgroups per XCD
-        std::vector<std::vector<int64_t>> lists(nb);
-        std::vector<double> load(nb, 0.0);
-        for (size_t t = 0; t < order.size(); ++t) {
-            const int j = (int)(t % nb);
-            const int b = nb % kXcd == 0 ? (j % per) * kXcd + j / per : j;
-            lists[b].push_back(order[t]);
-            load[b] += cst[order[t]];
-        }
+        nq = nb % kXcdQueues == 0 ? kXcdQueues : 1;
+        const int per = nb / nq;  // workgroups per queue
+        std::vector<std::vector<int64_t>> lists(nq);
+        for (size_t t = 0; t < order.size(); ++t) lists[(t % nb) / per].push_back(order[t]);
+        // imbalance estimate: each queue's list scheduled in order onto its
+        // `per` workgroups with the modelled costs
         double maxl = 0.0, suml = 0.0;
-        for (double l : load) {
-            maxl = std::max(maxl, l);
-            suml += l;
+        for (int q = 0; q < nq; ++q) {
+            using Load = std::pair<double, int>;
+            std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
+            for (int w = 0; w < per; ++w) heap.push(Load(0.0, w));
+            for (int64_t idx : lists[q]) {
+                Load l = heap.top();
+                heap.pop();
+                l.first += cst[idx];
+                suml += cst[idx];
+                heap.push(l);
+            }
+            while (!heap.empty()) {
+                maxl = std::max(maxl, heap.top().first);
+                heap.pop();
+            }
         }
         imbalance = suml > 0.0 ? maxl / (suml / nb) : 1.0;
         sorted.clear();
-        boff.assign(nb + 1, 0);
-        for (int b = 0; b < nb; ++b) {
-            boff[b] = (int)sorted.size();
-            for (int64_t idx : lists[b]) sorted.push_back(its[idx]);
+        qoff.assign(nq + 1, 0);
+        for (int q = 0; q < nq; ++q) {
+            qoff[q] = (int)sorted.size();
+            for (int64_t idx : lists[q]) sorted.push_back(its[idx]);
         }
-        boff[nb] = (int)sorted.size();
+        qoff[nq] = (int)sorted.size();
     };
     // resident workgroups per CU on the 256-thread launch: two (up to 256
     // VGPRs each; 38 KB LDS for FP32, 74 KB for BF16x3); the large-tile launch
@@ -568,23 +604,26 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
     const int nblocks = (int)std::min<int64_t>((int64_t)per_cu * cus, (int64_t)items.size());
     const int nblocks_big = (int)std::min<int64_t>(cus, (int64_t)items_big.size());
     std::vector<Item> sorted, sorted_big;
-    std::vector<int> boff_big;
+    std::vector<int> qoff_big;
+    int nq_big = 1;
     double imb = 1.0, imb_big = 1.0;
     lpt(items, cost, nblocks, sorted, imb);
-    xcd_waves(items_big, cost_big, nblocks_big, sorted_big, boff_big, imb_big);
+    xcd_waves(items_big, cost_big, nblocks_big, sorted_big, qoff_big, nq_big, imb_big);
 
-    // one device allocation: problems | items | split counters + queue |
-    // partials | large-tile items | large-tile offsets
+    // one device allocation: problems | items | split counters + queues |
+    // partials | large-tile items | large-tile queue offsets
     auto align = [](size_t x) { return (x + 255) / 256 * 256; };
     const size_t b_probs = align(probs.size() * sizeof(Prob));
     const size_t b_items = align(std::max<size_t>(sorted.size(), 1) * sizeof(Item));
-    const size_t b_cnt = align((slots + 2) * sizeof(int));  // split counters, queue
+    // split counters, the 256-thread launch's queue (2), the large-tile
+    // launch's queues (kXcdQueues heads + 1)
+    const size_t b_cnt = align((slots + 2 + kXcdQueues + 1) * sizeof(int));
     const size_t b_part = align(std::max<int64_t>(poff, 1) * sizeof(float));
     const size_t b_items_big = align(std::max<size_t>(sorted_big.size(), 1) * sizeof(Item));
-    const size_t b_boff_big = align(boff_big.size() * sizeof(int));
+    const size_t b_qoff_big = align(qoff_big.size() * sizeof(int));
     const size_t o_items = b_probs, o_cnt = o_items + b_items;
     const size_t o_part = o_cnt + b_cnt, o_items_big = o_part + b_part;
-    const size_t o_boff_big = o_items_big + b_items_big, total = o_boff_big + b_boff_big;
+    const size_t o_qoff_big = o_items_big + b_items_big, total = o_qoff_big + b_qoff_big;
     char *dmem = nullptr;
     HBS_HIP_TRY(hipMalloc(&dmem, total));
     hbs_grouped_plan *plan = new hbs_grouped_plan();
@@ -601,11 +640,13 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
     plan->d_items = dmem + o_items;
     plan->d_counters = (int *)(dmem + o_cnt);
     plan->d_queue = plan->d_counters + slots;
+    plan->d_queue_big = plan->d_queue + 2;
     plan->d_partials = (float *)(dmem + o_part);
     plan->nblocks_big = nblocks_big;
     plan->items_big = (int64_t)sorted_big.size();
     plan->d_items_big = dmem + o_items_big;
-    plan->d_boff_big = (const int *)(dmem + o_boff_big);
+    plan->d_qoff_big = (const int *)(dmem + o_qoff_big);
+    plan->nq_big = nq_big;
     hipError_t e = hipMemcpy(dmem, probs.data(), probs.size() * sizeof(Prob), hipMemcpyHostToDevice);
     if (e == hipSuccess && !sorted.empty())
         e = hipMemcpy(dmem + o_items, sorted.data(), sorted.size() * sizeof(Item),
@@ -614,7 +655,7 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
         e = hipMemcpy(dmem + o_items_big, sorted_big.data(), sorted_big.size() * sizeof(Item),
                       hipMemcpyHostToDevice);
     if (e == hipSuccess)
-        e = hipMemcpy(dmem + o_boff_big, boff_big.data(), boff_big.size() * sizeof(int),
+        e = hipMemcpy(dmem + o_qoff_big, qoff_big.data(), qoff_big.size() * sizeof(int),
                       hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(plan->d_counters, 0, b_cnt);
     if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -675,10 +716,10 @@ extern "C" int hbs_grouped_run(const hbs_grouped_plan *plan, void *stream) {
     if (plan->nblocks_big > 0) {
         if (fp32)
             grouped_big_kernel<HBS_PREC_FP32><<<plan->nblocks_big, kBigThreads, 0, s>>>(
-                pr, itb, plan->d_boff_big);
+                pr, itb, plan->d_qoff_big, plan->nq_big, plan->d_queue_big);
         else
             grouped_big_kernel<HBS_PREC_BF16X3><<<plan->nblocks_big, kBigThreads, 0, s>>>(
-                pr, itb, plan->d_boff_big);
+                pr, itb, plan->d_qoff_big, plan->nq_big, plan->d_queue_big);
         const int rc = hbs::launch_status();
         if (rc) return rc;
     }

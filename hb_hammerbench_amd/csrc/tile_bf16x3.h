@@ -25,18 +25,18 @@ struct B3Cfg {
     static constexpr int kStage = (BM + BN) * kLdsRow;    // bf16 elements per stage
     static_assert(kWTM % 32 == 0 && kWTN % 32 == 0, "wave tile");
     static_assert(2 * kStage * 2 <= 160 * 1024, "LDS");
-    // Sub-tile (wm, wn) of wave `wave`.  With 8 waves as 4 x 2, the waves of
-    // each pair (2s, 2s + 1) take different column halves and row blocks two
-    // apart, so that in an edge tile with only its top rows or left columns
-    // inside M x N both waves of a pair are rarely idle together.  With the
-    // MFMAs of such idle waves skipped, this pairing measured 1% faster on a
-    // group of ragged problems, where pairing w with w + 4 measured no gain:
-    // consistent with waves 2s, 2s + 1 sharing a SIMD.  Other shapes use
-    // wm = wave % WM, wn = wave / WM.
+    // Sub-tile (wm, wn) of wave `wave`.  With 8 waves as 4 x 2, waves w and
+    // w + 4 share a SIMD (read from HW_ID on MI355X: the four SIMDs take waves
+    // 0..3 in some order, then 4..7 in the same order).  Rotating the rows of
+    // the second wave column by half the tile gives those two waves different
+    // row halves and different column halves, so that in an edge tile with
+    // only its top rows or left columns inside M x N, where the MFMAs of
+    // waves wholly outside are skipped, every SIMD keeps work.  Other shapes
+    // use wm = wave % WM, wn = wave / WM.
     __device__ static void wave_pos(int wave, int &wm, int &wn) {
         if (WM == 4 && WN == 2) {
-            wn = wave & 1;
-            wm = ((wave >> 1) + 2 * wn) & 3;
+            wn = wave >> 2;
+            wm = (wave + 2 * wn) & 3;
         } else {
             wn = wave / WM;
             wm = wave % WM;

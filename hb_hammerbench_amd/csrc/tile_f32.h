@@ -34,6 +34,23 @@ struct F32Cfg {
     static_assert((BM * BK / 4) % kThreads == 0, "A tile must split evenly over threads");
     static_assert((BK * BN / 4) % kThreads == 0, "B tile must split evenly over threads");
     static_assert(BK % 16 == 0, "BK must be a multiple of 16 (two or more 8-k chunks)");
+    // Sub-tile (wm, wn) of wave `wave`.  With 8 waves as 4 x 2, waves w and
+    // w + 4 share a SIMD (read from HW_ID on MI355X: the four SIMDs take waves
+    // 0..3 in some order, then 4..7 in the same order).  Rotating the rows of
+    // the second wave column by half the tile gives those two waves different
+    // row halves and different column halves, so that in an edge tile with
+    // only its top rows or left columns inside M x N, where the MFMAs of
+    // waves wholly outside are skipped, every SIMD keeps work.  Other shapes
+    // use wm = wave % WM, wn = wave / WM.
+    __device__ static void wave_pos(int wave, int &wm, int &wn) {
+        if (WM == 4 && WN == 2) {
+            wn = wave >> 2;
+            wm = (wave + 2 * wn) & 3;
+        } else {
+            wn = wave / WM;
+            wm = wave % WM;
+        }
+    }
 };
 
 // Main loop: acc (this lane's kFM x kFN 32x32 blocks of the wave's sub-tile)
@@ -44,18 +61,17 @@ struct F32Cfg {
 // workgroup must call it.  MODE (TileMode): kTileInterior, kTileEdge (any
 // shape or alignment), kTileClamp (ragged M or N; K a multiple of BK, N of 4,
 // 16-byte loadable operands) or kTileClampK (the same with K a multiple of 4).
-template <int BM, int BN, int BK, int WM, int WN, int MODE>
-__device__ __forceinline__ void f32_block_mma(
+template <int BM, int BN, int BK, int WM, int WN, int MODE, bool COMPUTE>
+__device__ __forceinline__ void f32_block_mma_impl(
     float *smem, const float *__restrict__ Ag, const float *__restrict__ Bg, int M, int N, int K,
-    int lda, int ldb, int m0, int n0,
+    int lda, int ldb, int m0, int n0, int wm, int wn,
     floatx16 (&acc)[F32Cfg<BM, BN, BK, WM, WN>::kFM][F32Cfg<BM, BN, BK, WM, WN>::kFN]) {
     using G = F32Cfg<BM, BN, BK, WM, WN>;
     float *const As = smem;
     float *const Bs = smem + 2 * G::kStageA;
 
     const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
-    const int wm = wave % WM, wn = wave / WM;
+    const int lane = tid & 63;
     const int l32 = lane & 31, h = lane >> 5;
 
 #pragma unroll
@@ -162,6 +178,10 @@ __device__ __forceinline__ void f32_block_mma(
     // flight, so the matrix core is never left waiting on a fresh LDS read.
     constexpr int kChunks = BK / 8, kHook = kChunks / 2;
     auto compute = [&](int buf, auto &&hook) {
+        if constexpr (!COMPUTE) {
+            hook();
+            return;
+        }
         const float *as = As + buf * G::kStageA + (wm * G::kWTM + l32) * G::kLdA + 4 * h;
         const float *bs = Bs + buf * G::kStageB + 4 * h * G::kLdB + wn * G::kWTN + l32;
         floatx4 a0[G::kFM], a1[G::kFM];
@@ -204,7 +224,27 @@ __device__ __forceinline__ void f32_block_mma(
         ++kt;
     }
     compute(kt & 1, [&]() {});
+}
 
+// With 8 waves as 4 x 2, a wave whose sub-tile lies wholly outside M x N
+// (edge tiles outside kTileInterior) runs the same loop without fragment reads
+// and MFMAs: it still loads and stores its share of each stage and meets every
+// barrier.
+template <int BM, int BN, int BK, int WM, int WN, int MODE>
+__device__ __forceinline__ void f32_block_mma(
+    float *smem, const float *__restrict__ Ag, const float *__restrict__ Bg, int M, int N, int K,
+    int lda, int ldb, int m0, int n0,
+    floatx16 (&acc)[F32Cfg<BM, BN, BK, WM, WN>::kFM][F32Cfg<BM, BN, BK, WM, WN>::kFN]) {
+    using G = F32Cfg<BM, BN, BK, WM, WN>;
+    int wm, wn;
+    G::wave_pos(__builtin_amdgcn_readfirstlane(threadIdx.x >> 6), wm, wn);
+    constexpr bool kSkip = MODE != kTileInterior && WM == 4 && WN == 2;
+    if (!kSkip || (m0 + wm * G::kWTM < M && n0 + wn * G::kWTN < N))
+        f32_block_mma_impl<BM, BN, BK, WM, WN, MODE, true>(smem, Ag, Bg, M, N, K, lda, ldb, m0,
+                                                           n0, wm, wn, acc);
+    else
+        f32_block_mma_impl<BM, BN, BK, WM, WN, MODE, false>(smem, Ag, Bg, M, N, K, lda, ldb, m0,
+                                                            n0, wm, wn, acc);
 }
 
 // Epilogue: stores acc to C (Cg = the matrix) at the tile (m0, n0).  32x32
@@ -217,7 +257,8 @@ __device__ __forceinline__ void f32_block_store(
     const floatx16 (&acc)[F32Cfg<BM, BN, BK, WM, WN>::kFM][F32Cfg<BM, BN, BK, WM, WN>::kFN]) {
     using G = F32Cfg<BM, BN, BK, WM, WN>;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wm = wave % WM, wn = wave / WM;
+    int wm, wn;
+    G::wave_pos(wave, wm, wn);
     const int l32 = lane & 31, h = lane >> 5;
 #pragma unroll
     for (int r = 0; r < G::kFM; ++r) {

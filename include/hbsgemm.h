@@ -129,7 +129,7 @@ typedef struct {
  * with N, K, lda, ldb multiples of 4 (HBS_ERR_ALIGN otherwise).  Tile shape
  * and split-K come from `tuning_file` (written by the autotuner) when it is
  * non-NULL -- HBS_ERR_ARG if it cannot be read -- else from a size heuristic
- * (for BF16x3, HBS_TILE_256x256 whenever K is a multiple of 32).  Allocates and uploads the plan's device buffers (synchronous). */
+ * (HBS_TILE_256x256 whenever K is a multiple of 32).  Allocates and uploads the plan's device buffers (synchronous). */
 int hbs_grouped_plan_create(const hbs_gemm_problem *problems, int count, int precision,
                             const char *tuning_file, hbs_grouped_plan **plan);
 /* Same with one tile shape and split-K factor (1..64) for every problem (the

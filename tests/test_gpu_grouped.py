@@ -153,8 +153,8 @@ def big_tiles(shapes, prec, forced):
     for M, N, K in shapes:
         if forced:
             big = K % 32 == 0
-        else:  # the default: BF16x3 with K a multiple of 32
-            big = prec == hb.PREC_BF16X3 and K % 32 == 0
+        else:  # the default: K a multiple of 32
+            big = K % 32 == 0
         if big:
             n += ((M + 255) // 256) * ((N + 255) // 256)
     return n
@@ -184,8 +184,7 @@ def test_grouped_large_tiles_within_bound(prec, mode):
     info = plan.info()
     nbig = big_tiles(BIG, prec, mode != "heuristic")
     assert info["tiles"] > nbig  # (512, 512, 100) is never on the large-tile launch
-    if mode != "heuristic" or prec == hb.PREC_BF16X3:
-        assert nbig > 0
+    assert nbig > 0
     if mode == "forced-split2":
         assert info["split_tiles"] > 0
     plan.run()
@@ -234,18 +233,15 @@ def test_grouped_large_tiles_identity_reproduces_oracle_split():
 
 @pytest.mark.parametrize("prec", [hb.PREC_FP32, hb.PREC_BF16X3])
 def test_grouped_graph_replays_match_eager(prec):
-    # K = 100 keeps one BF16x3 problem on the 256-thread launch, so both
-    # launches are in the graph
+    # K = 100 keeps one problem on the 256-thread launch, so both launches are
+    # in the graph
     shapes = [(256, 192, 512), (64, 64, 2048), (96, 320, 64), (1024, 64, 1024), (520, 772, 512),
               (200, 300, 100)]
     probs, views, bufs = make_group(shapes, "uniform", seed=3)
     plan = hb.GroupedPlan(probs, prec)
     info = plan.info()
-    if prec == hb.PREC_BF16X3:
-        assert info["tiles"] == sum(((M + 255) // 256) * ((N + 255) // 256)
-                                    for M, N, _ in shapes[:5]) + 2 * 3
-    else:
-        assert info["tiles"] == sum(((M + 127) // 128) * ((N + 127) // 128) for M, N, _ in shapes)
+    assert info["tiles"] == sum(((M + 255) // 256) * ((N + 255) // 256)
+                                for M, N, _ in shapes[:5]) + 2 * 3
     plan.run()
     torch.cuda.synchronize()
     eager = [C.clone() for (_, _, C) in views]
