@@ -173,9 +173,10 @@ def test_grouped_large_tiles_integer_exact(prec):
 @pytest.mark.parametrize("prec", [hb.PREC_FP32, hb.PREC_BF16X3])
 @pytest.mark.parametrize("mode", ["forced", "forced-split2", "heuristic"])
 def test_grouped_large_tiles_within_bound(prec, mode):
-    """Large-tile cores plus remainders on the 256-thread launch: every
-    problem within its FP64 bound, a second run identical, and the plan's
-    tile count includes the 256 x 256 tiles."""
+    """Whole problems on the large-tile launch (edge tiles with idle waves,
+    per-XCD queues) and K = 100 on the 256-thread launch: every problem within
+    its FP64 bound, a second run identical (the queues reset themselves), and
+    the plan's tile count includes the 256 x 256 tiles."""
     probs, views, _ = make_group(BIG, "uniform", seed=29)
     if mode == "heuristic":
         plan = hb.GroupedPlan(probs, prec)
