@@ -218,6 +218,7 @@ static int group_mode(const char *shapes_path, const char *out_path, const std::
                    100.0 * cl.first, keep.tile >= 0 ? "override" : "heuristic", best);
             fflush(stdout);
         }
+        if (ov.empty()) best = base;  // no override: the re-times above were noise
         printf("%s tuned: %.3f ms (%.1f TFLOP/s), heuristic %.3f ms, %zu overrides\n", pname, best,
                total / (best * 1e-3) / 1e12, base, ov.size());
         fflush(stdout);
