@@ -129,7 +129,8 @@ typedef struct {
  * with N, K, lda, ldb multiples of 4 (HBS_ERR_ALIGN otherwise).  Tile shape
  * and split-K come from `tuning_file` (written by the autotuner) when it is
  * non-NULL -- HBS_ERR_ARG if it cannot be read -- else from a size heuristic
- * (HBS_TILE_256x256 whenever K is a multiple of 32).  Allocates and uploads the plan's device buffers (synchronous). */
+ * (HBS_TILE_256x256 whenever K is a multiple of 32).  Allocates and uploads
+ * the plan's device buffers (synchronous). */
 int hbs_grouped_plan_create(const hbs_gemm_problem *problems, int count, int precision,
                             const char *tuning_file, hbs_grouped_plan **plan);
 /* Same with one tile shape and split-K factor (1..64) for every problem (the
@@ -140,9 +141,11 @@ int hbs_grouped_plan_create_forced(const hbs_gemm_problem *problems, int count, 
                                    int tile, int split_k, hbs_grouped_plan **plan);
 int hbs_grouped_plan_info(const hbs_grouped_plan *plan, hbs_grouped_info *info);
 /* Computes every problem of the plan on `stream`: the large-tile launch (when
- * the plan has 256 x 256 tiles), then the 256-thread persistent launch.  Both
- * precisions read the operands in place.  Deterministic; runs of one plan must
- * not overlap (they share split-K counters and partial buffers). */
+ * the plan has 256 x 256 tiles), then the 256-thread persistent launch (when
+ * it has smaller tiles).  Workgroups take their items from queues in plan
+ * memory, which reset themselves at the end of each launch.  Both precisions
+ * read the operands in place.  Deterministic; runs of one plan must not
+ * overlap (they share the queues, split-K counters and partial buffers). */
 int hbs_grouped_run(const hbs_grouped_plan *plan, void *stream);
 int hbs_grouped_plan_destroy(hbs_grouped_plan *plan);
 
