@@ -265,10 +265,12 @@ __global__ __launch_bounds__(kThreads) void gemm_fp8res_kernel(GemmArgs p) {
                               __float_as_int(lo[3]), __float_as_int(hi[0]), __float_as_int(hi[1]),
                               __float_as_int(hi[2]), __float_as_int(hi[3])};
             }
+            // consecutive MFMAs share their second operand (as in the BF16x3
+            // tile, where that order measured faster)
 #pragma unroll
-            for (int r = 0; r < kFM; ++r)
+            for (int c = 0; c < kFN; ++c)
 #pragma unroll
-                for (int c = 0; c < kFN; ++c)
+                for (int r = 0; r < kFM; ++r)
                     acc[r][c] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
                         a8[r], b8[c], acc[r][c], 0, 0, 0, ea + 127, 0, eb + 127);
         }
@@ -293,12 +295,14 @@ __global__ __launch_bounds__(kThreads) void gemm_fp8res_kernel(GemmArgs p) {
                 bq[c] = fp8x8_to_bf16(b8[c][2 * s], b8[c][2 * s + 1], sb);
             }
 #pragma unroll
-            for (int r = 0; r < kFM; ++r)
+            for (int c = 0; c < kFN; ++c) {
 #pragma unroll
-                for (int c = 0; c < kFN; ++c) {
+                for (int r = 0; r < kFM; ++r)
                     acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ar[r], bq[c], acc[r][c], 0, 0, 0);
+#pragma unroll
+                for (int r = 0; r < kFM; ++r)
                     acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aq[r], br[c], acc[r][c], 0, 0, 0);
-                }
+            }
         }
     };
 

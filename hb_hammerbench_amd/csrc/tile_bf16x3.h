@@ -184,14 +184,24 @@ __device__ __forceinline__ void bf16x3_block_mma_impl(uint16_t *smem, const floa
             bhi[c] = *(const bf16x8 *)(bsm + c * 32 * kLdsRow + off);
             blo[c] = *(const bf16x8 *)(bsm + c * 32 * kLdsRow + off + 8);
         }
+        // Consecutive MFMAs share their second operand (the B fragment): per
+        // column block, each product term runs over the row blocks.  On the
+        // same inputs this order measured 8% faster than issuing the three
+        // terms of one block back to back (17% with all-zero data), while
+        // sharing the first operand instead measured no gain.  Each
+        // accumulator still receives lo.hi, hi.lo, hi.hi in that order.
 #pragma unroll
-        for (int r = 0; r < kFM; ++r)
+        for (int c = 0; c < kFN; ++c) {
 #pragma unroll
-            for (int c = 0; c < kFN; ++c) {
+            for (int r = 0; r < kFM; ++r)
                 acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[r], bhi[c], acc[r][c], 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < kFM; ++r)
                 acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[r], blo[c], acc[r][c], 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < kFM; ++r)
                 acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[r], bhi[c], acc[r][c], 0, 0, 0);
-            }
+        }
     };
     static_assert(kBK == 32, "two substeps per stage");
 
