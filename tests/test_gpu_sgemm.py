@@ -136,12 +136,16 @@ def test_sgemm_ragged_sizes(N, niter):
         assert np.all(np.abs(C[it] - R) <= gamma(N + 2) * S), (N, it)
 
 
-# (300, 260, 256) and (2400, 2100, 32) take the clamped-load tiles (ragged M
-# and N, K a multiple of the stage depth, aligned strides): the 64 x 64 and
-# the 256 x 256 x 32 configurations respectively.
+# (300, 260, 256) and (2400, 2100, 32 or 320) take the clamped-load tiles
+# (ragged M and N, K a multiple of the stage depth, aligned strides): the
+# 64 x 64 and the 256 x 256 x 32 configurations respectively.  In the 256 x 256
+# configuration, waves whose sub-tile lies outside M x N skip their MFMAs:
+# (4160, 1300, 96) has a bottom edge of 64 rows and a right edge of 20
+# columns, so its corner tiles run one wave of eight.
 @pytest.mark.parametrize("M,N,K,pad", [(200, 136, 72, (5, 3, 1)), (256, 256, 256, (4, 4, 4)),
                                        (64, 512, 1000, (0, 0, 0)), (513, 65, 33, (2, 0, 7)),
-                                       (300, 260, 256, (4, 4, 4)), (2400, 2100, 32, (4, 4, 8))])
+                                       (300, 260, 256, (4, 4, 4)), (2400, 2100, 32, (4, 4, 8)),
+                                       (2400, 2100, 320, (4, 4, 8)), (4160, 1300, 96, (0, 0, 0))])
 def test_sgemm_strided_rectangular(M, N, K, pad):
     batch = 3
     lda, ldb, ldc = K + pad[0], N + pad[1], N + pad[2]
