@@ -160,7 +160,8 @@ def setup_square(args, hb, torch):
     else:
         workload = (f"BF16x3 split SGEMM, square N={N}, NITER={NITER} slices per launch, "
                     f"|x| log-uniform in [1e-3, 1e3] with random sign, seed {args.seed}")
-    extra = {"verify": verify, "config": {"N": N, "niter": NITER},
+    extra = {"verify": verify,
+             "config": {"N": N, "niter": NITER, "bytes_algorithmic": 12.0 * N * N * NITER},
              "traffic": measured_traffic(args.variant, f"N={N} NITER={NITER}")}
     if args.variant == "fp32" and not args.no_compare:
         extra["compare"] = lambda fp32_tflops: bf16x3_comparison(args, hb, torch, a, b, c,
@@ -311,7 +312,8 @@ def setup_fp8res(args, hb, torch):
     workload = (f"FP8-e4m3 per-128x128-tile scales + BF16 residual GEMM, M=N=16384, K=4096, "
                 f"normal(0,1) with 1% x1e3 outliers seed {args.seed}")
     return step, 2.0 * M * N * K, workload, {
-        "verify": verify, "config": {"M": M, "N": N, "K": K},
+        "verify": verify,
+        "config": {"M": M, "N": N, "K": K, "bytes_algorithmic": 4.0 * (M * K + K * N + M * N)},
         "traffic": measured_traffic("fp8res", f"M={M} N={N} K={K}")}
 
 
