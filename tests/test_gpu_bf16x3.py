@@ -36,6 +36,43 @@ def dev_fill(n, kind, seed):
     return x
 
 
+def special_inputs(M=96, N=96, K=96, seed=41):
+    """Uniform inputs with planted NaN and infinities: lone +-Inf and NaN, an
+    Inf times an exact zero (NaN), and +Inf and -Inf products in one sum (NaN);
+    R is the product with IEEE semantics in float64 (elementwise, so no library
+    shortcut can skip a zero)."""
+    rng = np.random.default_rng(seed)
+    A = rng.uniform(-1, 1, (M, K)).astype(np.float32)
+    B = rng.uniform(-1, 1, (K, N)).astype(np.float32)
+    A[3, 5] = np.inf; A[10, 7] = -np.inf; A[20, 1] = np.nan
+    B[2, 30] = np.inf
+    A[60, 40] = np.inf; B[40, 70] = 0.0                        # Inf * 0
+    A[80, 11] = np.inf; A[80, 12] = np.inf; B[11, 90] = 1.0; B[12, 90] = -1.0  # Inf - Inf
+    with np.errstate(invalid="ignore", over="ignore"):
+        R = (A.astype(np.float64)[:, :, None] * B.astype(np.float64)[None, :, :]).sum(axis=1)
+    fa, fb = np.where(np.isfinite(A), np.abs(A), 0.0), np.where(np.isfinite(B), np.abs(B), 0.0)
+    S = fa.astype(np.float64) @ fb.astype(np.float64)
+    return A, B, R, S
+
+
+def test_bf16x3_special_values_stay_non_finite():
+    """A non-finite input makes its results non-finite, as in IEEE arithmetic,
+    and every NaN of the float64 product is NaN here.  An infinity may come
+    out as NaN: its lo part is not finite, and Inf x 0 cross terms are NaN.
+    Finite elements are within the split bound."""
+    A, B, R, S = special_inputs()
+    M, K = A.shape; N = B.shape[1]
+    a = torch.from_numpy(A.ravel()).cuda(); b = torch.from_numpy(B.ravel()).cuda()
+    c = torch.empty(M * N, dtype=torch.float32, device="cuda")
+    hb.gemm_bf16x3(a, b, c, M, N, K)
+    torch.cuda.synchronize()
+    C = c.cpu().numpy().reshape(M, N)
+    assert np.array_equal(np.isfinite(C), np.isfinite(R))
+    assert np.all(np.isnan(C[np.isnan(R)]))
+    fin = np.isfinite(R)
+    assert np.all(np.abs(C[fin] - R[fin]) <= (3.1 * 2.0 ** -16 + gamma(3 * K + 2)) * S[fin])
+
+
 def test_bf16x3_reference_pattern_N512_NITER2_bit_exact():
     N, NITER = 512, 2
     a = dev_empty(NITER * N * N); b = dev_empty(NITER * N * N); c = dev_empty(NITER * N * N)

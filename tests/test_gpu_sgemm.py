@@ -52,6 +52,25 @@ def pattern_closed_form_gpu(N, it):
     return (P.double() @ Q.double()).float()
 
 
+def special_inputs(M=96, N=96, K=96, seed=41):
+    """Uniform inputs with planted NaN and infinities: lone +-Inf and NaN, an
+    Inf times an exact zero (NaN), and +Inf and -Inf products in one sum (NaN);
+    R is the product with IEEE semantics in float64 (elementwise, so no library
+    shortcut can skip a zero)."""
+    rng = np.random.default_rng(seed)
+    A = rng.uniform(-1, 1, (M, K)).astype(np.float32)
+    B = rng.uniform(-1, 1, (K, N)).astype(np.float32)
+    A[3, 5] = np.inf; A[10, 7] = -np.inf; A[20, 1] = np.nan
+    B[2, 30] = np.inf
+    A[60, 40] = np.inf; B[40, 70] = 0.0                        # Inf * 0
+    A[80, 11] = np.inf; A[80, 12] = np.inf; B[11, 90] = 1.0; B[12, 90] = -1.0  # Inf - Inf
+    with np.errstate(invalid="ignore", over="ignore"):
+        R = (A.astype(np.float64)[:, :, None] * B.astype(np.float64)[None, :, :]).sum(axis=1)
+    fa, fb = np.where(np.isfinite(A), np.abs(A), 0.0), np.where(np.isfinite(B), np.abs(B), 0.0)
+    S = fa.astype(np.float64) @ fb.astype(np.float64)
+    return A, B, R, S
+
+
 # --- generators ---------------------------------------------------------------------
 
 def test_fill_pattern_matches_reference_init():
@@ -165,6 +184,25 @@ def test_sgemm_strided_rectangular(M, N, K, pad):
         for r in range(M):
             written[i * sC + r * ldc: i * sC + r * ldc + N] = True
     assert np.all(Ch[~written] == -7.0)  # padding and gaps untouched
+
+
+def test_sgemm_special_values_follow_ieee():
+    """NaN and infinities propagate as in IEEE arithmetic: every NaN, +Inf and
+    -Inf of the float64 product is in the same place, and the finite elements
+    are within the FP32 bound."""
+    A, B, R, S = special_inputs()
+    M, K = A.shape; N = B.shape[1]
+    a = torch.from_numpy(A.ravel()).cuda(); b = torch.from_numpy(B.ravel()).cuda()
+    c = dev_empty(M * N)
+    hb.sgemm_strided(a, b, c, M, N, K, K, N, N)
+    torch.cuda.synchronize()
+    C = c.cpu().numpy().reshape(M, N)
+    assert np.isnan(R).sum() > 100 and np.isposinf(R).any() and np.isneginf(R).any()
+    assert np.array_equal(np.isnan(C), np.isnan(R))
+    assert np.array_equal(np.isposinf(C), np.isposinf(R))
+    assert np.array_equal(np.isneginf(C), np.isneginf(R))
+    fin = np.isfinite(R)
+    assert np.all(np.abs(C[fin] - R[fin]) <= gamma(K + 2) * S[fin])
 
 
 def test_verify_f64_matches_oracle_metrics():
