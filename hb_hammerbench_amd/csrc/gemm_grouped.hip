@@ -59,7 +59,7 @@ struct hbs_grouped_plan {
     int *d_counters;      // split-K counters
     int *d_queue;         // queue head and finished-workgroup count (zero at rest)
     float *d_partials;
-    // large-tile launch (256 x 256 tiles over the 256-aligned cores)
+    // large-tile launch (whole problems in 256 x 256 tiles)
     int nblocks_big;
     int64_t items_big;
     const void *d_items_big;
@@ -266,8 +266,9 @@ __global__ __launch_bounds__(kThreads, 2) void grouped_kernel(const Prob *__rest
 }
 
 // ---------------------------------------------------------------------------
-// Large-tile launch: one 512-thread workgroup per CU walks its list of
-// 256 x 256 tiles with the square kernels' tile code.  A problem on this path
+// Large-tile launch: one 512-thread workgroup per CU takes 256 x 256 tiles
+// from its XCD's queue and runs them with the square kernels' tile code (waves
+// whose sub-tile lies outside M x N skip their MFMAs).  A problem on this path
 // is covered whole: tiles at its ragged bottom and right edges read clamped
 // rows and columns (kTileClamp) and store only what lies inside M x N.  No
 // split-K; K must be a multiple of the 32-deep stage.
@@ -549,7 +550,7 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
     // B panels in that XCD's L2.  Within an XCD the workgroups take tiles in
     // queue order as they finish, so edge tiles that run short (their idle
     // waves skip the MFMAs) and tiles that run long even out.
-    auto xcd_waves = [&](const std::vector<Item> &its, const std::vector<double> &cst, int nb,
+    auto xcd_queues = [&](const std::vector<Item> &its, const std::vector<double> &cst, int nb,
                          std::vector<Item> &sorted, std::vector<int> &qoff, int &nq,
                          double &imbalance) {
         std::vector<int64_t> order(its.size());
@@ -608,7 +609,7 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
     int nq_big = 1;
     double imb = 1.0, imb_big = 1.0;
     lpt(items, cost, nblocks, sorted, imb);
-    xcd_waves(items_big, cost_big, nblocks_big, sorted_big, qoff_big, nq_big, imb_big);
+    xcd_queues(items_big, cost_big, nblocks_big, sorted_big, qoff_big, nq_big, imb_big);
 
     // one device allocation: problems | items | split counters + queues |
     // partials | large-tile items | large-tile queue offsets
