@@ -14,7 +14,11 @@
 // --input uniform|loguniform draws seeded inputs on the device (the pattern is
 // the default); --golden f64 replaces the O(N^3) host golden by the device FP64
 // check (needed at N = 8192, where host_mm takes tens of minutes per slice);
-// --reps times repeated launches with HIP events.
+// --reps times repeated launches with HIP events.  --mode grouped runs the
+// NITER slices as the NITER problems of one grouped plan, replayed from a
+// hipGraph, with its tile shape and split-K from --tile / --split (one choice
+// for every problem) or from an autotuner tuning file (--tuning), else the
+// plan's size heuristic.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -67,13 +71,18 @@ struct Options {
     uint64_t seed = 1;
     int reps = 0;
     int max_print = 16;
+    std::string mode = "square";
+    int tile = -1;  // grouped: HBS_TILE_* for every problem, -1 = tuning file / heuristic
+    int split = 1;
+    std::string tuning;
 };
 
 static void usage(const char *prog) {
     fprintf(stderr,
             "usage: %s [--N n] [--NITER k] [--precision fp32|bf16x3]\n"
             "          [--input pattern|uniform|loguniform] [--seed s]\n"
-            "          [--golden host|f64|auto] [--reps r] [--max-print m]\n",
+            "          [--golden host|f64|auto] [--reps r] [--max-print m]\n"
+            "          [--mode square|grouped] [--tile auto|0..4] [--split s] [--tuning file]\n",
             prog);
 }
 
@@ -90,8 +99,16 @@ static bool parse(int argc, char **argv, Options &o) {
         else if (a == "--seed" && (v = next())) o.seed = strtoull(v, nullptr, 10);
         else if (a == "--reps" && (v = next())) o.reps = atoi(v);
         else if (a == "--max-print" && (v = next())) o.max_print = atoi(v);
+        else if (a == "--mode" && (v = next())) o.mode = v;
+        else if (a == "--tile" && (v = next())) o.tile = strcmp(v, "auto") == 0 ? -1 : atoi(v);
+        else if (a == "--split" && (v = next())) o.split = atoi(v);
+        else if (a == "--tuning" && (v = next())) o.tuning = v;
         else return false;
     }
+    if (o.mode != "square" && o.mode != "grouped") return false;
+    if (o.tile < -1 || o.tile > HBS_TILE_256x256 || o.split < 1 || o.split > 64) return false;
+    if (o.mode == "square" && (o.tile != -1 || o.split != 1 || !o.tuning.empty())) return false;
+    if (o.tile != -1 && !o.tuning.empty()) return false;
     if (o.N < 1 || o.NITER < 1 || o.reps < 0) return false;
     if (o.precision != "fp32" && o.precision != "bf16x3") return false;
     if (o.input != "pattern" && o.input != "uniform" && o.input != "loguniform") return false;
@@ -111,8 +128,8 @@ int main(int argc, char **argv) {
     const int N = o.N, NITER = o.NITER;
     printf("N=%d\n", N);
     printf("NITER=%d\n", NITER);
-    printf("precision=%s input=%s golden=%s\n", o.precision.c_str(), o.input.c_str(),
-           o.golden.c_str());
+    printf("precision=%s input=%s golden=%s mode=%s\n", o.precision.c_str(), o.input.c_str(),
+           o.golden.c_str(), o.mode.c_str());
     const int64_t nn = (int64_t)N * N, total = nn * NITER;
     const size_t bytes = (size_t)total * sizeof(float);
 
@@ -157,12 +174,39 @@ int main(int argc, char **argv) {
             host_mm(&mat_result[nn * i], &mat1[nn * i], &mat2[nn * i], N);
     }
 
+    // Grouped mode: one plan whose problems are the NITER slices, captured in
+    // a hipGraph (built once, outside the timed launches).
+    hbs_grouped_plan *plan = nullptr;
+    void *graph = nullptr;
+    if (o.mode == "grouped") {
+        const int prec = o.precision == "fp32" ? HBS_PREC_FP32 : HBS_PREC_BF16X3;
+        std::vector<hbs_gemm_problem> probs(NITER);
+        for (int i = 0; i < NITER; i++)
+            probs[i] = hbs_gemm_problem{d_mat1 + nn * i, d_mat2 + nn * i, d_result + nn * i, N, N, N,
+                                        N, N, N};
+        if (o.tile >= 0)
+            HBS_CALL(hbs_grouped_plan_create_forced(probs.data(), NITER, prec, o.tile, o.split,
+                                                    &plan));
+        else
+            HBS_CALL(hbs_grouped_plan_create(probs.data(), NITER, prec,
+                                             o.tuning.empty() ? nullptr : o.tuning.c_str(), &plan));
+        hbs_grouped_info info;
+        HBS_CALL(hbs_grouped_plan_info(plan, &info));
+        printf("grouped plan: tile=%s split=%d tuning=%s: %lld tiles, %lld work items, "
+               "%d workgroups\n", o.tile >= 0 ? std::to_string(o.tile).c_str() : "auto", o.split,
+               o.tuning.empty() ? "none" : o.tuning.c_str(), (long long)info.tiles,
+               (long long)info.work_items, info.workgroups);
+        HBS_CALL(hbs_grouped_graph_create(plan, &graph));
+    }
+
     // Launch all NITER slices (main.cpp:86-98).
     auto launch = [&]() -> int {
+        if (graph) return hbs_graph_launch(graph, nullptr);
         if (o.precision == "fp32") return hbs_sgemm(d_mat1, d_mat2, d_result, N, NITER, nullptr);
         return hbs_sgemm_bf16x3(d_mat1, d_mat2, d_result, N, NITER, nullptr);
     };
-    printf("Launching sgemm (%s) over %d slices\n", o.precision.c_str(), NITER);
+    printf("Launching sgemm (%s, %s) over %d slices\n", o.precision.c_str(), o.mode.c_str(),
+           NITER);
     HBS_CALL(launch());
     HIP_CALL(hipDeviceSynchronize());
 
@@ -222,6 +266,8 @@ int main(int argc, char **argv) {
         }
     }
 
+    if (graph) HBS_CALL(hbs_graph_destroy(graph));
+    if (plan) HBS_CALL(hbs_grouped_plan_destroy(plan));
     HIP_CALL(hipFree(d_mat1));
     HIP_CALL(hipFree(d_mat2));
     HIP_CALL(hipFree(d_result));

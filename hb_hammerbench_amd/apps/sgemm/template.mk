@@ -5,6 +5,14 @@ include app_path.mk
 
 SGEMM ?= $(APP_PATH)/sgemm
 ARGS  := --N $(N) --NITER $(NITER) --precision $(PREC) --input $(INPUT)
+ifeq ($(MODE),grouped)
+ARGS  += --mode grouped
+ifneq ($(TILE),auto)
+ARGS  += --tile $(TILE) --split $(SPLIT)
+else ifneq ($(TUNING),none)
+ARGS  += --tuning $(APP_PATH)/../../tuning/$(TUNING)_grouped.txt
+endif
+endif
 
 .PHONY: help
 help:

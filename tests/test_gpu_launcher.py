@@ -24,6 +24,15 @@ pytestmark = pytest.mark.gpu
      "--golden", "f64"],
     ["--N", "2048", "--NITER", "2", "--input", "uniform", "--golden", "f64", "--reps", "3"],
     ["--N", "4096", "--NITER", "1", "--golden", "f64"],               # pattern via FP64 check
+    # the slices as one grouped plan (hipGraph): heuristic, forced tile and
+    # split-K, the committed tuning file
+    ["--N", "512", "--NITER", "2", "--mode", "grouped"],
+    ["--N", "640", "--NITER", "3", "--mode", "grouped", "--precision", "bf16x3", "--tile", "0",
+     "--split", "2"],
+    ["--N", "300", "--NITER", "2", "--mode", "grouped", "--tile", "1", "--split", "3",
+     "--input", "uniform", "--golden", "f64", "--reps", "2"],
+    ["--N", "512", "--NITER", "2", "--mode", "grouped", "--tuning",
+     os.path.join(ROOT, "hb_hammerbench_amd", "tuning", "mi355x_grouped.txt")],
 ])
 def test_launcher_passes(args):
     assert os.path.exists(LAUNCHER), "launcher not built"
@@ -45,3 +54,10 @@ def test_make_generate_and_exec_reference_test(tmp_path):
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     log = (tmp_path / "N_512__NITER_2" / "exec.log").read_text()
     assert "N=512" in log and "NITER=2" in log and "HBS REGRESSION TEST PASSED" in log
+    name = "N_640__NITER_3__MODE_grouped__PREC_bf16x3__TILE_0__SPLIT_2"
+    r = subprocess.run(["make", "-s", "-C", str(tmp_path), name + ".exec"],
+                       capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    log = (tmp_path / name / "exec.log").read_text()
+    assert "mode=grouped" in log and "tile=0 split=2" in log
+    assert "HBS REGRESSION TEST PASSED" in log
