@@ -305,3 +305,44 @@ def test_grouped_512_problem_workload(prec):
         worst = max(worst, st["e_cw"])
     print(f"\n512-problem group prec={prec}: worst e_cw {worst:.3e}", flush=True)
     g.close()
+
+
+@pytest.mark.parametrize("prec", [hb.PREC_FP32, hb.PREC_BF16X3])
+def test_grouped_many_small_problems_through_the_queues(prec):
+    """300 problems of random shapes, a quarter with K not a multiple of 32
+    (so both launches run), some long enough in K for split-K on the
+    256-thread launch: thousands of queue fetches per run.  Two graph replays
+    must agree bit for bit with an eager run (the queues and split counters
+    reset themselves) and every problem is within its FP64 bound."""
+    rng = np.random.default_rng(2024)
+    shapes = []
+    for i in range(300):
+        M = int(rng.integers(1, 200))
+        N = 4 * int(rng.integers(1, 50))
+        K = 4 * int(rng.integers(1, 60)) if i % 4 == 0 else 32 * int(rng.integers(1, 9))
+        if i % 50 == 0:
+            M, N, K = 64, 64, 1028  # thin and long: split-K on the 256-thread launch
+        shapes.append((M, N, K))
+    probs, views, _ = make_group(shapes, "uniform", seed=71)
+    plan = hb.GroupedPlan(probs, prec)
+    info = plan.info()
+    assert info["split_tiles"] > 0 and info["work_items"] > info["tiles"]
+    plan.run()
+    torch.cuda.synchronize()
+    eager = [C.clone() for (_, _, C) in views]
+    g = plan.graph()
+    for rep in range(2):
+        for (_, _, C) in views:
+            C.fill_(float("nan"))
+        g.launch()
+        torch.cuda.synchronize()
+        for (_, _, C), e, shape in zip(views, eager, shapes):
+            assert torch.equal(C, e), (rep, shape)
+    g.close()
+    for (A, B, C), (M, N, K) in zip(views, shapes):
+        R, S = oracle.gemm_f64(A.cpu().numpy(), B.cpu().numpy())
+        if prec == hb.PREC_FP32:
+            bound = gamma(K + 2) * S
+        else:
+            bound = (3.1 * 2.0 ** -16 + gamma(3 * K + 2)) * S
+        assert np.all(np.abs(C.cpu().numpy() - R) <= bound), (M, N, K)
