@@ -18,6 +18,7 @@ __all__ = [
     "fill_pattern", "fill_random", "verify_f64",
     "GEN_UNIFORM", "GEN_LOGUNIFORM", "GEN_NORMAL_OUTLIER", "PREC_FP32", "PREC_BF16X3",
     "GroupedPlan", "Graph", "gemm_fp8res", "fp8res_workspace_size", "gemm_bf16x3",
+    "describe_plan",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -80,6 +81,8 @@ SIGNATURES = {
     "hbs_grouped_plan_create_forced": (_i, [ctypes.POINTER(Problem), _i, _i, _i, _i,
                                             ctypes.POINTER(_vp)]),
     "hbs_grouped_plan_info": (_i, [_vp, ctypes.POINTER(GroupedInfo)]),
+    "hbs_grouped_plan_describe": (_i, [ctypes.POINTER(Problem), _i, _i, ctypes.c_char_p, _i, _i,
+                                       _i, ctypes.POINTER(GroupedInfo)]),
     "hbs_grouped_run": (_i, [_vp, _vp]),
     "hbs_grouped_plan_destroy": (_i, [_vp]),
     "hbs_grouped_graph_create": (_i, [_vp, ctypes.POINTER(_vp)]),
@@ -201,6 +204,23 @@ def verify_f64(A, B, C, M, N, K, lda=None, ldb=None, ldc=None, row_begin=0, row_
                                 _stream(stream)),
            "hbs_verify_f64")
     return out.as_dict()
+
+
+def describe_plan(shapes, precision=PREC_FP32, tuning_file=None, tile=None, split_k=1, cus=256):
+    """Counts of the grouped plan for problems of these (M, N, K) shapes on a GPU
+    of `cus` compute units, without a device (hbs_grouped_plan_describe).
+    Operands are taken as contiguous (lda = K, ldb = N, ldc = N); the pointers
+    are placeholders that are never dereferenced."""
+    arr = (Problem * len(shapes))()
+    fake = 1 << 20  # a 16-byte-aligned placeholder address
+    for i, (M, N, K) in enumerate(shapes):
+        arr[i] = Problem(fake, fake, fake, M, N, K, K, N, N)
+    out = GroupedInfo()
+    path = tuning_file.encode() if tuning_file else None
+    _check(lib().hbs_grouped_plan_describe(arr, len(shapes), precision, path,
+                                           -1 if tile is None else tile, split_k, cus,
+                                           ctypes.byref(out)), "hbs_grouped_plan_describe")
+    return {name: getattr(out, name) for name, _ in GroupedInfo._fields_}
 
 
 class GroupedPlan:

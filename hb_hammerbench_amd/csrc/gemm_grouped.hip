@@ -403,10 +403,7 @@ int validate(const hbs_gemm_problem &q) {
     return HBS_SUCCESS;
 }
 
-int build_plan(const hbs_gemm_problem *problems, int count, int precision, const Choice *forced,
-               const std::map<TuneKey, Choice> *table, hbs_grouped_plan **out) {
-    if (!out) return HBS_ERR_NULL;
-    *out = nullptr;
+int validate_all(const hbs_gemm_problem *problems, int count, int precision) {
     if (count < 1) return HBS_ERR_ARG;
     if (!problems) return HBS_ERR_NULL;
     if (precision != HBS_PREC_FP32 && precision != HBS_PREC_BF16X3) return HBS_ERR_ARG;
@@ -414,11 +411,28 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
         const int v = validate(problems[i]);
         if (v) return v;
     }
-    int st = hbs::check_device();
-    if (st) return st;
-    int dev = 0, cus = 256;
-    HBS_HIP_TRY(hipGetDevice(&dev));
-    HBS_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    return HBS_SUCCESS;
+}
+
+// The host half of a plan: descriptors, work items in queue order and the
+// launch sizes for a GPU of `cus` compute units.  No HIP call.
+struct HostPlan {
+    std::vector<Prob> probs;
+    std::vector<Item> sorted, sorted_big;  // 256-thread list; large-tile queues
+    std::vector<int> qoff_big;
+    int nq_big = 1, nblocks = 0, nblocks_big = 0;
+    int64_t slots = 0, poff = 0, tiles_total = 0;
+    double imb = 1.0, imb_big = 1.0;
+    double est_imbalance() const {
+        return std::max(nblocks > 0 ? imb : 1.0, nblocks_big > 0 ? imb_big : 1.0);
+    }
+};
+
+int plan_host(const hbs_gemm_problem *problems, int count, int precision, const Choice *forced,
+              const std::map<TuneKey, Choice> *table, int cus, HostPlan &hp) {
+    const int v = validate_all(problems, count, precision);
+    if (v) return v;
+    if (cus < 1) return HBS_ERR_ARG;
 
     // Problems on the 256-thread launch, each with its tile shape and split-K.
     struct Small {
@@ -611,6 +625,41 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
     lpt(items, cost, nblocks, sorted, imb);
     xcd_queues(items_big, cost_big, nblocks_big, sorted_big, qoff_big, nq_big, imb_big);
 
+    hp.probs = std::move(probs);
+    hp.sorted = std::move(sorted);
+    hp.sorted_big = std::move(sorted_big);
+    hp.qoff_big = std::move(qoff_big);
+    hp.nq_big = nq_big;
+    hp.nblocks = nblocks;
+    hp.nblocks_big = nblocks_big;
+    hp.slots = slots;
+    hp.poff = poff;
+    hp.tiles_total = tiles_total;
+    hp.imb = imb;
+    hp.imb_big = imb_big;
+    return HBS_SUCCESS;
+}
+
+int build_plan(const hbs_gemm_problem *problems, int count, int precision, const Choice *forced,
+               const std::map<TuneKey, Choice> *table, hbs_grouped_plan **out) {
+    if (!out) return HBS_ERR_NULL;
+    *out = nullptr;
+    int st = validate_all(problems, count, precision);
+    if (st) return st;
+    st = hbs::check_device();
+    if (st) return st;
+    int dev = 0, cus = 256;
+    HBS_HIP_TRY(hipGetDevice(&dev));
+    HBS_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    HostPlan hp;
+    st = plan_host(problems, count, precision, forced, table, cus, hp);
+    if (st) return st;
+    const std::vector<Prob> &probs = hp.probs;
+    const std::vector<Item> &sorted = hp.sorted, &sorted_big = hp.sorted_big;
+    const std::vector<int> &qoff_big = hp.qoff_big;
+    const int nblocks = hp.nblocks, nblocks_big = hp.nblocks_big;
+    const int64_t slots = hp.slots, poff = hp.poff, tiles_total = hp.tiles_total;
+
     // one device allocation: problems | items | split counters + queues |
     // partials | large-tile items | large-tile queue offsets
     auto align = [](size_t x) { return (x + 255) / 256 * 256; };
@@ -635,7 +684,7 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
     plan->items = (int64_t)sorted.size();
     plan->tiles = tiles_total;
     plan->split_tiles = slots;
-    plan->est_imbalance = std::max(nblocks > 0 ? imb : 1.0, nblocks_big > 0 ? imb_big : 1.0);
+    plan->est_imbalance = hp.est_imbalance();
     plan->dmem = dmem;
     plan->d_probs = dmem;
     plan->d_items = dmem + o_items;
@@ -647,7 +696,7 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
     plan->items_big = (int64_t)sorted_big.size();
     plan->d_items_big = dmem + o_items_big;
     plan->d_qoff_big = (const int *)(dmem + o_qoff_big);
-    plan->nq_big = nq_big;
+    plan->nq_big = hp.nq_big;
     hipError_t e = hipMemcpy(dmem, probs.data(), probs.size() * sizeof(Prob), hipMemcpyHostToDevice);
     if (e == hipSuccess && !sorted.empty())
         e = hipMemcpy(dmem + o_items, sorted.data(), sorted.size() * sizeof(Item),
@@ -691,6 +740,32 @@ extern "C" int hbs_grouped_plan_create_forced(const hbs_gemm_problem *problems, 
     }
     const Choice c{tile, split_k, 0.0};
     return build_plan(problems, count, precision, &c, nullptr, plan);
+}
+
+extern "C" int hbs_grouped_plan_describe(const hbs_gemm_problem *problems, int count,
+                                         int precision, const char *tuning_file, int tile,
+                                         int split_k, int cus, hbs_grouped_info *info) {
+    if (!info) return HBS_ERR_NULL;
+    const bool tuned = tuning_file && tuning_file[0];
+    std::map<TuneKey, Choice> table;
+    const Choice c{tile, split_k, 0.0};
+    if (tile >= 0) {
+        if (tuned || !tile_supported(precision, tile) || split_k < 1 || split_k > 64)
+            return HBS_ERR_ARG;
+    } else if (tile != -1 || (tuned && !load_tuning(tuning_file, table))) {
+        return HBS_ERR_ARG;
+    }
+    HostPlan hp;
+    const int st = plan_host(problems, count, precision, tile >= 0 ? &c : nullptr,
+                             tuned ? &table : nullptr, cus, hp);
+    if (st) return st;
+    info->problems = count;
+    info->workgroups = hp.nblocks + hp.nblocks_big;
+    info->tiles = hp.tiles_total;
+    info->work_items = (int64_t)(hp.sorted.size() + hp.sorted_big.size());
+    info->split_tiles = hp.slots;
+    info->est_imbalance = hp.est_imbalance();
+    return HBS_SUCCESS;
 }
 
 extern "C" int hbs_grouped_plan_info(const hbs_grouped_plan *plan, hbs_grouped_info *info) {

@@ -140,6 +140,18 @@ int hbs_grouped_plan_create(const hbs_gemm_problem *problems, int count, int pre
 int hbs_grouped_plan_create_forced(const hbs_gemm_problem *problems, int count, int precision,
                                    int tile, int split_k, hbs_grouped_plan **plan);
 int hbs_grouped_plan_info(const hbs_grouped_plan *plan, hbs_grouped_info *info);
+/* Dry run of plan creation, no reference counterpart: fills `info` with the
+ * counts a plan for these problems would have on a GPU of `cus` compute units,
+ * without a device, device memory or any HIP call (capacity planning, and
+ * tests of the planner on machines without a GPU).  tile = -1 takes the
+ * tuning file when `tuning_file` is non-NULL, else the size heuristic, as
+ * hbs_grouped_plan_create does; tile >= 0 forces tile and split_k as
+ * hbs_grouped_plan_create_forced does (then `tuning_file` must be NULL).  The
+ * problems are validated as for a real plan; their pointers are not
+ * dereferenced. */
+int hbs_grouped_plan_describe(const hbs_gemm_problem *problems, int count, int precision,
+                              const char *tuning_file, int tile, int split_k, int cus,
+                              hbs_grouped_info *info);
 /* Computes every problem of the plan on `stream`: the large-tile launch (when
  * the plan has 256 x 256 tiles), then the 256-thread persistent launch (when
  * it has smaller tiles).  Workgroups take their items from queues in plan

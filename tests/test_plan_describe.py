@@ -1,0 +1,112 @@
+"""The grouped planner on the host (no GPU): hbs_grouped_plan_describe runs the
+same planning as hbs_grouped_plan_create -- size heuristic, tuning file or
+forced tile / split-K, work-item expansion, launch sizes -- without a device.
+Expected counts follow from the planning rules:
+
+  * a problem whose K is a multiple of 32 takes the large-tile launch (by
+    default and with tile 4): ceil(M/256) ceil(N/256) tiles, no split-K;
+  * any other problem takes the 256-thread launch with the heuristic tile
+    (64 x 64 when M, N <= 64, 64 x 128 when M <= 64, 128 x 64 when N <= 64,
+    else 128 x 128) and split-K 4 (2) when it has at most 4 (16) tiles and
+    K >= 1024; a split K range is ceil(K / s) rounded up to the stage depth
+    (16 for FP32, 32 for BF16x3), giving ceil(K / range) work items per tile;
+  * workgroups: one per CU on the large-tile launch, two per CU on the other,
+    never more than the launch's work items.
+"""
+import math
+
+import pytest
+
+import hb_hammerbench_amd as hb
+
+TILE = {0: (128, 128), 1: (64, 64), 2: (128, 64), 3: (64, 128)}
+
+
+def heuristic_tile(M, N):
+    if M <= 64 and N <= 64:
+        return 1
+    if M <= 64:
+        return 3
+    if N <= 64:
+        return 2
+    return 0
+
+
+def expected(shapes, prec, cus=256, tile=None, split=1):
+    stage = 16 if prec == hb.PREC_FP32 else 32
+    big_tiles = small_tiles = small_items = split_tiles = 0
+    for M, N, K in shapes:
+        big = K % 32 == 0 and (tile is None or tile == 4)
+        if big:
+            big_tiles += math.ceil(M / 256) * math.ceil(N / 256)
+            continue
+        t = heuristic_tile(M, N) if tile in (None, 4) else tile
+        BM, BN = TILE[t]
+        n = math.ceil(M / BM) * math.ceil(N / BN)
+        if tile is None:
+            s = 4 if (n <= 4 and K >= 1024) else 2 if (n <= 16 and K >= 1024) else 1
+        else:
+            s = split
+        chunk = math.ceil(math.ceil(K / s) / stage) * stage
+        ns = math.ceil(K / chunk)
+        small_tiles += n
+        small_items += n * ns
+        split_tiles += n if ns > 1 else 0
+    return {"tiles": big_tiles + small_tiles, "work_items": big_tiles + small_items,
+            "split_tiles": split_tiles,
+            "workgroups": min(cus, big_tiles) + min(2 * cus, small_items)}
+
+
+MIXED = [(1000, 1028, 768), (64, 2048, 256), (300, 260, 256), (200, 68, 100),
+         (64, 64, 2052), (512, 512, 100), (130, 132, 132), (1, 4, 4), (64, 520, 4000)]
+
+
+def check(info, want, count):
+    for k, v in want.items():
+        assert info[k] == v, (k, info[k], v)
+    assert info["problems"] == count and info["est_imbalance"] >= 1.0
+
+
+@pytest.mark.parametrize("prec", [hb.PREC_FP32, hb.PREC_BF16X3])
+@pytest.mark.parametrize("cus", [256, 8])
+def test_describe_size_heuristic(prec, cus):
+    info = hb.describe_plan(MIXED, prec, cus=cus)
+    want = expected(MIXED, prec, cus)
+    assert want["split_tiles"] > 0 and want["tiles"] > 20
+    check(info, want, len(MIXED))
+
+
+@pytest.mark.parametrize("prec", [hb.PREC_FP32, hb.PREC_BF16X3])
+@pytest.mark.parametrize("tile,split", [(0, 1), (1, 3), (2, 2), (3, 5), (4, 1), (4, 2)])
+def test_describe_forced_tiles(prec, tile, split):
+    info = hb.describe_plan(MIXED, prec, tile=tile, split_k=split)
+    check(info, expected(MIXED, prec, tile=tile, split=split), len(MIXED))
+
+
+def test_describe_tuning_file_overrides_the_default(tmp_path):
+    shapes = [(64, 64, 2048)] * 3 + [(512, 512, 512)]
+    assert hb.describe_plan(shapes)["tiles"] == 3 + 4  # all on 256 x 256 tiles
+    path = tmp_path / "tune.txt"
+    path.write_text("# test\nfp32 64 64 2048 1 4 1000.0\n")
+    info = hb.describe_plan(shapes, tuning_file=str(path))
+    assert info["tiles"] == 3 + 4 and info["work_items"] == 3 * 4 + 4
+    assert info["split_tiles"] == 3 and info["workgroups"] == 4 + 12
+    # a class the table does not name keeps the default; BF16x3 ignores fp32 lines
+    info = hb.describe_plan(shapes, hb.PREC_BF16X3, tuning_file=str(path))
+    assert info["work_items"] == 3 + 4 and info["split_tiles"] == 0
+
+
+def test_describe_rejects_bad_arguments(tmp_path):
+    def code(*a, **k):
+        with pytest.raises(hb.HbsError) as e:
+            hb.describe_plan(*a, **k)
+        return e.value.code
+    assert code([]) == -1                                   # no problems
+    assert code([(64, 62, 64)]) == -3                      # N not a multiple of 4
+    assert code([(0, 64, 64)]) == -1
+    assert code([(64, 64, 64)], precision=7) == -1
+    assert code([(64, 64, 64)], tile=5) == -1
+    assert code([(64, 64, 64)], tile=0, split_k=0) == -1
+    assert code([(64, 64, 64)], tile=0, tuning_file=str(tmp_path / "t.txt")) == -1
+    assert code([(64, 64, 64)], tuning_file=str(tmp_path / "missing.txt")) == -1
+    assert code([(64, 64, 64)], cus=0) == -1
