@@ -39,7 +39,7 @@ def test_python_binding_covers_every_declared_symbol():
 
 def test_strerror_and_version():
     L = hb.lib()
-    assert L.hbs_version() == 200  # 0.2.0: BF16x3 entries take no workspace
+    assert L.hbs_version() == 300  # 0.3.0: hbs_grouped_plan_describe
     for code in [0, -1, -2, -3, -4, -5, -6]:
         assert L.hbs_strerror(code)
     assert L.hbs_strerror(-99) == b"unknown status code"
