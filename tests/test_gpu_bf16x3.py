@@ -55,6 +55,38 @@ def special_inputs(M=96, N=96, K=96, seed=41):
     return A, B, R, S
 
 
+def subnormal_inputs(N=256, seed=5):
+    """Uniform values with 4000 entries replaced by subnormal and tiny normal
+    magnitudes (down to the smallest subnormal, 1.4e-45), whose bf16 lo parts
+    are subnormal too."""
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-1, 1, (N, N)).astype(np.float32)
+    mags = np.array([1e-39, 2.5e-40, 7e-45, 1.4e-45, 1.17549435e-38, 3e-38, 1e-37, 5e-37, 2e-36],
+                    np.float32)
+    idx = rng.integers(0, N * N, 4000)
+    sign = np.sign(rng.uniform(-1, 1, 4000)).astype(np.float32)
+    x.reshape(-1)[idx] = mags[rng.integers(0, len(mags), 4000)] * sign
+    return x
+
+
+def test_bf16x3_split_keeps_subnormals():
+    """With B = I, C is the oracle's hi + lo of A bit for bit, also where A or
+    its lo part is subnormal: neither the hardware bf16 conversion nor the
+    BF16 matrix cores flush them."""
+    x = subnormal_inputs()
+    N = x.shape[0]
+    xs = torch.from_numpy(x.ravel()).cuda()
+    eye = torch.eye(N, dtype=torch.float32, device="cuda").reshape(-1).contiguous()
+    c = torch.empty(N * N, dtype=torch.float32, device="cuda")
+    hb.sgemm_bf16x3(xs, eye, c, N, 1)
+    torch.cuda.synchronize()
+    hi, lo = oracle.split_bf16(x)
+    lo_f = oracle.bf16_bits_to_f32(lo)
+    assert ((np.abs(lo_f) < 2.0 ** -126) & (lo_f != 0)).sum() > 100
+    want = oracle.bf16_bits_to_f32(hi) + lo_f
+    assert np.array_equal(c.cpu().numpy().reshape(N, N), want)
+
+
 def test_bf16x3_special_values_stay_non_finite():
     """A non-finite input makes its results non-finite, as in IEEE arithmetic,
     and every NaN of the float64 product is NaN here.  An infinity may come

@@ -71,6 +71,20 @@ def special_inputs(M=96, N=96, K=96, seed=41):
     return A, B, R, S
 
 
+def subnormal_inputs(N=256, seed=5):
+    """Uniform values with 4000 entries replaced by subnormal and tiny normal
+    magnitudes (down to the smallest subnormal, 1.4e-45), whose bf16 lo parts
+    are subnormal too."""
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-1, 1, (N, N)).astype(np.float32)
+    mags = np.array([1e-39, 2.5e-40, 7e-45, 1.4e-45, 1.17549435e-38, 3e-38, 1e-37, 5e-37, 2e-36],
+                    np.float32)
+    idx = rng.integers(0, N * N, 4000)
+    sign = np.sign(rng.uniform(-1, 1, 4000)).astype(np.float32)
+    x.reshape(-1)[idx] = mags[rng.integers(0, len(mags), 4000)] * sign
+    return x
+
+
 # --- generators ---------------------------------------------------------------------
 
 def test_fill_pattern_matches_reference_init():
@@ -203,6 +217,20 @@ def test_sgemm_special_values_follow_ieee():
     assert np.array_equal(np.isneginf(C), np.isneginf(R))
     fin = np.isfinite(R)
     assert np.all(np.abs(C[fin] - R[fin]) <= gamma(K + 2) * S[fin])
+
+
+def test_sgemm_keeps_subnormals():
+    """C = X . I returns X bit for bit, subnormal entries included: the FP32
+    matrix cores neither flush subnormal inputs nor subnormal products."""
+    x = subnormal_inputs()
+    N = x.shape[0]
+    assert ((np.abs(x) < 2.0 ** -126) & (x != 0)).sum() > 1000
+    xs = torch.from_numpy(x.ravel()).cuda()
+    eye = torch.eye(N, dtype=torch.float32, device="cuda").reshape(-1).contiguous()
+    c = dev_empty(N * N)
+    hb.sgemm(xs, eye, c, N, 1)
+    torch.cuda.synchronize()
+    assert np.array_equal(c.cpu().numpy().reshape(N, N), x)
 
 
 def test_verify_f64_matches_oracle_metrics():
