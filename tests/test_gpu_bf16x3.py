@@ -87,6 +87,26 @@ def test_bf16x3_split_keeps_subnormals():
     assert np.array_equal(c.cpu().numpy().reshape(N, N), want)
 
 
+def test_bf16x3_bf16_overflow_threshold():
+    """Finite inputs below the bf16 overflow threshold (2^128 - 2^119, about
+    3.3962e38) come through finite; at or above it, bf16 rounding of x is an
+    infinity and the result is NaN (one value per row, B = I)."""
+    vals = np.array([3.38e38, 3.3895e38, 3.395e38, 3.3962e38, 3.4e38, np.finfo(np.float32).max,
+                     -3.4e38, 1.0], np.float32)
+    N = 64
+    x = np.zeros((N, N), np.float32)
+    x[np.arange(len(vals)), 0] = vals
+    eye = torch.eye(N, dtype=torch.float32, device="cuda").reshape(-1).contiguous()
+    c = torch.empty(N * N, dtype=torch.float32, device="cuda")
+    hb.sgemm_bf16x3(torch.from_numpy(x.ravel()).cuda(), eye, c, N, 1)
+    torch.cuda.synchronize()
+    got = c.cpu().numpy().reshape(N, N)[np.arange(len(vals)), 0]
+    below = np.abs(vals) < 2.0 ** 128 - 2.0 ** 119
+    assert np.all(np.isfinite(got[below]))
+    assert np.all(np.abs(got[below] - vals[below]) <= 2.0 ** -15 * np.abs(vals[below]))
+    assert np.all(np.isnan(got[~below]))
+
+
 def test_bf16x3_special_values_stay_non_finite():
     """A non-finite input makes its results non-finite, as in IEEE arithmetic,
     and every NaN of the float64 product is NaN here.  An infinity may come
