@@ -7,6 +7,8 @@
 // dropped.  The split happens in registers on the way to LDS.
 #pragma once
 
+#include <type_traits>
+
 #include "hbs_internal.h"
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -99,17 +101,18 @@ __device__ __forceinline__ void bf16x3_block_mma_impl(uint16_t *smem, const floa
 
     float ra[kItemsA][8], rb[kItemsB][8];
 
-    auto gload = [&](int k0) {
+    auto gload_m = [&](auto mode, int k0) {
+        constexpr int LM = decltype(mode)::value;
 #pragma unroll
         for (int i = 0; i < kItemsA; ++i) {
             const int f = tid + i * kThreads;
             const int row = f / kGroups, gk = k0 + (f % kGroups) * 8;
-            const int srow = MODE >= kTileClamp ? min(row, M - m0 - 1) : row;
+            const int srow = LM >= kTileClamp ? min(row, M - m0 - 1) : row;
             const float *x = Ag + (int64_t)srow * lda + gk;
-            if (MODE != kTileEdge) {
+            if (LM != kTileEdge) {
                 const floatx4 zero = {0, 0, 0, 0};
-                const bool k0ok = MODE != kTileClampK || gk < K;
-                const bool k1ok = MODE != kTileClampK || gk + 4 < K;
+                const bool k0ok = LM != kTileClampK || gk < K;
+                const bool k1ok = LM != kTileClampK || gk + 4 < K;
                 const floatx4 v0 = k0ok ? *(const floatx4 *)x : zero;
                 const floatx4 v1 = k1ok ? *(const floatx4 *)(x + 4) : zero;
 #pragma unroll
@@ -127,19 +130,27 @@ __device__ __forceinline__ void bf16x3_block_mma_impl(uint16_t *smem, const floa
         for (int i = 0; i < kItemsB; ++i) {
             const int f = tid + i * kThreads;
             const unsigned col =
-                MODE >= kTileClamp ? (unsigned)min(f % kBN, N - n0 - 1) : (unsigned)(f % kBN);
+                LM >= kTileClamp ? (unsigned)min(f % kBN, N - n0 - 1) : (unsigned)(f % kBN);
             // the k group is the same for the whole wave (kBN is a multiple of 64)
             // and each k row pointer is uniform: scalar base + 32-bit lane offset
             const int gk = k0 + __builtin_amdgcn_readfirstlane(f / kBN) * 8;
-            const bool cok = MODE != kTileEdge || n0 + (int)col < N;
+            const bool cok = LM != kTileEdge || n0 + (int)col < N;
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 const float *rowp = Bg + (int64_t)(gk + j) * ldb;
-                const bool ok = MODE == kTileInterior || MODE == kTileClamp ||
-                                (MODE == kTileClampK ? gk + j < K : (cok && gk + j < K));
+                const bool ok = LM == kTileInterior || LM == kTileClamp ||
+                                (LM == kTileClampK ? gk + j < K : (cok && gk + j < K));
                 rb[i][j] = ok ? rowp[col] : 0.0f;
             }
         }
+    };
+    // With kTileClampK only the stage that runs past K needs K predicates;
+    // every full stage takes the clamped loads (the choice is uniform).
+    auto gload = [&](int k0) {
+        if (MODE == kTileClampK && k0 + kBK > K)
+            gload_m(std::integral_constant<int, kTileClampK>{}, k0);
+        else
+            gload_m(std::integral_constant<int, MODE == kTileClampK ? kTileClamp : MODE>{}, k0);
     };
     auto sstore = [&](int buf) {
         uint16_t *as = smem + buf * kStage;

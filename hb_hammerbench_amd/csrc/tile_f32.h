@@ -12,6 +12,8 @@
 //     (lane half h takes k = 4h..4h+3 of each 8-wide k chunk).
 #pragma once
 
+#include <type_traits>
+
 #include "hbs_internal.h"
 
 namespace hbs {
@@ -81,15 +83,16 @@ __device__ __forceinline__ void f32_block_mma_impl(
 
     floatx4 ra[G::kAF4], rb[G::kBF4];
 
-    auto gload = [&](int k0) {
+    auto gload_m = [&](auto mode, int k0) {
+        constexpr int LM = decltype(mode)::value;
 #pragma unroll
         for (int i = 0; i < G::kAF4; ++i) {
             const int f = tid + i * G::kThreads;
             const int row = f / (BK / 4), c4 = f % (BK / 4);
-            if constexpr (MODE != kTileEdge) {
-                const int srow = MODE >= kTileClamp ? min(row, M - m0 - 1) : row;
+            if constexpr (LM != kTileEdge) {
+                const int srow = LM >= kTileClamp ? min(row, M - m0 - 1) : row;
                 const floatx4 zero = {0, 0, 0, 0};
-                ra[i] = (MODE != kTileClampK || k0 + c4 * 4 < K)
+                ra[i] = (LM != kTileClampK || k0 + c4 * 4 < K)
                             ? *(const floatx4 *)(Ag + (int64_t)srow * lda + k0 + c4 * 4)
                             : zero;
             } else {
@@ -104,10 +107,10 @@ __device__ __forceinline__ void f32_block_mma_impl(
         for (int i = 0; i < G::kBF4; ++i) {
             const int f = tid + i * G::kThreads;
             const int kr = f / (BN / 4), c4 = f % (BN / 4);
-            if constexpr (MODE != kTileEdge) {
-                const int sc = MODE >= kTileClamp ? min(c4 * 4, N - n0 - 4) : c4 * 4;
+            if constexpr (LM != kTileEdge) {
+                const int sc = LM >= kTileClamp ? min(c4 * 4, N - n0 - 4) : c4 * 4;
                 const floatx4 zero = {0, 0, 0, 0};
-                rb[i] = (MODE != kTileClampK || k0 + kr < K)
+                rb[i] = (LM != kTileClampK || k0 + kr < K)
                             ? *(const floatx4 *)(Bg + (int64_t)(k0 + kr) * ldb + sc)
                             : zero;
             } else {
@@ -119,6 +122,14 @@ __device__ __forceinline__ void f32_block_mma_impl(
                 }
             }
         }
+    };
+    // With kTileClampK only the stage that runs past K needs K predicates;
+    // every full stage takes the clamped loads (the choice is uniform).
+    auto gload = [&](int k0) {
+        if (MODE == kTileClampK && k0 + BK > K)
+            gload_m(std::integral_constant<int, kTileClampK>{}, k0);
+        else
+            gload_m(std::integral_constant<int, MODE == kTileClampK ? kTileClamp : MODE>{}, k0);
     };
 
     auto sstore = [&](int buf) {
