@@ -9,7 +9,8 @@
 //
 // One launch, no workspace: each workgroup computes one output tile
 // (tile_bf16x3.h) from float32 tiles of A and B split in registers; 256 x 256
-// tile, BK = 32, LDS double buffer, 8 waves each 64 x 128.
+// tile, BK = 32, LDS double buffer, 8 waves each 64 x 128 (grids under 256
+// such tiles take 128 x 128 or 64 x 64 tiles of 2 x 2 waves).
 #include "tile_bf16x3.h"
 
 namespace {
@@ -60,6 +61,22 @@ int launch_gemm(const float *A, const float *B, float *C, int M, int N, int K, i
     return hbs::launch_status();
 }
 
+// Largest tile whose grid still covers the 256 CUs: the 256 x 256 production
+// tile, else 128 x 128 or 64 x 64 tiles of 2 x 2 waves (the same main loop).
+int run_bf16x3(const float *A, const float *B, float *C, int M, int N, int K, int lda, int ldb,
+               int ldc, int64_t sA, int64_t sB, int64_t sC, int batch, hipStream_t stream) {
+    const int64_t big_tiles = (int64_t)((M + 255) / 256) * ((N + 255) / 256) * batch;
+    if (big_tiles >= 256)
+        return launch_gemm<hbs::B3Main>(A, B, C, M, N, K, lda, ldb, ldc, sA, sB, sC, batch, 8,
+                                        stream);
+    const int64_t mid_tiles = (int64_t)((M + 127) / 128) * ((N + 127) / 128) * batch;
+    if (mid_tiles >= 128)
+        return launch_gemm<hbs::B3Cfg<128, 128, 32, 2, 2>>(A, B, C, M, N, K, lda, ldb, ldc, sA,
+                                                          sB, sC, batch, 8, stream);
+    return launch_gemm<hbs::B3Cfg<64, 64, 32, 2, 2>>(A, B, C, M, N, K, lda, ldb, ldc, sA, sB, sC,
+                                                    batch, 8, stream);
+}
+
 }  // namespace
 
 extern "C" int hbs_gemm_bf16x3(const float *A, const float *B, float *C, int M, int N, int K,
@@ -68,8 +85,7 @@ extern "C" int hbs_gemm_bf16x3(const float *A, const float *B, float *C, int M, 
     if (!A || !B || !C) return HBS_ERR_NULL;
     const int st = hbs::check_device();
     if (st) return st;
-    return launch_gemm<hbs::B3Main>(A, B, C, M, N, K, lda, ldb, ldc, 0, 0, 0, 1, 8,
-                                    (hipStream_t)stream);
+    return run_bf16x3(A, B, C, M, N, K, lda, ldb, ldc, 0, 0, 0, 1, (hipStream_t)stream);
 }
 
 extern "C" int hbs_sgemm_bf16x3(const float *A, const float *B, float *C, int N, int niter,
@@ -80,6 +96,5 @@ extern "C" int hbs_sgemm_bf16x3(const float *A, const float *B, float *C, int N,
     if (!A || !B || !C) return HBS_ERR_NULL;
     const int st = hbs::check_device();
     if (st) return st;
-    return launch_gemm<hbs::B3Main>(A, B, C, N, N, N, N, N, N, nn, nn, nn, niter, 8,
-                                    (hipStream_t)stream);
+    return run_bf16x3(A, B, C, N, N, N, N, N, N, nn, nn, nn, niter, (hipStream_t)stream);
 }

@@ -155,12 +155,14 @@ def test_bf16x3_within_split_bound(N, niter, kind):
         assert e["e_cw"] <= 1e-5 * np.sqrt(N)
 
 
-@pytest.mark.parametrize("N", [300, 512])
+@pytest.mark.parametrize("N", [300, 512, 1536])
 def test_bf16x3_identity_reproduces_oracle_split(N):
     """With B = I (and with A = I) each output is lo + 0 + hi of the other
     operand's element, and hi + lo is exact in FP32, so C equals the oracle's
     split summed in float32 bit for bit: this pins the in-register split of
-    the A path and of the B path (N = 300: ragged tiles; 512: interior)."""
+    the A path and of the B path (N = 300: ragged 64 x 64 tiles; 512: interior
+    64 x 64 tiles; 1536: 128 x 128 tiles, the grid being under 256 tiles of
+    256 x 256)."""
     x = dev_fill(N * N, hb.GEN_LOGUNIFORM, 41)
     eye = torch.eye(N, dtype=torch.float32, device="cuda").reshape(-1).contiguous()
     c1 = dev_empty(N * N); c2 = dev_empty(N * N)
