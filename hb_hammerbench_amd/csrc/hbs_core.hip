@@ -22,6 +22,22 @@ int check_device() {
     return state[dev] == 1 ? HBS_SUCCESS : HBS_ERR_DEVICE;
 }
 
+int cu_count() {
+    static std::mutex mu;
+    static int cus[64];  // 0 unknown
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    std::lock_guard<std::mutex> lk(mu);
+    if (cus[dev] == 0) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            n < 1)
+            n = 256;
+        cus[dev] = n;
+    }
+    return cus[dev];
+}
+
 }  // namespace hbs
 
 extern "C" const char *hbs_strerror(int code) {

@@ -32,6 +32,19 @@ enum TileMode { kTileInterior = 0, kTileEdge = 1, kTileClamp = 2, kTileClampK = 
 // Checks that the current device is a gfx950 GPU (cached per device).
 int check_device();
 
+// Compute units of the current device (cached per device; 256 on MI355X).
+int cu_count();
+
+// Whether a square or rectangular GEMM should take 256 x 256 tiles (one
+// workgroup per CU) rather than 128 x 128 tiles (two per CU): the one that
+// finishes in less time counted in rounds of resident workgroups, a round of
+// 128 x 128 tiles costing `mid_round` of a round of 256 x 256 tiles.
+inline bool prefer_big_tiles(int64_t big_tiles, int64_t mid_tiles, int cus, double mid_round) {
+    const int64_t big_rounds = (big_tiles + cus - 1) / cus;
+    const int64_t mid_rounds = (mid_tiles + 2 * cus - 1) / (2 * cus);
+    return (double)big_rounds <= mid_round * (double)mid_rounds;
+}
+
 // Returns the launch error of the last kernel launch as a status code.
 inline int launch_status() {
     hipError_t e = hipGetLastError();

@@ -61,15 +61,17 @@ int launch_gemm(const float *A, const float *B, float *C, int M, int N, int K, i
     return hbs::launch_status();
 }
 
-// Largest tile whose grid still covers the 256 CUs: the 256 x 256 production
-// tile, else 128 x 128 or 64 x 64 tiles of 2 x 2 waves (the same main loop).
+// The 256 x 256 production tile unless 128 x 128 tiles of 2 x 2 waves (the
+// same main loop) finish in fewer weighted rounds (a round of them measured at
+// 0.66 of a 256 x 256 round, N = 4096); grids of under 128 such tiles take
+// 64 x 64 tiles.
 int run_bf16x3(const float *A, const float *B, float *C, int M, int N, int K, int lda, int ldb,
                int ldc, int64_t sA, int64_t sB, int64_t sC, int batch, hipStream_t stream) {
     const int64_t big_tiles = (int64_t)((M + 255) / 256) * ((N + 255) / 256) * batch;
-    if (big_tiles >= 256)
+    const int64_t mid_tiles = (int64_t)((M + 127) / 128) * ((N + 127) / 128) * batch;
+    if (hbs::prefer_big_tiles(big_tiles, mid_tiles, hbs::cu_count(), 0.66))
         return launch_gemm<hbs::B3Main>(A, B, C, M, N, K, lda, ldb, ldc, sA, sB, sC, batch, 8,
                                         stream);
-    const int64_t mid_tiles = (int64_t)((M + 127) / 128) * ((N + 127) / 128) * batch;
     if (mid_tiles >= 128)
         return launch_gemm<hbs::B3Cfg<128, 128, 32, 2, 2>>(A, B, C, M, N, K, lda, ldb, ldc, sA,
                                                           sB, sC, batch, 8, stream);

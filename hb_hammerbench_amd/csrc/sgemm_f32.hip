@@ -61,10 +61,14 @@ int launch_cfg(GemmArgs p, bool vec_ok, hipStream_t stream) {
 int run_f32(GemmArgs p, hipStream_t stream) {
     const bool vec_ok = hbs::aligned16(p.A) && hbs::aligned16(p.B) && (p.lda % 4 == 0) &&
                         (p.ldb % 4 == 0) && (p.sA % 4 == 0) && (p.sB % 4 == 0);
-    // Largest tile whose grid still covers the 256 CUs.
+    // 256 x 256 tiles unless 128 x 128 tiles finish in fewer weighted rounds
+    // (a round of them measured at 0.52 of a 256 x 256 round, N = 4096); grids
+    // of under 128 such tiles take 64 x 64 tiles.
+    const int cus = hbs::cu_count();
     const int64_t big_tiles = (int64_t)((p.M + 255) / 256) * ((p.N + 255) / 256) * p.batch;
-    if (big_tiles >= 256) return launch_cfg<256, 256, 32, 4, 2>(p, vec_ok, stream);
     const int64_t mid_tiles = (int64_t)((p.M + 127) / 128) * ((p.N + 127) / 128) * p.batch;
+    if (hbs::prefer_big_tiles(big_tiles, mid_tiles, cus, 0.52))
+        return launch_cfg<256, 256, 32, 4, 2>(p, vec_ok, stream);
     if (mid_tiles >= 128) return launch_cfg<128, 128, 16, 2, 2>(p, vec_ok, stream);
     return launch_cfg<64, 64, 16, 2, 2>(p, vec_ok, stream);
 }
