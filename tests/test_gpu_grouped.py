@@ -346,3 +346,32 @@ def test_grouped_many_small_problems_through_the_queues(prec):
         else:
             bound = (3.1 * 2.0 ** -16 + gamma(3 * K + 2)) * S
         assert np.all(np.abs(C.cpu().numpy() - R) <= bound), (M, N, K)
+
+
+@pytest.mark.parametrize("prec", [hb.PREC_FP32, hb.PREC_BF16X3])
+def test_grouped_split_64_long_k(prec):
+    """The long-K route for a product with few output tiles: one problem,
+    forced 128 x 128 tiles with split-K 64 (4 tiles x 64 K ranges).  Within
+    the FP64 bound and identical between an eager run and a graph replay."""
+    shapes = [(256, 256, 16384)]
+    probs, views, _ = make_group(shapes, "uniform", seed=81)
+    plan = hb.GroupedPlan(probs, prec, tile=hb.TILE_128x128, split_k=64)
+    info = plan.info()
+    assert info["tiles"] == 4 and info["work_items"] == 4 * 64 and info["split_tiles"] == 4
+    plan.run()
+    torch.cuda.synchronize()
+    (A, B, C), = views
+    eager = C.clone()
+    C.fill_(float("nan"))
+    g = plan.graph()
+    g.launch()
+    torch.cuda.synchronize()
+    assert torch.equal(C, eager)
+    g.close()
+    M, N, K = shapes[0]
+    R, S = oracle.gemm_f64(A.cpu().numpy(), B.cpu().numpy())
+    if prec == hb.PREC_FP32:
+        bound = gamma(K + 2) * S
+    else:
+        bound = (3.1 * 2.0 ** -16 + gamma(3 * K + 2)) * S
+    assert np.all(np.abs(C.cpu().numpy() - R) <= bound)
