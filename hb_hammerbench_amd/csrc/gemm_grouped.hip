@@ -150,7 +150,7 @@ __device__ __forceinline__ void finish_tile(floatx16 (&acc)[BM / 64][BN / 64], c
 #pragma unroll
             for (int q = 0; q < 16; ++q) {
                 const int dr = 8 * (q >> 2) + (q & 3);
-                if (col < p.N && row0 + dr < p.M) cp[(int64_t)dr * p.ldc] = v[q];
+                if (col < p.N && row0 + dr < p.M) hbs::stg(cp + (int64_t)dr * p.ldc, v[q]);
             }
         }
         if (tid == 0)
@@ -170,7 +170,7 @@ __device__ __forceinline__ void finish_tile(floatx16 (&acc)[BM / 64][BN / 64], c
 #pragma unroll
             for (int q = 0; q < 16; ++q) {
                 const int dr = 8 * (q >> 2) + (q & 3);
-                if (colok && row0 + dr < p.M) cp[(int64_t)dr * p.ldc] = acc[r][c][q];
+                if (colok && row0 + dr < p.M) hbs::stg(cp + (int64_t)dr * p.ldc, acc[r][c][q]);
             }
         }
     }
@@ -191,8 +191,10 @@ __device__ void f32_tile(float *smem, const Prob &p, const Item &it, float *part
     const int m0 = it.tm * BM, n0 = it.tn * BN, K = it.k1 - it.k0;
     const float *Ag = p.A + (int64_t)m0 * p.lda + it.k0;
     const float *Bg = p.B + (int64_t)it.k0 * p.ldb + n0;
-    hbs::f32_block_mma<BM, BN, 16, 2, 2, hbs::kTileClampK>(smem, Ag, Bg, p.M, p.N, K, p.lda,
-                                                            p.ldb, m0, n0, acc);
+    const hbs::TileArgs t{Ag, Bg, p.M, p.N, K, p.lda, p.ldb, m0, n0};
+    typename G::Regs regs;
+    hbs::NoChainGlobal ch;  // operands from the problem table: global loads
+    hbs::f32_block_mma<BM, BN, 16, 2, 2, hbs::kTileClampK>(smem, t, acc, regs, ch);
     finish_tile<BM, BN>(acc, p, it, partials, counters, s_flag);
 }
 
@@ -204,8 +206,10 @@ __device__ void b3_tile(uint16_t *smem, const Prob &p, const Item &it, float *pa
     const int m0 = it.tm * BM, n0 = it.tn * BN, K = it.k1 - it.k0;
     const float *Ag = p.A + (int64_t)m0 * p.lda + it.k0;
     const float *Bg = p.B + (int64_t)it.k0 * p.ldb + n0;
-    hbs::bf16x3_block_mma<G, hbs::kTileClampK>(smem, Ag, Bg, p.M, p.N, K, p.lda, p.ldb, m0, n0,
-                                               acc);
+    const hbs::TileArgs t{Ag, Bg, p.M, p.N, K, p.lda, p.ldb, m0, n0};
+    typename G::Regs regs;
+    hbs::NoChainGlobal ch;  // operands from the problem table: global loads
+    hbs::bf16x3_block_mma<G, hbs::kTileClampK>(smem, t, acc, regs, ch);
     finish_tile<BM, BN>(acc, p, it, partials, counters, s_flag);
 }
 
@@ -283,39 +287,74 @@ __global__ __launch_bounds__(kBigThreads) void grouped_big_kernel(const Prob *__
                                                                   const Item *__restrict__ items,
                                                                   const int *__restrict__ qoff,
                                                                   int nq, int *queue) {
-    constexpr int LDS = PREC == HBS_PREC_FP32 ? BigF32::kLdsFloats * 4 : 2 * BigB3::kStage * 2;
+    constexpr bool kF32 = PREC == HBS_PREC_FP32;
+    constexpr int LDS = kF32 ? BigF32::kLdsFloats * 4 : 2 * BigB3::kStage * 2;
+    using Regs = std::conditional_t<kF32, BigF32::Regs, BigB3::Regs>;
+    static_assert(BigF32::kFM == BigB3::kFM && BigF32::kFN == BigB3::kFN, "accumulators");
     __shared__ __attribute__((aligned(16))) char smem[LDS];
-    __shared__ int s_next;
+    __shared__ int s_next[2];
     // queue q = blockIdx.x % nq (the workgroup's XCD when nq = 8) hands out
     // items[qoff[q] .. qoff[q + 1]) in order; queue[q] is its head and
     // queue[kXcdQueues] counts the workgroups that have left
-    const int q = blockIdx.x % nq, end = qoff[q + 1];
-    if (threadIdx.x == 0)
-        s_next = qoff[q] +
-                 __hip_atomic_fetch_add(&queue[q], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    int i = s_next;
-    while (i < end) {
-        int next = 0;
-        if (threadIdx.x == 0)
-            next = qoff[q] + __hip_atomic_fetch_add(&queue[q], 1, __ATOMIC_RELAXED,
-                                                    __HIP_MEMORY_SCOPE_AGENT);
+    const int q = blockIdx.x % nq, first = qoff[q], end = qoff[q + 1];
+    auto fetch = [&]() {
+        return first +
+               __hip_atomic_fetch_add(&queue[q], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
+    auto tile = [&](int i, hbs::TileArgs &t, float *&Cg, int &ldc) {
         const Item it = items[i];
         const Prob p = probs[it.prob];
         const int m0 = it.tm * kBig, n0 = it.tn * kBig;
-        const float *Ag = p.A + (int64_t)m0 * p.lda;
-        const float *Bg = p.B + n0;
-        if constexpr (PREC == HBS_PREC_FP32)
-            hbs::f32_block_tile<256, 256, 32, 4, 2, hbs::kTileClamp>(
-                (float *)smem, Ag, Bg, p.C, p.M, p.N, p.K, p.lda, p.ldb, p.ldc, m0, n0);
-        else
-            hbs::bf16x3_block_tile<BigB3, hbs::kTileClamp>((uint16_t *)smem, Ag, Bg, p.C, p.M, p.N,
-                                                           p.K, p.lda, p.ldb, p.ldc, m0, n0);
-        if (threadIdx.x == 0) s_next = next;
-        __syncthreads();  // LDS is reused by the next item; s_next is set
-        i = s_next;
-        __syncthreads();  // every thread has read s_next
+        t = hbs::TileArgs{p.A + (int64_t)m0 * p.lda, p.B + n0, p.M, p.N, p.K, p.lda, p.ldb, m0, n0};
+        Cg = p.C;
+        ldc = p.ldc;
+    };
+    if (threadIdx.x == 0) s_next[0] = fetch();
+    __syncthreads();
+    const int i0 = __builtin_amdgcn_readfirstlane(s_next[0]);
+    if (i0 < end) {
+        // Tiles run back to back through one pipeline: while a tile of three
+        // or more K stages finishes, the next one's first two stages are
+        // loaded (hbs::TileChain).  Thread 0 fetches the next item at the
+        // start of each tile and publishes it in s_next[pw]; the two slots
+        // alternate, so a slot is rewritten only after every thread has read it.
+        hbs::TileArgs t;
+        float *Cg, *nCg = nullptr;
+        int ldc, nldc = 0, nxt = 0, pw = 1;
+        tile(i0, t, Cg, ldc);
+        auto publish = [&]() {
+            if (threadIdx.x == 0) s_next[pw] = nxt;
+        };
+        auto peek = [&](hbs::TileArgs &nt) {
+            const int j = __builtin_amdgcn_readfirstlane(s_next[pw]);
+            if (j >= end) return false;
+            tile(j, nt, nCg, nldc);
+            return true;
+        };
+        auto ch = hbs::make_chain(publish, peek);
+        Regs regs;
+        floatx16 acc[BigF32::kFM][BigF32::kFN];
+        for (;;) {
+            if (threadIdx.x == 0) nxt = fetch();
+            if constexpr (kF32) {
+                hbs::f32_block_mma<256, 256, 32, 4, 2, hbs::kTileClamp>((float *)smem, t, acc, regs,
+                                                                        ch);
+                hbs::f32_block_store<256, 256, 32, 4, 2, hbs::kTileClamp>(Cg, t.M, t.N, ldc, t.m0,
+                                                                          t.n0, acc);
+            } else {
+                hbs::bf16x3_block_mma<BigB3, hbs::kTileClamp>((uint16_t *)smem, t, acc, regs, ch);
+                hbs::bf16x3_block_store<BigB3, hbs::kTileClamp>(Cg, t.M, t.N, ldc, t.m0, t.n0,
+                                                                acc);
+            }
+            if (!ch.has_next) break;
+            t = ch.next;
+            Cg = nCg;
+            ldc = nldc;
+            pw ^= 1;
+        }
     }
+    // every fetch of this workgroup has returned (its value was published);
+    // the last workgroup to get here resets the queues
     if (threadIdx.x == 0 &&
         __hip_atomic_fetch_add(&queue[kXcdQueues], 1, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT) == (int)gridDim.x - 1) {

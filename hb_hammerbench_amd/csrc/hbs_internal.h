@@ -29,6 +29,69 @@ namespace hbs {
 //                  beyond K are also zero.
 enum TileMode { kTileInterior = 0, kTileEdge = 1, kTileClamp = 2, kTileClampK = 3 };
 
+// Loads and stores of matrix elements in device memory, typed as global
+// accesses: an operand pointer read from memory (a grouped problem table)
+// would otherwise be accessed with flat instructions, which also count against
+// the LDS wait counter and so make LDS fragment reads wait for global loads.
+template <class T>
+__device__ __forceinline__ T ldg(const T *p) {
+    return *(const __attribute__((address_space(1))) T *)p;
+}
+template <bool GLOBAL, class T>
+__device__ __forceinline__ T ld(const T *p) {
+    if constexpr (GLOBAL)
+        return ldg(p);
+    else
+        return *p;
+}
+template <class T>
+__device__ __forceinline__ void stg(T *p, T v) {
+    *(__attribute__((address_space(1))) T *)p = v;
+}
+
+// Operands of one block tile's main loop: Ag = A + m0 * lda (the tile's first
+// row), Bg = B + n0 (its first column); M, N, K bound the predicates.
+struct TileArgs {
+    const float *Ag, *Bg;
+    int M, N, K, lda, ldb, m0, n0;
+};
+
+// Cross-tile pipelining of a workgroup that runs block tiles back to back
+// (the grouped kernel's large-tile launch).  A main loop given a chain with
+// kOn set, on a tile of at least three K stages:
+//   * calls publish() before the barrier that ends step 0 (the caller makes
+//     the next tile's identity visible in LDS there) and peek(next) after it,
+//     which returns whether there is a next tile and fills its operands;
+//   * issues the next tile's stage-0 loads in the second-to-last step and, in
+//     the last step, stores that stage to the free LDS buffer and issues the
+//     stage-1 loads, then meets one more barrier;
+//   * returns pre = true and buf0 = that buffer: the next main loop skips its
+//     prologue (stage 0 already in LDS, stage 1 in the load registers).
+// On a shorter tile the loop runs unchained and ends with publish(), a barrier
+// and peek(next).  Every thread of the workgroup takes the same branches.
+// kGlobal: the operand pointers were read from memory (a problem table), so
+// the loads are typed as global (ldg); kernel-argument pointers need no cast.
+struct NoChain {
+    static constexpr bool kOn = false, kGlobal = false;
+};
+struct NoChainGlobal {
+    static constexpr bool kOn = false, kGlobal = true;
+};
+template <class Publish, class Peek>
+struct TileChain {
+    static constexpr bool kOn = true, kGlobal = true;
+    Publish publish;
+    Peek peek;
+    bool pre;       // in: this tile's stages 0 and 1 are already staged
+    int buf0;       // in: LDS buffer of stage 0; out: the next tile's
+    bool has_next;  // out: peek found a next tile (its operands in next)
+    TileArgs next;
+};
+template <class Publish, class Peek>
+__device__ __forceinline__ TileChain<Publish, Peek> make_chain(Publish p, Peek k) {
+    return TileChain<Publish, Peek>{p, k, false, 0, false, TileArgs{}};
+}
+
 // Checks that the current device is a gfx950 GPU (cached per device).
 int check_device();
 

@@ -25,8 +25,17 @@ struct B3Cfg {
     // bf16, plus 16 bytes of padding (conflict-free 128-bit fragment reads)
     static constexpr int kLdsRow = BK * 2 + 8;
     static constexpr int kStage = (BM + BN) * kLdsRow;    // bf16 elements per stage
+    static constexpr int kGroups = BK / 8;                 // 8-k groups per tile row
+    static constexpr int kItemsA = BM * kGroups / kThreads;
+    static constexpr int kItemsB = BN * kGroups / kThreads;
     static_assert(kWTM % 32 == 0 && kWTN % 32 == 0, "wave tile");
     static_assert(2 * kStage * 2 <= 160 * 1024, "LDS");
+    static_assert(BM * kGroups % kThreads == 0 && BN * kGroups % kThreads == 0, "items");
+    static_assert(BN % 64 == 0 && kThreads % BN == 0, "B items: one k group per wave");
+    // one stage in registers between its global loads and its split to LDS
+    struct Regs {
+        float a[kItemsA][8], b[kItemsB][8];
+    };
     // Sub-tile (wm, wn) of wave `wave`.  With 8 waves as 4 x 2, waves w and
     // w + 4 share a SIMD (read from HW_ID on MI355X: the four SIMDs take waves
     // 0..3 in some order, then 4..7 in the same order).  Rotating the rows of
@@ -73,22 +82,21 @@ __device__ __forceinline__ void split_pk(float x0, float x1, uint32_t &hi, uint3
 // = A[m0.., 0..K) . B[0..K, n0..), with Ag = A + m0 * lda (first row of the
 // tile) and Bg = B + n0 (first column); a K range [k0, k1) of a larger
 // product is Ag + k0, Bg + k0 * ldb and K = k1 - k0.  M, N, K bound the
-// predicates.  smem holds 2 * G::kStage bf16.  Every thread of the workgroup
-// must call it.
-template <class G, int MODE, bool COMPUTE>
+// predicates.
+// smem holds 2 * G::kStage bf16; regs is the stage in flight.  Every thread of
+// the workgroup must call it.  ch: NoChain, or a TileChain to pipeline the
+// next tile's first stages into this tile's last steps.
+template <class G, int MODE, bool COMPUTE, class Chain>
 __device__ __forceinline__ void bf16x3_block_mma_impl(uint16_t *smem, const float *__restrict__ Ag,
                                                       const float *__restrict__ Bg, int M, int N,
                                                       int K, int lda, int ldb, int m0, int n0,
                                                       int wm, int wn,
-                                                      floatx16 (&acc)[G::kFM][G::kFN]) {
+                                                      floatx16 (&acc)[G::kFM][G::kFN],
+                                                      typename G::Regs &regs, Chain &ch) {
     constexpr int kBM = G::BM, kBN = G::BN, kBK = G::BK, kThreads = G::kThreads;
     constexpr int kWTM = G::kWTM, kWTN = G::kWTN, kFM = G::kFM, kFN = G::kFN;
     constexpr int kLdsRow = G::kLdsRow, kStage = G::kStage;
-    constexpr int kGroups = kBK / 8;                       // 8-k groups per tile row
-    constexpr int kItemsA = kBM * kGroups / kThreads;
-    constexpr int kItemsB = kBN * kGroups / kThreads;
-    static_assert(kBM * kGroups % kThreads == 0 && kBN * kGroups % kThreads == 0, "items");
-    static_assert(kBN % 64 == 0 && kThreads % kBN == 0, "B items: one k group per wave");
+    constexpr int kGroups = G::kGroups, kItemsA = G::kItemsA, kItemsB = G::kItemsB;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -99,55 +107,76 @@ __device__ __forceinline__ void bf16x3_block_mma_impl(uint16_t *smem, const floa
 #pragma unroll
         for (int c = 0; c < kFN; ++c) acc[r][c] = floatx16{0};
 
-    float ra[kItemsA][8], rb[kItemsB][8];
+    // the stage in flight: the caller's registers when chained (they carry
+    // the next tile's stage 1 across the call), else two local arrays
+    float own_a[kItemsA][8], own_b[kItemsB][8];
+    auto &ra = *(Chain::kOn ? &regs.a : &own_a);
+    auto &rb = *(Chain::kOn ? &regs.b : &own_b);
 
+    // Operands of the global loads: this tile's, until a chained loop points
+    // them at the next tile once this tile's last stage has been loaded.
+    const float *lA = Ag, *lB = Bg;
+    int lM = M, lN = N, lK = K, llda = lda, lldb = ldb, lm0 = m0, ln0 = n0;
+    auto retarget = [&](const TileArgs &n) {
+        lA = n.Ag;
+        lB = n.Bg;
+        lM = n.M;
+        lN = n.N;
+        lK = n.K;
+        llda = n.lda;
+        lldb = n.ldb;
+        lm0 = n.m0;
+        ln0 = n.n0;
+    };
+    constexpr bool kG = Chain::kGlobal;
     auto gload_m = [&](auto mode, int k0) {
         constexpr int LM = decltype(mode)::value;
 #pragma unroll
         for (int i = 0; i < kItemsA; ++i) {
             const int f = tid + i * kThreads;
             const int row = f / kGroups, gk = k0 + (f % kGroups) * 8;
-            const int srow = LM >= kTileClamp ? min(row, M - m0 - 1) : row;
-            const float *x = Ag + (int64_t)srow * lda + gk;
+            const int srow = LM >= kTileClamp ? min(row, lM - lm0 - 1) : row;
+            const float *x = lA + (int64_t)srow * llda + gk;
             if (LM != kTileEdge) {
                 const floatx4 zero = {0, 0, 0, 0};
-                const bool k0ok = LM != kTileClampK || gk < K;
-                const bool k1ok = LM != kTileClampK || gk + 4 < K;
-                const floatx4 v0 = k0ok ? *(const floatx4 *)x : zero;
-                const floatx4 v1 = k1ok ? *(const floatx4 *)(x + 4) : zero;
+                const bool k0ok = LM != kTileClampK || gk < lK;
+                const bool k1ok = LM != kTileClampK || gk + 4 < lK;
+                const floatx4 v0 = k0ok ? ld<kG>((const floatx4 *)x) : zero;
+                const floatx4 v1 = k1ok ? ld<kG>((const floatx4 *)(x + 4)) : zero;
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     ra[i][j] = v0[j];
                     ra[i][j + 4] = v1[j];
                 }
             } else {
-                const bool rok = m0 + row < M;
+                const bool rok = lm0 + row < lM;
 #pragma unroll
-                for (int j = 0; j < 8; ++j) ra[i][j] = (rok && gk + j < K) ? x[j] : 0.0f;
+                for (int j = 0; j < 8; ++j)
+                    ra[i][j] = (rok && gk + j < lK) ? ld<kG>(x + j) : 0.0f;
             }
         }
 #pragma unroll
         for (int i = 0; i < kItemsB; ++i) {
             const int f = tid + i * kThreads;
-            const unsigned col =
-                LM >= kTileClamp ? (unsigned)min(f % kBN, N - n0 - 1) : (unsigned)(f % kBN);
+            const unsigned col = LM >= kTileClamp ? (unsigned)min(f % kBN, lN - ln0 - 1)
+                                                  : (unsigned)(f % kBN);
             // the k group is the same for the whole wave (kBN is a multiple of 64)
             // and each k row pointer is uniform: scalar base + 32-bit lane offset
             const int gk = k0 + __builtin_amdgcn_readfirstlane(f / kBN) * 8;
-            const bool cok = LM != kTileEdge || n0 + (int)col < N;
+            const bool cok = LM != kTileEdge || ln0 + (int)col < lN;
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
-                const float *rowp = Bg + (int64_t)(gk + j) * ldb;
+                const float *rowp = lB + (int64_t)(gk + j) * lldb;
                 const bool ok = LM == kTileInterior || LM == kTileClamp ||
-                                (LM == kTileClampK ? gk + j < K : (cok && gk + j < K));
-                rb[i][j] = ok ? rowp[col] : 0.0f;
+                                (LM == kTileClampK ? gk + j < lK : (cok && gk + j < lK));
+                rb[i][j] = ok ? ld<kG>(rowp + col) : 0.0f;
             }
         }
     };
     // With kTileClampK only the stage that runs past K needs K predicates;
     // every full stage takes the clamped loads (the choice is uniform).
     auto gload = [&](int k0) {
-        if (MODE == kTileClampK && k0 + kBK > K)
+        if (MODE == kTileClampK && k0 + kBK > lK)
             gload_m(std::integral_constant<int, kTileClampK>{}, k0);
         else
             gload_m(std::integral_constant<int, MODE == kTileClampK ? kTileClamp : MODE>{}, k0);
@@ -219,49 +248,103 @@ __device__ __forceinline__ void bf16x3_block_mma_impl(uint16_t *smem, const floa
     // Pipeline: the global loads of stage kt + 2 are issued in the middle of
     // step kt, right after stage kt + 1 has been split into the other LDS
     // buffer, so each load has a whole step to arrive; the split of one
-    // stage runs between the two substeps of the previous one.
+    // stage runs between the two substeps of the previous one.  Stage kt is
+    // in LDS buffer (kt + b0) & 1.
     const int KT = (K + kBK - 1) / kBK;
-    gload(0);
-    sstore(0);
-    if (KT > 1) gload(kBK);
-    __syncthreads();
+    int b0 = 0;
+    bool pre = false;
+    if constexpr (Chain::kOn) {
+        b0 = ch.buf0;
+        pre = ch.pre;
+    }
+    if (!pre) {
+        gload(0);
+        sstore(b0);
+        if (KT > 1) gload(kBK);
+        __syncthreads();
+    }
     int kt = 0;
     for (; kt + 2 < KT; ++kt) {
-        const int cur = kt & 1;
+        const int cur = (kt + b0) & 1;
         compute(cur, 0);
         sstore(cur ^ 1);
         gload((kt + 2) * kBK);
         compute(cur, 1);
+        if constexpr (Chain::kOn) {
+            if (kt == 0) ch.publish();
+        }
         __syncthreads();
     }
+    bool chain = false;
+    if constexpr (Chain::kOn) {
+        if (KT >= 3) {
+            ch.has_next = ch.peek(ch.next);
+            chain = ch.has_next;
+        }
+    }
     if (kt + 1 < KT) {
-        const int cur = kt & 1;
+        const int cur = (kt + b0) & 1;
         compute(cur, 0);
         sstore(cur ^ 1);
+        if constexpr (Chain::kOn) {
+            if (chain) {
+                retarget(ch.next);
+                gload(0);
+            }
+        }
         compute(cur, 1);
         __syncthreads();
         ++kt;
     }
-    compute(kt & 1, 0);
-    compute(kt & 1, 1);
+    const int cur = (kt + b0) & 1;
+    compute(cur, 0);
+    if constexpr (Chain::kOn) {
+        if (chain) {
+            sstore(cur ^ 1);
+            if (ch.next.K > kBK) gload(kBK);
+        }
+    }
+    compute(cur, 1);
+    if constexpr (Chain::kOn) {
+        if (chain) {
+            __syncthreads();
+        } else if (KT < 3) {
+            ch.publish();
+            __syncthreads();
+            ch.has_next = ch.peek(ch.next);
+        }
+        ch.pre = chain;
+        ch.buf0 = cur ^ 1;
+    }
 }
 
 // With 8 waves as 4 x 2, a wave whose sub-tile lies wholly outside M x N
 // (edge tiles outside kTileInterior) runs the same loop without fragment reads
 // and MFMAs: it still loads and splits its share of each stage and meets every
 // barrier.
+template <class G, int MODE, class Chain>
+__device__ __forceinline__ void bf16x3_block_mma(uint16_t *smem, const TileArgs &t,
+                                                 floatx16 (&acc)[G::kFM][G::kFN],
+                                                 typename G::Regs &regs, Chain &ch) {
+    int wm, wn;
+    G::wave_pos(__builtin_amdgcn_readfirstlane(threadIdx.x >> 6), wm, wn);
+    constexpr bool kSkip = MODE != kTileInterior && G::WM == 4 && G::WN == 2;
+    if (!kSkip || (t.m0 + wm * G::kWTM < t.M && t.n0 + wn * G::kWTN < t.N))
+        bf16x3_block_mma_impl<G, MODE, true>(smem, t.Ag, t.Bg, t.M, t.N, t.K, t.lda, t.ldb, t.m0,
+                                             t.n0, wm, wn, acc, regs, ch);
+    else
+        bf16x3_block_mma_impl<G, MODE, false>(smem, t.Ag, t.Bg, t.M, t.N, t.K, t.lda, t.ldb, t.m0,
+                                              t.n0, wm, wn, acc, regs, ch);
+}
 template <class G, int MODE>
 __device__ __forceinline__ void bf16x3_block_mma(uint16_t *smem, const float *__restrict__ Ag,
                                                  const float *__restrict__ Bg, int M, int N, int K,
                                                  int lda, int ldb, int m0, int n0,
                                                  floatx16 (&acc)[G::kFM][G::kFN]) {
-    int wm, wn;
-    G::wave_pos(__builtin_amdgcn_readfirstlane(threadIdx.x >> 6), wm, wn);
-    constexpr bool kSkip = MODE != kTileInterior && G::WM == 4 && G::WN == 2;
-    if (!kSkip || (m0 + wm * G::kWTM < M && n0 + wn * G::kWTN < N))
-        bf16x3_block_mma_impl<G, MODE, true>(smem, Ag, Bg, M, N, K, lda, ldb, m0, n0, wm, wn, acc);
-    else
-        bf16x3_block_mma_impl<G, MODE, false>(smem, Ag, Bg, M, N, K, lda, ldb, m0, n0, wm, wn, acc);
+    const TileArgs t{Ag, Bg, M, N, K, lda, ldb, m0, n0};
+    typename G::Regs regs;
+    NoChain ch;
+    bf16x3_block_mma<G, MODE>(smem, t, acc, regs, ch);
 }
 
 // Epilogue: stores acc to C (Cg = the matrix) at the tile (m0, n0); outside
@@ -284,7 +367,7 @@ __device__ __forceinline__ void bf16x3_block_store(float *__restrict__ Cg, int M
             for (int q = 0; q < 16; ++q) {
                 const int row = m0 + wm * kWTM + r * 32 + 8 * (q >> 2) + 4 * h + (q & 3);
                 if (MODE == kTileInterior || (row < M && col < N))
-                    Cg[(int64_t)row * ldc + col] = acc[r][c][q];
+                    stg(Cg + (int64_t)row * ldc + col, acc[r][c][q]);
             }
         }
     }

@@ -32,6 +32,10 @@ struct F32Cfg {
     static constexpr int kStageA = BM * kLdA;
     static constexpr int kStageB = BK * kLdB;
     static constexpr int kLdsFloats = 2 * (kStageA + kStageB);
+    // one stage in registers between its global loads and its store to LDS
+    struct Regs {
+        floatx4 a[kAF4], b[kBF4];
+    };
     static_assert(kWTM % 32 == 0 && kWTN % 32 == 0, "wave tile must be a multiple of 32");
     static_assert((BM * BK / 4) % kThreads == 0, "A tile must split evenly over threads");
     static_assert((BK * BN / 4) % kThreads == 0, "B tile must split evenly over threads");
@@ -63,11 +67,14 @@ struct F32Cfg {
 // workgroup must call it.  MODE (TileMode): kTileInterior, kTileEdge (any
 // shape or alignment), kTileClamp (ragged M or N; K a multiple of BK, N of 4,
 // 16-byte loadable operands) or kTileClampK (the same with K a multiple of 4).
-template <int BM, int BN, int BK, int WM, int WN, int MODE, bool COMPUTE>
+// regs is the stage in flight.  ch: NoChain, or a TileChain to pipeline the
+// next tile's first stages into this tile's last steps.
+template <int BM, int BN, int BK, int WM, int WN, int MODE, bool COMPUTE, class Chain>
 __device__ __forceinline__ void f32_block_mma_impl(
     float *smem, const float *__restrict__ Ag, const float *__restrict__ Bg, int M, int N, int K,
     int lda, int ldb, int m0, int n0, int wm, int wn,
-    floatx16 (&acc)[F32Cfg<BM, BN, BK, WM, WN>::kFM][F32Cfg<BM, BN, BK, WM, WN>::kFN]) {
+    floatx16 (&acc)[F32Cfg<BM, BN, BK, WM, WN>::kFM][F32Cfg<BM, BN, BK, WM, WN>::kFN],
+    typename F32Cfg<BM, BN, BK, WM, WN>::Regs &regs, Chain &ch) {
     using G = F32Cfg<BM, BN, BK, WM, WN>;
     float *const As = smem;
     float *const Bs = smem + 2 * G::kStageA;
@@ -81,8 +88,28 @@ __device__ __forceinline__ void f32_block_mma_impl(
 #pragma unroll
         for (int c = 0; c < G::kFN; ++c) acc[r][c] = floatx16{0};
 
-    floatx4 ra[G::kAF4], rb[G::kBF4];
+    // the stage in flight: the caller's registers when chained (they carry
+    // the next tile's stage 1 across the call), else two local arrays
+    floatx4 own_a[G::kAF4], own_b[G::kBF4];
+    auto &ra = *(Chain::kOn ? &regs.a : &own_a);
+    auto &rb = *(Chain::kOn ? &regs.b : &own_b);
 
+    // Operands of the global loads: this tile's, until a chained loop points
+    // them at the next tile once this tile's last stage has been loaded.
+    const float *lA = Ag, *lB = Bg;
+    int lM = M, lN = N, lK = K, llda = lda, lldb = ldb, lm0 = m0, ln0 = n0;
+    auto retarget = [&](const TileArgs &n) {
+        lA = n.Ag;
+        lB = n.Bg;
+        lM = n.M;
+        lN = n.N;
+        lK = n.K;
+        llda = n.lda;
+        lldb = n.ldb;
+        lm0 = n.m0;
+        ln0 = n.n0;
+    };
+    constexpr bool kG = Chain::kGlobal;
     auto gload_m = [&](auto mode, int k0) {
         constexpr int LM = decltype(mode)::value;
 #pragma unroll
@@ -90,16 +117,17 @@ __device__ __forceinline__ void f32_block_mma_impl(
             const int f = tid + i * G::kThreads;
             const int row = f / (BK / 4), c4 = f % (BK / 4);
             if constexpr (LM != kTileEdge) {
-                const int srow = LM >= kTileClamp ? min(row, M - m0 - 1) : row;
+                const int srow = LM >= kTileClamp ? min(row, lM - lm0 - 1) : row;
                 const floatx4 zero = {0, 0, 0, 0};
-                ra[i] = (LM != kTileClampK || k0 + c4 * 4 < K)
-                            ? *(const floatx4 *)(Ag + (int64_t)srow * lda + k0 + c4 * 4)
+                ra[i] = (LM != kTileClampK || k0 + c4 * 4 < lK)
+                            ? ld<kG>((const floatx4 *)(lA + (int64_t)srow * llda + k0 + c4 * 4))
                             : zero;
             } else {
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     const int gk = k0 + c4 * 4 + j;
-                    ra[i][j] = (m0 + row < M && gk < K) ? Ag[(int64_t)row * lda + gk] : 0.0f;
+                    ra[i][j] =
+                        (lm0 + row < lM && gk < lK) ? ld<kG>(lA + (int64_t)row * llda + gk) : 0.0f;
                 }
             }
         }
@@ -108,17 +136,18 @@ __device__ __forceinline__ void f32_block_mma_impl(
             const int f = tid + i * G::kThreads;
             const int kr = f / (BN / 4), c4 = f % (BN / 4);
             if constexpr (LM != kTileEdge) {
-                const int sc = LM >= kTileClamp ? min(c4 * 4, N - n0 - 4) : c4 * 4;
+                const int sc = LM >= kTileClamp ? min(c4 * 4, lN - ln0 - 4) : c4 * 4;
                 const floatx4 zero = {0, 0, 0, 0};
-                rb[i] = (LM != kTileClampK || k0 + kr < K)
-                            ? *(const floatx4 *)(Bg + (int64_t)(k0 + kr) * ldb + sc)
+                rb[i] = (LM != kTileClampK || k0 + kr < lK)
+                            ? ld<kG>((const floatx4 *)(lB + (int64_t)(k0 + kr) * lldb + sc))
                             : zero;
             } else {
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    const int gn = n0 + c4 * 4 + j;
-                    rb[i][j] = (k0 + kr < K && gn < N) ? Bg[(int64_t)(k0 + kr) * ldb + c4 * 4 + j]
-                                                          : 0.0f;
+                    const int gn = ln0 + c4 * 4 + j;
+                    rb[i][j] = (k0 + kr < lK && gn < lN)
+                                   ? ld<kG>(lB + (int64_t)(k0 + kr) * lldb + c4 * 4 + j)
+                                   : 0.0f;
                 }
             }
         }
@@ -126,7 +155,7 @@ __device__ __forceinline__ void f32_block_mma_impl(
     // With kTileClampK only the stage that runs past K needs K predicates;
     // every full stage takes the clamped loads (the choice is uniform).
     auto gload = [&](int k0) {
-        if (MODE == kTileClampK && k0 + BK > K)
+        if (MODE == kTileClampK && k0 + BK > lK)
             gload_m(std::integral_constant<int, kTileClampK>{}, k0);
         else
             gload_m(std::integral_constant<int, MODE == kTileClampK ? kTileClamp : MODE>{}, k0);
@@ -214,48 +243,104 @@ __device__ __forceinline__ void f32_block_mma_impl(
     // Pipeline: stage kt + 1 is stored to the other LDS buffer in the middle
     // of step kt and the global loads of stage kt + 2 are issued right after,
     // so every load has a whole step to arrive and the store overlaps MFMAs.
+    // Stage kt is in LDS buffer (kt + b0) & 1.
     const int KT = (K + BK - 1) / BK;
-    gload(0);
-    sstore(0);
-    if (KT > 1) gload(BK);
-    __syncthreads();
+    int b0 = 0;
+    bool pre = false;
+    if constexpr (Chain::kOn) {
+        b0 = ch.buf0;
+        pre = ch.pre;
+    }
+    if (!pre) {
+        gload(0);
+        sstore(b0);
+        if (KT > 1) gload(BK);
+        __syncthreads();
+    }
     int kt = 0;
     for (; kt + 2 < KT; ++kt) {
-        const int cur = kt & 1;
+        const int cur = (kt + b0) & 1;
         compute(cur, [&]() {
             sstore(cur ^ 1);
             gload((kt + 2) * BK);
         });
+        if constexpr (Chain::kOn) {
+            if (kt == 0) ch.publish();
+        }
         __syncthreads();
     }
+    bool chain = false;
+    if constexpr (Chain::kOn) {
+        if (KT >= 3) {
+            ch.has_next = ch.peek(ch.next);
+            chain = ch.has_next;
+        }
+    }
     if (kt + 1 < KT) {
-        const int cur = kt & 1;
-        compute(cur, [&]() { sstore(cur ^ 1); });
+        const int cur = (kt + b0) & 1;
+        compute(cur, [&]() {
+            sstore(cur ^ 1);
+            if constexpr (Chain::kOn) {
+                if (chain) {
+                    retarget(ch.next);
+                    gload(0);
+                }
+            }
+        });
         __syncthreads();
         ++kt;
     }
-    compute(kt & 1, [&]() {});
+    const int cur = (kt + b0) & 1;
+    compute(cur, [&]() {
+        if constexpr (Chain::kOn) {
+            if (chain) {
+                sstore(cur ^ 1);
+                if (ch.next.K > BK) gload(BK);
+            }
+        }
+    });
+    if constexpr (Chain::kOn) {
+        if (chain) {
+            __syncthreads();
+        } else if (KT < 3) {
+            ch.publish();
+            __syncthreads();
+            ch.has_next = ch.peek(ch.next);
+        }
+        ch.pre = chain;
+        ch.buf0 = cur ^ 1;
+    }
 }
 
 // With 8 waves as 4 x 2, a wave whose sub-tile lies wholly outside M x N
 // (edge tiles outside kTileInterior) runs the same loop without fragment reads
 // and MFMAs: it still loads and stores its share of each stage and meets every
 // barrier.
+template <int BM, int BN, int BK, int WM, int WN, int MODE, class Chain>
+__device__ __forceinline__ void f32_block_mma(
+    float *smem, const TileArgs &t,
+    floatx16 (&acc)[F32Cfg<BM, BN, BK, WM, WN>::kFM][F32Cfg<BM, BN, BK, WM, WN>::kFN],
+    typename F32Cfg<BM, BN, BK, WM, WN>::Regs &regs, Chain &ch) {
+    using G = F32Cfg<BM, BN, BK, WM, WN>;
+    int wm, wn;
+    G::wave_pos(__builtin_amdgcn_readfirstlane(threadIdx.x >> 6), wm, wn);
+    constexpr bool kSkip = MODE != kTileInterior && WM == 4 && WN == 2;
+    if (!kSkip || (t.m0 + wm * G::kWTM < t.M && t.n0 + wn * G::kWTN < t.N))
+        f32_block_mma_impl<BM, BN, BK, WM, WN, MODE, true>(
+            smem, t.Ag, t.Bg, t.M, t.N, t.K, t.lda, t.ldb, t.m0, t.n0, wm, wn, acc, regs, ch);
+    else
+        f32_block_mma_impl<BM, BN, BK, WM, WN, MODE, false>(
+            smem, t.Ag, t.Bg, t.M, t.N, t.K, t.lda, t.ldb, t.m0, t.n0, wm, wn, acc, regs, ch);
+}
 template <int BM, int BN, int BK, int WM, int WN, int MODE>
 __device__ __forceinline__ void f32_block_mma(
     float *smem, const float *__restrict__ Ag, const float *__restrict__ Bg, int M, int N, int K,
     int lda, int ldb, int m0, int n0,
     floatx16 (&acc)[F32Cfg<BM, BN, BK, WM, WN>::kFM][F32Cfg<BM, BN, BK, WM, WN>::kFN]) {
-    using G = F32Cfg<BM, BN, BK, WM, WN>;
-    int wm, wn;
-    G::wave_pos(__builtin_amdgcn_readfirstlane(threadIdx.x >> 6), wm, wn);
-    constexpr bool kSkip = MODE != kTileInterior && WM == 4 && WN == 2;
-    if (!kSkip || (m0 + wm * G::kWTM < M && n0 + wn * G::kWTN < N))
-        f32_block_mma_impl<BM, BN, BK, WM, WN, MODE, true>(smem, Ag, Bg, M, N, K, lda, ldb, m0,
-                                                           n0, wm, wn, acc);
-    else
-        f32_block_mma_impl<BM, BN, BK, WM, WN, MODE, false>(smem, Ag, Bg, M, N, K, lda, ldb, m0,
-                                                            n0, wm, wn, acc);
+    const TileArgs t{Ag, Bg, M, N, K, lda, ldb, m0, n0};
+    typename F32Cfg<BM, BN, BK, WM, WN>::Regs regs;
+    NoChain ch;
+    f32_block_mma<BM, BN, BK, WM, WN, MODE>(smem, t, acc, regs, ch);
 }
 
 // Epilogue: stores acc to C (Cg = the matrix) at the tile (m0, n0).  32x32
@@ -280,7 +365,7 @@ __device__ __forceinline__ void f32_block_store(
             for (int q = 0; q < 16; ++q) {
                 const int row = m0 + wm * G::kWTM + r * 32 + 8 * (q >> 2) + 4 * h + (q & 3);
                 if (MODE == kTileInterior || (row < M && col < N))
-                    Cg[(int64_t)row * ldc + col] = acc[r][c][q];
+                    stg(Cg + (int64_t)row * ldc + col, acc[r][c][q]);
             }
         }
     }
