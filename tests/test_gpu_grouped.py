@@ -375,3 +375,41 @@ def test_grouped_split_64_long_k(prec):
     else:
         bound = (3.1 * 2.0 ** -16 + gamma(3 * K + 2)) * S
     assert np.all(np.abs(C.cpu().numpy() - R) <= bound)
+
+
+@pytest.mark.parametrize("prec", [hb.PREC_FP32, hb.PREC_BF16X3])
+def test_grouped_chained_tiles_match_problems_run_alone(prec):
+    """The large-tile launch runs a workgroup's tiles back to back, loading the
+    next tile's first stages during the current tile's last steps.  A tile's
+    arithmetic does not depend on which tile ran before it, so a group of
+    about a thousand 256 x 256 tiles with K of 1, 2, 3, 4 and 8 stages (32 to
+    256) and ragged M and N must give, bit for bit, what each problem gives in
+    a plan of its own (one or a few tiles per workgroup)."""
+    rng = np.random.default_rng(77)
+    shapes = []
+    for i in range(130):
+        M = int(rng.integers(1, 1100))
+        N = 4 * int(rng.integers(1, 275))
+        K = (32, 64, 96, 128, 256)[i % 5]
+        shapes.append((M, N, K))
+    probs, views, _ = make_group(shapes, "uniform", seed=91)
+    plan = hb.GroupedPlan(probs, prec)
+    info = plan.info()
+    tiles = sum(-(-M // 256) * -(-N // 256) for M, N, K in shapes)
+    assert info["tiles"] == tiles and tiles > 3 * 256 and info["split_tiles"] == 0
+    plan.run()
+    torch.cuda.synchronize()
+    together = [C.clone() for (_, _, C) in views]
+    for pr, (_, _, C), t, shape in zip(probs, views, together, shapes):
+        C.fill_(float("nan"))
+        alone = hb.GroupedPlan([pr], prec)
+        alone.run()
+        torch.cuda.synchronize()
+        assert torch.equal(C, t), shape
+    for (A, B, C), (M, N, K) in zip(views[:10], shapes[:10]):
+        R, S = oracle.gemm_f64(A.cpu().numpy(), B.cpu().numpy())
+        if prec == hb.PREC_FP32:
+            bound = gamma(K + 2) * S
+        else:
+            bound = (3.1 * 2.0 ** -16 + gamma(3 * K + 2)) * S
+        assert np.all(np.abs(C.cpu().numpy() - R) <= bound), (M, N, K)
