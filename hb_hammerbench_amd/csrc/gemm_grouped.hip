@@ -59,13 +59,15 @@ struct hbs_grouped_plan {
     int *d_counters;      // split-K counters
     int *d_queue;         // queue head and finished-workgroup count (zero at rest)
     float *d_partials;
-    // large-tile launch (whole problems in 256 x 256 tiles)
-    int nblocks_big;
-    int64_t items_big;
-    const void *d_items_big;
-    const int *d_qoff_big;  // item range of each of its nq_big queues
-    int nq_big;
-    int *d_queue_big;       // nq_big heads and a finished-workgroup count
+    // large-tile launches (whole problems in 256 x 256 tiles): [0] problems
+    // whose K is a multiple of the 32-deep stage (kTileClamp), [1] BF16x3
+    // problems with any other K (kTileClampK)
+    int nblocks_big[2];
+    int64_t items_big[2];
+    const void *d_items_big[2];
+    const int *d_qoff_big[2];  // item range of each of its nq_big queues
+    int nq_big[2];
+    int *d_queue_big[2];       // nq_big heads and a finished-workgroup count
 };
 
 namespace {
@@ -275,19 +277,23 @@ __global__ __launch_bounds__(kThreads, 2) void grouped_kernel(const Prob *__rest
 // whose sub-tile lies outside M x N skip their MFMAs).  A problem on this path
 // is covered whole: tiles at its ragged bottom and right edges read clamped
 // rows and columns (kTileClamp) and store only what lies inside M x N.  No
-// split-K; K must be a multiple of the 32-deep stage.
+// split-K.  MODE kTileClamp takes problems whose K is a multiple of the
+// 32-deep stage; kTileClampK (BF16x3 only, a launch of its own) the others,
+// zero-filling past K in the last stage.  One kernel holding both modes ran
+// out of registers and cost the first kind 5%.
 // ---------------------------------------------------------------------------
 using BigF32 = hbs::F32Cfg<256, 256, 32, 4, 2>;
 constexpr int kXcdQueues = 8;  // one item queue per XCD on the large-tile launch
 using BigB3 = hbs::B3Main;
 static_assert(BigF32::kThreads == kBigThreads && BigB3::kThreads == kBigThreads, "big tiles");
 
-template <int PREC>
+template <int PREC, int MODE>
 __global__ __launch_bounds__(kBigThreads) void grouped_big_kernel(const Prob *__restrict__ probs,
                                                                   const Item *__restrict__ items,
                                                                   const int *__restrict__ qoff,
                                                                   int nq, int *queue) {
     constexpr bool kF32 = PREC == HBS_PREC_FP32;
+    static_assert(MODE == hbs::kTileClamp || (MODE == hbs::kTileClampK && !kF32), "mode");
     constexpr int LDS = kF32 ? BigF32::kLdsFloats * 4 : 2 * BigB3::kStage * 2;
     using Regs = std::conditional_t<kF32, BigF32::Regs, BigB3::Regs>;
     static_assert(BigF32::kFM == BigB3::kFM && BigF32::kFN == BigB3::kFN, "accumulators");
@@ -337,12 +343,12 @@ __global__ __launch_bounds__(kBigThreads) void grouped_big_kernel(const Prob *__
         for (;;) {
             if (threadIdx.x == 0) nxt = fetch();
             if constexpr (kF32) {
-                hbs::f32_block_mma<256, 256, 32, 4, 2, hbs::kTileClamp>((float *)smem, t, acc, regs,
+                hbs::f32_block_mma<256, 256, 32, 4, 2, MODE>((float *)smem, t, acc, regs,
                                                                         ch);
                 hbs::f32_block_store<256, 256, 32, 4, 2, hbs::kTileClamp>(Cg, t.M, t.N, ldc, t.m0,
                                                                           t.n0, acc);
             } else {
-                hbs::bf16x3_block_mma<BigB3, hbs::kTileClamp>((uint16_t *)smem, t, acc, regs, ch);
+                hbs::bf16x3_block_mma<BigB3, MODE>((uint16_t *)smem, t, acc, regs, ch);
                 hbs::bf16x3_block_store<BigB3, hbs::kTileClamp>(Cg, t.M, t.N, ldc, t.m0, t.n0,
                                                                 acc);
             }
@@ -408,18 +414,22 @@ bool tile_supported(int precision, int tile) {
     return tile >= 0 && tile <= kTileBig;
 }
 
-// The large-tile path needs K a multiple of its 32-deep stage.  By default
-// every problem that allows it takes it, whatever its M, N and K, in both
-// precisions: the 256 x 256 tile runs at a higher per-CU rate than the 128 x
-// 128 tiles of two workgroups per CU (for BF16x3 two to five times), its edge
-// tiles skip the MFMAs of waves outside M x N and its per-XCD queues even out
-// their shorter times, and one launch leaves no second tail.  Tuning files and
-// forced plans can still select the 256-thread tiles per size class.
-bool big_possible(int M, int N, int K) { (void)M; (void)N; return K % 32 == 0; }
-bool big_default(int precision, int M, int N, int K) {
-    (void)precision;
-    return big_possible(M, N, K);
+// By default every problem that can take the large-tile path takes it,
+// whatever its M, N and K: the 256 x 256 tile runs at a higher per-CU rate than
+// the 128 x 128 tiles of two workgroups per CU (for BF16x3 two to five times),
+// its edge tiles skip the MFMAs of waves outside M x N and its per-XCD queues
+// even out their shorter times, and one launch leaves no second tail.  BF16x3
+// takes any K there: a tile whose K is not a multiple of the 32-deep stage
+// zero-fills past K in its last stage (kTileClampK, chosen per tile).  FP32
+// keeps such problems on the 256-thread launch: its large tile in that mode
+// spills and gained 2.7% on such a group, against 30% for BF16x3.  Tuning
+// files and forced plans can still select the 256-thread tiles per size class.
+bool big_possible(int precision, int M, int N, int K) {
+    (void)M;
+    (void)N;
+    return precision == HBS_PREC_BF16X3 || K % 32 == 0;
 }
+bool big_default(int precision, int M, int N, int K) { return big_possible(precision, M, N, K); }
 
 Choice heuristic(int precision, int M, int N, int K) {
     Choice c{0, 1, 0.0};
@@ -457,13 +467,17 @@ int validate_all(const hbs_gemm_problem *problems, int count, int precision) {
 // launch sizes for a GPU of `cus` compute units.  No HIP call.
 struct HostPlan {
     std::vector<Prob> probs;
-    std::vector<Item> sorted, sorted_big;  // 256-thread list; large-tile queues
-    std::vector<int> qoff_big;
-    int nq_big = 1, nblocks = 0, nblocks_big = 0;
+    std::vector<Item> sorted;         // 256-thread list
+    std::vector<Item> sorted_big[2];  // large-tile launches, in queue order
+    std::vector<int> qoff_big[2];
+    int nq_big[2] = {1, 1}, nblocks = 0, nblocks_big[2] = {0, 0};
     int64_t slots = 0, poff = 0, tiles_total = 0;
-    double imb = 1.0, imb_big = 1.0;
+    double imb = 1.0, imb_big[2] = {1.0, 1.0};
     double est_imbalance() const {
-        return std::max(nblocks > 0 ? imb : 1.0, nblocks_big > 0 ? imb_big : 1.0);
+        double e = nblocks > 0 ? imb : 1.0;
+        for (int g = 0; g < 2; ++g)
+            if (nblocks_big[g] > 0) e = std::max(e, imb_big[g]);
+        return e;
     }
 };
 
@@ -480,19 +494,19 @@ int plan_host(const hbs_gemm_problem *problems, int count, int precision, const 
     };
     std::vector<Prob> probs(count);
     std::vector<Small> small;
-    std::vector<Item> items_big;
-    std::vector<double> cost_big;
+    std::vector<Item> items_big[2];  // by launch: K a multiple of 32, other K
+    std::vector<double> cost_big[2];
     auto lookup = [&](int M, int N, int K, bool &big) {
         Choice c = heuristic(precision, M, N, K);
         big = big_default(precision, M, N, K);
         if (forced) {
-            big = forced->tile == kTileBig && big_possible(M, N, K);
+            big = forced->tile == kTileBig && big_possible(precision, M, N, K);
             if (forced->tile != kTileBig) c = *forced;
             else c.split = forced->split;  // for problems that cannot take the path
         } else if (table) {
             auto f = table->find(TuneKey(precision, size_class(M), size_class(N), size_class(K)));
             if (f != table->end() && tile_supported(precision, f->second.tile)) {
-                big = f->second.tile == kTileBig && big_possible(M, N, K);
+                big = f->second.tile == kTileBig && big_possible(precision, M, N, K);
                 if (f->second.tile != kTileBig) c = f->second;
             }
         }
@@ -508,16 +522,18 @@ int plan_host(const hbs_gemm_problem *problems, int count, int precision, const 
             continue;
         }
         // the whole problem in 256 x 256 tiles; edge tiles clamp their loads
+        const int g = q.K % 32 == 0 ? 0 : 1;
         for (int tm = 0; tm < (q.M + kBig - 1) / kBig; ++tm)
             for (int tn = 0; tn < (q.N + kBig - 1) / kBig; ++tn) {
-                items_big.push_back(Item{i, tm, tn, 0, q.K, kTileBig, 1, 0, -1, 0});
-                cost_big.push_back((double)q.K);
+                items_big[g].push_back(Item{i, tm, tn, 0, q.K, kTileBig, 1, 0, -1, 0});
+                cost_big[g].push_back((double)((q.K + 31) / 32 * 32));
             }
     }
 
     std::vector<Item> items;
     std::vector<double> cost;
-    int64_t slots = 0, poff = 0, tiles_total = (int64_t)items_big.size();
+    int64_t slots = 0, poff = 0,
+            tiles_total = (int64_t)(items_big[0].size() + items_big[1].size());
     for (const Small &sp : small) {
         const Prob &q = probs[sp.prob];
         const Choice &c = sp.c;
@@ -563,7 +579,9 @@ int plan_host(const hbs_gemm_problem *problems, int count, int precision, const 
     }
     if (items.size() > 0x7fffffffULL) return HBS_ERR_ARG;
 
-    if (items.size() > 0x7fffffffULL || items_big.size() > 0x7fffffffULL) return HBS_ERR_ARG;
+    if (items.size() > 0x7fffffffULL || items_big[0].size() > 0x7fffffffULL ||
+        items_big[1].size() > 0x7fffffffULL)
+        return HBS_ERR_ARG;
 
     // 256-thread launch: items in decreasing cost, handed out in that order to
     // whichever workgroup is free first.  With the modelled costs this is
@@ -656,26 +674,22 @@ int plan_host(const hbs_gemm_problem *problems, int count, int precision, const 
     // holds one per CU
     const int per_cu = 2;
     const int nblocks = (int)std::min<int64_t>((int64_t)per_cu * cus, (int64_t)items.size());
-    const int nblocks_big = (int)std::min<int64_t>(cus, (int64_t)items_big.size());
-    std::vector<Item> sorted, sorted_big;
-    std::vector<int> qoff_big;
-    int nq_big = 1;
-    double imb = 1.0, imb_big = 1.0;
+    std::vector<Item> sorted;
+    double imb = 1.0;
     lpt(items, cost, nblocks, sorted, imb);
-    xcd_queues(items_big, cost_big, nblocks_big, sorted_big, qoff_big, nq_big, imb_big);
+    for (int g = 0; g < 2; ++g) {
+        hp.nblocks_big[g] = (int)std::min<int64_t>(cus, (int64_t)items_big[g].size());
+        xcd_queues(items_big[g], cost_big[g], hp.nblocks_big[g], hp.sorted_big[g],
+                   hp.qoff_big[g], hp.nq_big[g], hp.imb_big[g]);
+    }
 
     hp.probs = std::move(probs);
     hp.sorted = std::move(sorted);
-    hp.sorted_big = std::move(sorted_big);
-    hp.qoff_big = std::move(qoff_big);
-    hp.nq_big = nq_big;
     hp.nblocks = nblocks;
-    hp.nblocks_big = nblocks_big;
     hp.slots = slots;
     hp.poff = poff;
     hp.tiles_total = tiles_total;
     hp.imb = imb;
-    hp.imb_big = imb_big;
     return HBS_SUCCESS;
 }
 
@@ -694,25 +708,28 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
     st = plan_host(problems, count, precision, forced, table, cus, hp);
     if (st) return st;
     const std::vector<Prob> &probs = hp.probs;
-    const std::vector<Item> &sorted = hp.sorted, &sorted_big = hp.sorted_big;
-    const std::vector<int> &qoff_big = hp.qoff_big;
-    const int nblocks = hp.nblocks, nblocks_big = hp.nblocks_big;
+    const std::vector<Item> &sorted = hp.sorted;
+    const int nblocks = hp.nblocks;
     const int64_t slots = hp.slots, poff = hp.poff, tiles_total = hp.tiles_total;
 
     // one device allocation: problems | items | split counters + queues |
-    // partials | large-tile items | large-tile queue offsets
+    // partials | per large-tile launch: items, queue offsets
     auto align = [](size_t x) { return (x + 255) / 256 * 256; };
     const size_t b_probs = align(probs.size() * sizeof(Prob));
     const size_t b_items = align(std::max<size_t>(sorted.size(), 1) * sizeof(Item));
-    // split counters, the 256-thread launch's queue (2), the large-tile
+    // split counters, the 256-thread launch's queue (2), each large-tile
     // launch's queues (kXcdQueues heads + 1)
-    const size_t b_cnt = align((slots + 2 + kXcdQueues + 1) * sizeof(int));
+    const size_t b_cnt = align((slots + 2 + 2 * (kXcdQueues + 1)) * sizeof(int));
     const size_t b_part = align(std::max<int64_t>(poff, 1) * sizeof(float));
-    const size_t b_items_big = align(std::max<size_t>(sorted_big.size(), 1) * sizeof(Item));
-    const size_t b_qoff_big = align(qoff_big.size() * sizeof(int));
     const size_t o_items = b_probs, o_cnt = o_items + b_items;
-    const size_t o_part = o_cnt + b_cnt, o_items_big = o_part + b_part;
-    const size_t o_qoff_big = o_items_big + b_items_big, total = o_qoff_big + b_qoff_big;
+    const size_t o_part = o_cnt + b_cnt;
+    size_t o_items_big[2], o_qoff_big[2], total = o_part + b_part;
+    for (int g = 0; g < 2; ++g) {
+        o_items_big[g] = total;
+        total += align(std::max<size_t>(hp.sorted_big[g].size(), 1) * sizeof(Item));
+        o_qoff_big[g] = total;
+        total += align(hp.qoff_big[g].size() * sizeof(int));
+    }
     char *dmem = nullptr;
     HBS_HIP_TRY(hipMalloc(&dmem, total));
     hbs_grouped_plan *plan = new hbs_grouped_plan();
@@ -729,23 +746,29 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
     plan->d_items = dmem + o_items;
     plan->d_counters = (int *)(dmem + o_cnt);
     plan->d_queue = plan->d_counters + slots;
-    plan->d_queue_big = plan->d_queue + 2;
     plan->d_partials = (float *)(dmem + o_part);
-    plan->nblocks_big = nblocks_big;
-    plan->items_big = (int64_t)sorted_big.size();
-    plan->d_items_big = dmem + o_items_big;
-    plan->d_qoff_big = (const int *)(dmem + o_qoff_big);
-    plan->nq_big = hp.nq_big;
+    for (int g = 0; g < 2; ++g) {
+        plan->d_queue_big[g] = plan->d_queue + 2 + g * (kXcdQueues + 1);
+        plan->nblocks_big[g] = hp.nblocks_big[g];
+        plan->items_big[g] = (int64_t)hp.sorted_big[g].size();
+        plan->d_items_big[g] = dmem + o_items_big[g];
+        plan->d_qoff_big[g] = (const int *)(dmem + o_qoff_big[g]);
+        plan->nq_big[g] = hp.nq_big[g];
+    }
     hipError_t e = hipMemcpy(dmem, probs.data(), probs.size() * sizeof(Prob), hipMemcpyHostToDevice);
     if (e == hipSuccess && !sorted.empty())
         e = hipMemcpy(dmem + o_items, sorted.data(), sorted.size() * sizeof(Item),
                       hipMemcpyHostToDevice);
-    if (e == hipSuccess && !sorted_big.empty())
-        e = hipMemcpy(dmem + o_items_big, sorted_big.data(), sorted_big.size() * sizeof(Item),
-                      hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = hipMemcpy(dmem + o_qoff_big, qoff_big.data(), qoff_big.size() * sizeof(int),
-                      hipMemcpyHostToDevice);
+    for (int g = 0; g < 2; ++g) {
+        const std::vector<Item> &sb = hp.sorted_big[g];
+        const std::vector<int> &qo = hp.qoff_big[g];
+        if (e == hipSuccess && !sb.empty())
+            e = hipMemcpy(dmem + o_items_big[g], sb.data(), sb.size() * sizeof(Item),
+                          hipMemcpyHostToDevice);
+        if (e == hipSuccess && !qo.empty())
+            e = hipMemcpy(dmem + o_qoff_big[g], qo.data(), qo.size() * sizeof(int),
+                          hipMemcpyHostToDevice);
+    }
     if (e == hipSuccess) e = hipMemset(plan->d_counters, 0, b_cnt);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
@@ -799,9 +822,10 @@ extern "C" int hbs_grouped_plan_describe(const hbs_gemm_problem *problems, int c
                              tuned ? &table : nullptr, cus, hp);
     if (st) return st;
     info->problems = count;
-    info->workgroups = hp.nblocks + hp.nblocks_big;
+    info->workgroups = hp.nblocks + hp.nblocks_big[0] + hp.nblocks_big[1];
     info->tiles = hp.tiles_total;
-    info->work_items = (int64_t)(hp.sorted.size() + hp.sorted_big.size());
+    info->work_items = (int64_t)(hp.sorted.size() + hp.sorted_big[0].size() +
+                                 hp.sorted_big[1].size());
     info->split_tiles = hp.slots;
     info->est_imbalance = hp.est_imbalance();
     return HBS_SUCCESS;
@@ -810,9 +834,9 @@ extern "C" int hbs_grouped_plan_describe(const hbs_gemm_problem *problems, int c
 extern "C" int hbs_grouped_plan_info(const hbs_grouped_plan *plan, hbs_grouped_info *info) {
     if (!plan || !info) return HBS_ERR_NULL;
     info->problems = plan->count;
-    info->workgroups = plan->nblocks + plan->nblocks_big;
+    info->workgroups = plan->nblocks + plan->nblocks_big[0] + plan->nblocks_big[1];
     info->tiles = plan->tiles;
-    info->work_items = plan->items + plan->items_big;
+    info->work_items = plan->items + plan->items_big[0] + plan->items_big[1];
     info->split_tiles = plan->split_tiles;
     info->est_imbalance = plan->est_imbalance;
     return HBS_SUCCESS;
@@ -826,15 +850,22 @@ extern "C" int hbs_grouped_run(const hbs_grouped_plan *plan, void *stream) {
     hipStream_t s = (hipStream_t)stream;
     const Prob *pr = (const Prob *)plan->d_probs;
     const Item *it = (const Item *)plan->d_items;
-    const Item *itb = (const Item *)plan->d_items_big;
     const bool fp32 = plan->precision == HBS_PREC_FP32;
-    if (plan->nblocks_big > 0) {
-        if (fp32)
-            grouped_big_kernel<HBS_PREC_FP32><<<plan->nblocks_big, kBigThreads, 0, s>>>(
-                pr, itb, plan->d_qoff_big, plan->nq_big, plan->d_queue_big);
+    for (int g = 0; g < 2; ++g) {
+        if (plan->nblocks_big[g] == 0) continue;
+        const Item *itb = (const Item *)plan->d_items_big[g];
+        const dim3 grid(plan->nblocks_big[g]);
+        if (fp32 && g == 0)
+            grouped_big_kernel<HBS_PREC_FP32, hbs::kTileClamp><<<grid, kBigThreads, 0, s>>>(
+                pr, itb, plan->d_qoff_big[g], plan->nq_big[g], plan->d_queue_big[g]);
+        else if (!fp32 && g == 0)
+            grouped_big_kernel<HBS_PREC_BF16X3, hbs::kTileClamp><<<grid, kBigThreads, 0, s>>>(
+                pr, itb, plan->d_qoff_big[g], plan->nq_big[g], plan->d_queue_big[g]);
+        else if (!fp32)
+            grouped_big_kernel<HBS_PREC_BF16X3, hbs::kTileClampK><<<grid, kBigThreads, 0, s>>>(
+                pr, itb, plan->d_qoff_big[g], plan->nq_big[g], plan->d_queue_big[g]);
         else
-            grouped_big_kernel<HBS_PREC_BF16X3><<<plan->nblocks_big, kBigThreads, 0, s>>>(
-                pr, itb, plan->d_qoff_big, plan->nq_big, plan->d_queue_big);
+            return HBS_ERR_ARG;  // FP32 plans never fill the second launch
         const int rc = hbs::launch_status();
         if (rc) return rc;
     }

@@ -3,14 +3,16 @@ same planning as hbs_grouped_plan_create -- size heuristic, tuning file or
 forced tile / split-K, work-item expansion, launch sizes -- without a device.
 Expected counts follow from the planning rules:
 
-  * a problem whose K is a multiple of 32 takes the large-tile launch (by
-    default and with tile 4): ceil(M/256) ceil(N/256) tiles, no split-K;
+  * a problem takes the large-tile launch (by default and with tile 4) when
+    its K is a multiple of 32, and in BF16x3 whatever its K: ceil(M/256)
+    ceil(N/256) tiles, no split-K.  BF16x3 problems whose K is not a
+    multiple of 32 form a second large-tile launch (zero fill past K);
   * any other problem takes the 256-thread launch with the heuristic tile
     (64 x 64 when M, N <= 64, 64 x 128 when M <= 64, 128 x 64 when N <= 64,
     else 128 x 128) and split-K 4 (2) when it has at most 4 (16) tiles and
     K >= 1024; a split K range is ceil(K / s) rounded up to the stage depth
     (16 for FP32, 32 for BF16x3), giving ceil(K / range) work items per tile;
-  * workgroups: one per CU on the large-tile launch, two per CU on the other,
+  * workgroups: one per CU on each large-tile launch, two per CU on the other,
     never more than the launch's work items.
 """
 import math
@@ -34,11 +36,12 @@ def heuristic_tile(M, N):
 
 def expected(shapes, prec, cus=256, tile=None, split=1):
     stage = 16 if prec == hb.PREC_FP32 else 32
-    big_tiles = small_tiles = small_items = split_tiles = 0
+    big_tiles = [0, 0]  # large-tile launches: K a multiple of 32, other K
+    small_tiles = small_items = split_tiles = 0
     for M, N, K in shapes:
-        big = K % 32 == 0 and (tile is None or tile == 4)
+        big = (K % 32 == 0 or prec == hb.PREC_BF16X3) and (tile is None or tile == 4)
         if big:
-            big_tiles += math.ceil(M / 256) * math.ceil(N / 256)
+            big_tiles[0 if K % 32 == 0 else 1] += math.ceil(M / 256) * math.ceil(N / 256)
             continue
         t = heuristic_tile(M, N) if tile in (None, 4) else tile
         BM, BN = TILE[t]
@@ -52,9 +55,9 @@ def expected(shapes, prec, cus=256, tile=None, split=1):
         small_tiles += n
         small_items += n * ns
         split_tiles += n if ns > 1 else 0
-    return {"tiles": big_tiles + small_tiles, "work_items": big_tiles + small_items,
+    return {"tiles": sum(big_tiles) + small_tiles, "work_items": sum(big_tiles) + small_items,
             "split_tiles": split_tiles,
-            "workgroups": min(cus, big_tiles) + min(2 * cus, small_items)}
+            "workgroups": sum(min(cus, b) for b in big_tiles) + min(2 * cus, small_items)}
 
 
 MIXED = [(1000, 1028, 768), (64, 2048, 256), (300, 260, 256), (200, 68, 100),
@@ -72,7 +75,10 @@ def check(info, want, count):
 def test_describe_size_heuristic(prec, cus):
     info = hb.describe_plan(MIXED, prec, cus=cus)
     want = expected(MIXED, prec, cus)
-    assert want["split_tiles"] > 0 and want["tiles"] > 20
+    assert want["tiles"] > 20
+    # FP32 keeps the problems whose K is not a multiple of 32 on the 256-thread
+    # launch (some with split-K); BF16x3 runs every problem on large tiles
+    assert (want["split_tiles"] > 0) == (prec == hb.PREC_FP32)
     check(info, want, len(MIXED))
 
 
