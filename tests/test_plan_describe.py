@@ -17,6 +17,7 @@ Expected counts follow from the planning rules:
 """
 import math
 
+import numpy as np
 import pytest
 
 import hb_hammerbench_amd as hb
@@ -116,3 +117,20 @@ def test_describe_rejects_bad_arguments(tmp_path):
     assert code([(64, 64, 64)], tile=0, tuning_file=str(tmp_path / "t.txt")) == -1
     assert code([(64, 64, 64)], tuning_file=str(tmp_path / "missing.txt")) == -1
     assert code([(64, 64, 64)], cus=0) == -1
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_describe_random_groups(seed):
+    """Random groups (K any multiple of 4, so both large-tile launches of
+    BF16x3 and the 256-thread launch of FP32 all appear), random CU counts,
+    default and forced plans: the dry run's counts follow the rules above."""
+    rng = np.random.default_rng(seed)
+    shapes = [(int(rng.integers(1, 3000)), 4 * int(rng.integers(1, 700)),
+               4 * int(rng.integers(1, 1100))) for _ in range(int(rng.integers(1, 60)))]
+    cus = int(rng.choice([8, 24, 104, 256]))
+    for prec in (hb.PREC_FP32, hb.PREC_BF16X3):
+        check(hb.describe_plan(shapes, prec, cus=cus), expected(shapes, prec, cus), len(shapes))
+        tile = int(rng.integers(0, 5))
+        split = int(rng.integers(1, 9)) if tile < 4 else 1
+        check(hb.describe_plan(shapes, prec, tile=tile, split_k=split, cus=cus),
+              expected(shapes, prec, cus, tile=tile, split=split), len(shapes))
