@@ -343,8 +343,7 @@ __global__ __launch_bounds__(kBigThreads) void grouped_big_kernel(const Prob *__
         for (;;) {
             if (threadIdx.x == 0) nxt = fetch();
             if constexpr (kF32) {
-                hbs::f32_block_mma<256, 256, 32, 4, 2, MODE>((float *)smem, t, acc, regs,
-                                                                        ch);
+                hbs::f32_block_mma<256, 256, 32, 4, 2, MODE>((float *)smem, t, acc, regs, ch);
                 hbs::f32_block_store<256, 256, 32, 4, 2, hbs::kTileClamp>(Cg, t.M, t.N, ldc, t.m0,
                                                                           t.n0, acc);
             } else {
