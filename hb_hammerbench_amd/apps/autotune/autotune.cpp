@@ -15,7 +15,7 @@
 //    over --reps launches after one warm-up launch; writes one line per class
 //    with the measured time per work item (the plan's load-balancing cost);
 //  * --group FILE (lines "M N K"): tunes for that actual problem set.  Starting
-//    from the plan's size heuristic it visits the size classes in decreasing
+//    from the plan's default it visits the size classes in decreasing
 //    share of the flops, times the whole group with each candidate override of
 //    that class (tile ids 0..4, split-K 1, 2, 4), and keeps the best override
 //    only when it runs >= 2% faster and a confirming re-time with twice the
@@ -86,7 +86,7 @@ static void write_table(const char *path, const char *pname, const std::vector<O
 }
 
 // Median time (ms) of `reps` runs of the whole group under tuning file `path`
-// ("" = the plan's heuristic).
+// ("" = the plan's default).
 static double time_group(const std::vector<hbs_gemm_problem> &probs, int prec, const char *path,
                          int reps, hipEvent_t e0, hipEvent_t e1) {
     hbs_grouped_plan *plan = nullptr;
@@ -176,7 +176,7 @@ static int group_mode(const char *shapes_path, const char *out_path, const std::
         std::vector<Override> ov;
         const double base = time_group(probs, prec, "", reps, e0, e1);
         double best = base;
-        printf("%s group of %d: heuristic %.3f ms (%.1f TFLOP/s)\n", pname, count, base,
+        printf("%s group of %d: default %.3f ms (%.1f TFLOP/s)\n", pname, count, base,
                total / (base * 1e-3) / 1e12);
         fflush(stdout);
         for (auto &cl : classes) {
@@ -215,15 +215,15 @@ static int group_mode(const char *shapes_path, const char *out_path, const std::
                 }
             }
             printf("%s class M%d N%d K%d (%.1f%% of flops): %s -> %.3f ms\n", pname, mc, nc, kc,
-                   100.0 * cl.first, keep.tile >= 0 ? "override" : "heuristic", best);
+                   100.0 * cl.first, keep.tile >= 0 ? "override" : "default", best);
             fflush(stdout);
         }
         if (ov.empty()) best = base;  // no override: the re-times above were noise
-        printf("%s tuned: %.3f ms (%.1f TFLOP/s), heuristic %.3f ms, %zu overrides\n", pname, best,
+        printf("%s tuned: %.3f ms (%.1f TFLOP/s), default %.3f ms, %zu overrides\n", pname, best,
                total / (best * 1e-3) / 1e12, base, ov.size());
         fflush(stdout);
         char line[256];
-        snprintf(line, sizeof line, "# %s: group of %d problems from %s: heuristic %.3f ms, tuned %.3f ms\n",
+        snprintf(line, sizeof line, "# %s: group of %d problems from %s: default %.3f ms, tuned %.3f ms\n",
                  pname, count, shapes_path, base, best);
         merged += line;
         for (const Override &o : ov) {

@@ -18,7 +18,7 @@
 // NITER slices as the NITER problems of one grouped plan, replayed from a
 // hipGraph, with its tile shape and split-K from --tile / --split (one choice
 // for every problem) or from an autotuner tuning file (--tuning), else the
-// plan's size heuristic.
+// plan's default (the 256 x 256 large tile for every problem).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -72,7 +72,7 @@ struct Options {
     int reps = 0;
     int max_print = 16;
     std::string mode = "square";
-    int tile = -1;  // grouped: HBS_TILE_* for every problem, -1 = tuning file / heuristic
+    int tile = -1;  // grouped: HBS_TILE_* for every problem, -1 = tuning file / default
     int split = 1;
     std::string tuning;
 };

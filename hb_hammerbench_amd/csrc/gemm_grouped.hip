@@ -7,9 +7,9 @@
 //
 // Design:
 //  * a plan (host side, built once) chooses a tile shape and split-K factor per
-//    problem -- from the autotuner's tuning file when one is given, else from a
-//    size heuristic -- and expands every problem into work items (one output
-//    tile times one K range);
+//    problem -- from the autotuner's tuning file or a forced choice, else the
+//    256 x 256 large tile for every problem -- and expands every problem into
+//    work items (one output tile times one K range);
 //  * two persistent launches, captured together in a hipGraph, each with a
 //    fixed number of workgroups walking their own item lists (no host work or
 //    memset between replays):
@@ -60,8 +60,8 @@ struct hbs_grouped_plan {
     int *d_queue;         // queue head and finished-workgroup count (zero at rest)
     float *d_partials;
     // large-tile launches (whole problems in 256 x 256 tiles): [0] problems
-    // whose K is a multiple of the 32-deep stage (kTileClamp), [1] BF16x3
-    // problems with any other K (kTileClampK)
+    // whose K is a multiple of the 32-deep stage (kTileClamp), [1] problems
+    // with any other K (kTileClampK)
     int nblocks_big[2];
     int64_t items_big[2];
     const void *d_items_big[2];
@@ -278,9 +278,9 @@ __global__ __launch_bounds__(kThreads, 2) void grouped_kernel(const Prob *__rest
 // is covered whole: tiles at its ragged bottom and right edges read clamped
 // rows and columns (kTileClamp) and store only what lies inside M x N.  No
 // split-K.  MODE kTileClamp takes problems whose K is a multiple of the
-// 32-deep stage; kTileClampK (BF16x3 only, a launch of its own) the others,
-// zero-filling past K in the last stage.  One kernel holding both modes ran
-// out of registers and cost the first kind 5%.
+// 32-deep stage; kTileClampK (a launch of its own) the others, zero-filling
+// past K in the last stage.  One kernel holding both modes ran out of
+// registers and cost the first kind 5%.
 // ---------------------------------------------------------------------------
 using BigF32 = hbs::F32Cfg<256, 256, 32, 4, 2>;
 constexpr int kXcdQueues = 8;  // one item queue per XCD on the large-tile launch
@@ -293,7 +293,7 @@ __global__ __launch_bounds__(kBigThreads) void grouped_big_kernel(const Prob *__
                                                                   const int *__restrict__ qoff,
                                                                   int nq, int *queue) {
     constexpr bool kF32 = PREC == HBS_PREC_FP32;
-    static_assert(MODE == hbs::kTileClamp || (MODE == hbs::kTileClampK && !kF32), "mode");
+    static_assert(MODE == hbs::kTileClamp || MODE == hbs::kTileClampK, "mode");
     constexpr int LDS = kF32 ? BigF32::kLdsFloats * 4 : 2 * BigB3::kStage * 2;
     using Regs = std::conditional_t<kF32, BigF32::Regs, BigB3::Regs>;
     static_assert(BigF32::kFM == BigB3::kFM && BigF32::kFN == BigB3::kFN, "accumulators");
@@ -413,34 +413,17 @@ bool tile_supported(int precision, int tile) {
     return tile >= 0 && tile <= kTileBig;
 }
 
-// By default every problem that can take the large-tile path takes it,
-// whatever its M, N and K: the 256 x 256 tile runs at a higher per-CU rate than
-// the 128 x 128 tiles of two workgroups per CU (for BF16x3 two to five times),
-// its edge tiles skip the MFMAs of waves outside M x N and its per-XCD queues
-// even out their shorter times, and one launch leaves no second tail.  BF16x3
-// takes any K there: a tile whose K is not a multiple of the 32-deep stage
-// zero-fills past K in its last stage (kTileClampK, chosen per tile).  FP32
-// keeps such problems on the 256-thread launch: its large tile in that mode
-// spills and gained 2.7% on such a group, against 30% for BF16x3.  Tuning
-// files and forced plans can still select the 256-thread tiles per size class.
-bool big_possible(int precision, int M, int N, int K) {
-    (void)M;
-    (void)N;
-    return precision == HBS_PREC_BF16X3 || K % 32 == 0;
-}
-bool big_default(int precision, int M, int N, int K) { return big_possible(precision, M, N, K); }
-
-Choice heuristic(int precision, int M, int N, int K) {
-    Choice c{0, 1, 0.0};
-    if (M <= 64 && N <= 64) c.tile = 1;
-    else if (M <= 64) c.tile = 3;
-    else if (N <= 64) c.tile = 2;
-    const int64_t tiles = (int64_t)((M + kTileBM[c.tile] - 1) / kTileBM[c.tile]) *
-                          ((N + kTileBN[c.tile] - 1) / kTileBN[c.tile]);
-    if (tiles <= 4 && K >= 1024) c.split = 4;
-    else if (tiles <= 16 && K >= 1024) c.split = 2;
-    return c;
-}
+// By default every problem takes the large-tile path, whatever its M, N and K:
+// the 256 x 256 tile runs at a higher per-CU rate than the 128 x 128 tiles of
+// two workgroups per CU (for BF16x3 two to five times), its edge tiles skip the
+// MFMAs of waves outside M x N and its per-XCD queues even out their shorter
+// times, and one launch leaves no second tail.  Any K
+// (a multiple of 4, as the grouped API requires) can take it: problems whose K
+// is not a multiple of the 32-deep stage form a second launch in which the
+// last stage zero-fills past K (kTileClampK).  On such a group that launch
+// beat the 256-thread one by 33% (BF16x3) and 2.4% (FP32, whose large tile
+// spills 56 bytes in that mode).  Tuning files and forced plans can still
+// select the 256-thread tiles per size class.
 
 int validate(const hbs_gemm_problem &q) {
     if (q.M < 1 || q.N < 1 || q.K < 1) return HBS_ERR_ARG;
@@ -496,17 +479,16 @@ int plan_host(const hbs_gemm_problem *problems, int count, int precision, const 
     std::vector<Item> items_big[2];  // by launch: K a multiple of 32, other K
     std::vector<double> cost_big[2];
     auto lookup = [&](int M, int N, int K, bool &big) {
-        Choice c = heuristic(precision, M, N, K);
-        big = big_default(precision, M, N, K);
+        Choice c{0, 1, 0.0};  // the 256-thread tile, when one is chosen
+        big = true;           // the default (see above)
         if (forced) {
-            big = forced->tile == kTileBig && big_possible(precision, M, N, K);
-            if (forced->tile != kTileBig) c = *forced;
-            else c.split = forced->split;  // for problems that cannot take the path
+            big = forced->tile == kTileBig;
+            if (!big) c = *forced;
         } else if (table) {
             auto f = table->find(TuneKey(precision, size_class(M), size_class(N), size_class(K)));
             if (f != table->end() && tile_supported(precision, f->second.tile)) {
-                big = f->second.tile == kTileBig && big_possible(precision, M, N, K);
-                if (f->second.tile != kTileBig) c = f->second;
+                big = f->second.tile == kTileBig;
+                if (!big) c = f->second;
             }
         }
         return c;
@@ -864,7 +846,8 @@ extern "C" int hbs_grouped_run(const hbs_grouped_plan *plan, void *stream) {
             grouped_big_kernel<HBS_PREC_BF16X3, hbs::kTileClampK><<<grid, kBigThreads, 0, s>>>(
                 pr, itb, plan->d_qoff_big[g], plan->nq_big[g], plan->d_queue_big[g]);
         else
-            return HBS_ERR_ARG;  // FP32 plans never fill the second launch
+            grouped_big_kernel<HBS_PREC_FP32, hbs::kTileClampK><<<grid, kBigThreads, 0, s>>>(
+                pr, itb, plan->d_qoff_big[g], plan->nq_big[g], plan->d_queue_big[g]);
         const int rc = hbs::launch_status();
         if (rc) return rc;
     }

@@ -103,8 +103,8 @@ enum { HBS_PREC_FP32 = 0, HBS_PREC_BF16X3 = 1 };
 /* Tile shapes of the grouped kernel (BM x BN).  0..3 run on the 256-thread
  * persistent launch (with optional split-K).  HBS_TILE_256x256 runs the whole
  * problem on a 512-thread large-tile launch with the square kernels' tile (no
- * split-K; edge tiles clamp their loads): FP32 when K is a multiple of 32,
- * BF16x3 for any K (a second launch zero-fills past K when it is not). */
+ * split-K; edge tiles clamp their loads; problems whose K is not a multiple of
+ * 32 run on a second such launch that zero-fills past K). */
 enum {
     HBS_TILE_128x128 = 0,
     HBS_TILE_64x64 = 1,
@@ -129,16 +129,13 @@ typedef struct {
  * problem: M, N, K >= 1, lda >= K, ldb >= N, ldc >= N; A and B 16-byte aligned
  * with N, K, lda, ldb multiples of 4 (HBS_ERR_ALIGN otherwise).  Tile shape
  * and split-K come from `tuning_file` (written by the autotuner) when it is
- * non-NULL -- HBS_ERR_ARG if it cannot be read -- else from a size heuristic
- * (HBS_TILE_256x256 for every BF16x3 problem, and for FP32 whenever K is a
- * multiple of 32).  Allocates and uploads
+ * non-NULL -- HBS_ERR_ARG if it cannot be read -- for the size classes it
+ * names; every other problem takes HBS_TILE_256x256.  Allocates and uploads
  * the plan's device buffers (synchronous). */
 int hbs_grouped_plan_create(const hbs_gemm_problem *problems, int count, int precision,
                             const char *tuning_file, hbs_grouped_plan **plan);
 /* Same with one tile shape and split-K factor (1..64) for every problem (the
- * autotuner's probe).  With HBS_TILE_256x256, problems that cannot take the
- * large-tile path (FP32 with K not a multiple of 32) use the size heuristic's
- * tile with this split-K. */
+ * autotuner's probe; split-K does not apply to HBS_TILE_256x256). */
 int hbs_grouped_plan_create_forced(const hbs_gemm_problem *problems, int count, int precision,
                                    int tile, int split_k, hbs_grouped_plan **plan);
 int hbs_grouped_plan_info(const hbs_grouped_plan *plan, hbs_grouped_info *info);
@@ -146,7 +143,7 @@ int hbs_grouped_plan_info(const hbs_grouped_plan *plan, hbs_grouped_info *info);
  * counts a plan for these problems would have on a GPU of `cus` compute units,
  * without a device, device memory or any HIP call (capacity planning, and
  * tests of the planner on machines without a GPU).  tile = -1 takes the
- * tuning file when `tuning_file` is non-NULL, else the size heuristic, as
+ * tuning file when `tuning_file` is non-NULL, else the default tile, as
  * hbs_grouped_plan_create does; tile >= 0 forces tile and split_k as
  * hbs_grouped_plan_create_forced does (then `tuning_file` must be NULL).  The
  * problems are validated as for a real plan; their pointers are not

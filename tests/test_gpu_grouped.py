@@ -142,25 +142,20 @@ def test_grouped_plan_info_counts_tiles():
 
 # Shapes for the large-tile path: whole problems in 256 x 256 tiles, ragged at
 # the bottom and right (clamped loads) or not, plus one K that is not a
-# multiple of 32: FP32 keeps it on the 256-thread launch, BF16x3 runs it on
-# a second large-tile launch that zero-fills past K.
+# multiple of 32, which runs on a second large-tile launch that zero-fills
+# past K.
 BIG = [(300, 260, 256), (512, 512, 320), (257, 1024, 256), (256, 256, 512), (1000, 1028, 768),
        (1024, 300, 1024), (512, 512, 100), (64, 2048, 256)]
 
 
-def big_tiles(shapes, prec):
-    """256 x 256 tiles the plan puts on the large-tile launches (default or
-    tile 4): K a multiple of 32, and in BF16x3 any K."""
-    n = 0
-    for M, N, K in shapes:
-        if K % 32 == 0 or prec == hb.PREC_BF16X3:
-            n += ((M + 255) // 256) * ((N + 255) // 256)
-    return n
+def big_tiles(shapes):
+    """256 x 256 tiles of these problems on the large-tile launches."""
+    return sum(((M + 255) // 256) * ((N + 255) // 256) for M, N, _ in shapes)
 
 
 # Tuning lines that send the thin, long-K class (M, N about 64, K about 1024
 # or 2048) to 64 x 64 tiles with split-K 4 on the 256-thread launch, so that a
-# BF16x3 group also exercises that launch.
+# group also exercises that launch.
 THIN_SPLIT = "".join(f"{p} 64 64 {k} 1 4 0\n" for p in ("fp32", "bf16x3") for k in (1024, 2048))
 
 
@@ -175,26 +170,20 @@ def test_grouped_large_tiles_integer_exact(prec):
 
 
 @pytest.mark.parametrize("prec", [hb.PREC_FP32, hb.PREC_BF16X3])
-@pytest.mark.parametrize("mode", ["forced", "forced-split2", "heuristic"])
+@pytest.mark.parametrize("mode", ["forced", "forced-split2", "default"])
 def test_grouped_large_tiles_within_bound(prec, mode):
     """Whole problems on the large-tile launch (edge tiles with idle waves,
-    per-XCD queues) and K = 100 on the 256-thread launch (FP32) or the
-    zero-filling large-tile launch (BF16x3): every problem within its FP64
-    bound, a second run identical (the queues reset themselves), and the
-    plan's tile count includes the 256 x 256 tiles."""
+    per-XCD queues) and K = 100 on the zero-filling large-tile launch: every
+    problem within its FP64 bound, a second run identical (the queues reset
+    themselves), and the plan's tiles all 256 x 256 (split-K does not apply)."""
     probs, views, _ = make_group(BIG, "uniform", seed=29)
-    if mode == "heuristic":
+    if mode == "default":
         plan = hb.GroupedPlan(probs, prec)
     else:
         plan = hb.GroupedPlan(probs, prec, tile=hb.TILE_256x256, split_k=2 if "split" in mode else 1)
     info = plan.info()
-    nbig = big_tiles(BIG, prec)
-    if prec == hb.PREC_FP32:
-        assert info["tiles"] > nbig > 0  # (512, 512, 100) on the 256-thread launch
-    else:
-        assert info["tiles"] == nbig and info["work_items"] == nbig
-    if mode == "forced-split2":
-        assert (info["split_tiles"] > 0) == (prec == hb.PREC_FP32)
+    nbig = big_tiles(BIG)
+    assert info["tiles"] == nbig and info["work_items"] == nbig and info["split_tiles"] == 0
     plan.run()
     torch.cuda.synchronize()
     first = [C.clone() for (_, _, C) in views]
@@ -242,9 +231,8 @@ def test_grouped_large_tiles_identity_reproduces_oracle_split():
 @pytest.mark.parametrize("prec", [hb.PREC_FP32, hb.PREC_BF16X3])
 def test_grouped_graph_replays_match_eager(prec, tmp_path):
     # Every launch in one graph: the large-tile launch; (64, 64, 2048) on the
-    # 256-thread launch with split-K (tuning file); K = 100 on the 256-thread
-    # launch (FP32, 128 x 128 tiles) or the zero-filling large-tile launch
-    # (BF16x3)
+    # 256-thread launch with split-K (tuning file); K = 100 on the
+    # zero-filling large-tile launch
     shapes = [(256, 192, 512), (64, 64, 2048), (96, 320, 64), (1024, 64, 1024), (520, 772, 512),
               (200, 300, 100)]
     probs, views, bufs = make_group(shapes, "uniform", seed=3)
@@ -252,9 +240,8 @@ def test_grouped_graph_replays_match_eager(prec, tmp_path):
     path.write_text(THIN_SPLIT)
     plan = hb.GroupedPlan(probs, prec, tuning_file=str(path))
     info = plan.info()
-    big = [s for i, s in enumerate(shapes) if i not in (1, 5)]
-    odd = 1 * 2 if prec == hb.PREC_BF16X3 else 2 * 3
-    assert info["tiles"] == big_tiles(big, prec) + 1 + odd
+    big = [s for i, s in enumerate(shapes) if i != 1]
+    assert info["tiles"] == big_tiles(big) + 1
     assert info["split_tiles"] == 1 and info["work_items"] == info["tiles"] + 3
     plan.run()
     torch.cuda.synchronize()
@@ -323,9 +310,9 @@ def test_grouped_512_problem_workload(prec):
 @pytest.mark.parametrize("prec", [hb.PREC_FP32, hb.PREC_BF16X3])
 def test_grouped_many_small_problems_through_the_queues(prec, tmp_path):
     """300 problems of random shapes, a quarter with K not a multiple of 32
-    (the 256-thread launch in FP32, the zero-filling large-tile launch in
-    BF16x3), some thin and long in K on the 256-thread launch with split-K (a
-    tuning file): every launch runs, with thousands of queue fetches per run.
+    (the zero-filling large-tile launch), some thin and long in K on the
+    256-thread launch with split-K (a tuning file): every launch runs, with
+    thousands of queue fetches per run.
     Two graph replays must agree bit for bit with an eager run (the queues and
     split counters reset themselves) and every problem is within its FP64
     bound."""
@@ -396,6 +383,7 @@ def test_grouped_split_64_long_k(prec):
 
 @pytest.mark.parametrize("prec,ks", [(hb.PREC_FP32, (32, 64, 96, 128, 256)),
                                      (hb.PREC_BF16X3, (32, 64, 96, 128, 256)),
+                                     (hb.PREC_FP32, (4, 36, 68, 100, 260)),
                                      (hb.PREC_BF16X3, (4, 36, 68, 100, 260))])
 def test_grouped_chained_tiles_match_problems_run_alone(prec, ks):
     """The large-tile launch runs a workgroup's tiles back to back, loading the
@@ -404,7 +392,7 @@ def test_grouped_chained_tiles_match_problems_run_alone(prec, ks):
     about a thousand 256 x 256 tiles with K of 1, 2, 3, 4 and 8 or 9 stages
     and ragged M and N must give, bit for bit, what each problem gives in a
     plan of its own (one or a few tiles per workgroup).  The second K set (not
-    multiples of 32) runs on BF16x3's zero-filling large-tile launch."""
+    multiples of 32) runs on the zero-filling large-tile launch."""
     rng = np.random.default_rng(77)
     shapes = []
     for i in range(130):

@@ -24,7 +24,7 @@ pytestmark = pytest.mark.gpu
      "--golden", "f64"],
     ["--N", "2048", "--NITER", "2", "--input", "uniform", "--golden", "f64", "--reps", "3"],
     ["--N", "4096", "--NITER", "1", "--golden", "f64"],               # pattern via FP64 check
-    # the slices as one grouped plan (hipGraph): heuristic, forced tile and
+    # the slices as one grouped plan (hipGraph): default, forced tile and
     # split-K, the committed tuning file
     ["--N", "512", "--NITER", "2", "--mode", "grouped"],
     ["--N", "640", "--NITER", "3", "--mode", "grouped", "--precision", "bf16x3", "--tile", "0",

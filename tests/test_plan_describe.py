@@ -1,16 +1,13 @@
 """The grouped planner on the host (no GPU): hbs_grouped_plan_describe runs the
-same planning as hbs_grouped_plan_create -- size heuristic, tuning file or
-forced tile / split-K, work-item expansion, launch sizes -- without a device.
+same planning as hbs_grouped_plan_create -- default, tuning file or forced
+tile / split-K, work-item expansion, launch sizes -- without a device.
 Expected counts follow from the planning rules:
 
-  * a problem takes the large-tile launch (by default and with tile 4) when
-    its K is a multiple of 32, and in BF16x3 whatever its K: ceil(M/256)
-    ceil(N/256) tiles, no split-K.  BF16x3 problems whose K is not a
-    multiple of 32 form a second large-tile launch (zero fill past K);
-  * any other problem takes the 256-thread launch with the heuristic tile
-    (64 x 64 when M, N <= 64, 64 x 128 when M <= 64, 128 x 64 when N <= 64,
-    else 128 x 128) and split-K 4 (2) when it has at most 4 (16) tiles and
-    K >= 1024; a split K range is ceil(K / s) rounded up to the stage depth
+  * by default and with tile 4 a problem takes a large-tile launch:
+    ceil(M/256) ceil(N/256) tiles, no split-K; problems whose K is a multiple
+    of 32 form one launch, the others a second one (zero fill past K);
+  * with a forced tile 0..3 a problem takes the 256-thread launch with that
+    tile and split-K s: a K range is ceil(K / s) rounded up to the stage depth
     (16 for FP32, 32 for BF16x3), giving ceil(K / range) work items per tile;
   * workgroups: one per CU on each large-tile launch, two per CU on the other,
     never more than the launch's work items.
@@ -25,33 +22,17 @@ import hb_hammerbench_amd as hb
 TILE = {0: (128, 128), 1: (64, 64), 2: (128, 64), 3: (64, 128)}
 
 
-def heuristic_tile(M, N):
-    if M <= 64 and N <= 64:
-        return 1
-    if M <= 64:
-        return 3
-    if N <= 64:
-        return 2
-    return 0
-
-
 def expected(shapes, prec, cus=256, tile=None, split=1):
     stage = 16 if prec == hb.PREC_FP32 else 32
     big_tiles = [0, 0]  # large-tile launches: K a multiple of 32, other K
     small_tiles = small_items = split_tiles = 0
     for M, N, K in shapes:
-        big = (K % 32 == 0 or prec == hb.PREC_BF16X3) and (tile is None or tile == 4)
-        if big:
+        if tile is None or tile == 4:
             big_tiles[0 if K % 32 == 0 else 1] += math.ceil(M / 256) * math.ceil(N / 256)
             continue
-        t = heuristic_tile(M, N) if tile in (None, 4) else tile
-        BM, BN = TILE[t]
+        BM, BN = TILE[tile]
         n = math.ceil(M / BM) * math.ceil(N / BN)
-        if tile is None:
-            s = 4 if (n <= 4 and K >= 1024) else 2 if (n <= 16 and K >= 1024) else 1
-        else:
-            s = split
-        chunk = math.ceil(math.ceil(K / s) / stage) * stage
+        chunk = math.ceil(math.ceil(K / split) / stage) * stage
         ns = math.ceil(K / chunk)
         small_tiles += n
         small_items += n * ns
@@ -73,13 +54,12 @@ def check(info, want, count):
 
 @pytest.mark.parametrize("prec", [hb.PREC_FP32, hb.PREC_BF16X3])
 @pytest.mark.parametrize("cus", [256, 8])
-def test_describe_size_heuristic(prec, cus):
+def test_describe_default_plan(prec, cus):
     info = hb.describe_plan(MIXED, prec, cus=cus)
     want = expected(MIXED, prec, cus)
-    assert want["tiles"] > 20
-    # FP32 keeps the problems whose K is not a multiple of 32 on the 256-thread
-    # launch (some with split-K); BF16x3 runs every problem on large tiles
-    assert (want["split_tiles"] > 0) == (prec == hb.PREC_FP32)
+    # every problem on the two large-tile launches, no split-K
+    assert want["tiles"] > 20 and want["split_tiles"] == 0
+    assert want["workgroups"] == min(cus, 35) + min(cus, 8)  # tiles with K % 32 == 0, others
     check(info, want, len(MIXED))
 
 
