@@ -39,7 +39,7 @@ def parse():
                    help="grouped: write the seeded problem shapes (M N K per line) and exit")
     p.add_argument("--tuning", default=os.path.join(ROOT, "hb_hammerbench_amd", "tuning",
                                                     "mi355x_grouped.txt"),
-                   help="grouped: autotuner tuning file ('' for the size heuristic)")
+                   help="grouped: autotuner tuning file ('' for the default plan)")
     p.add_argument("--n", type=int, default=8192)
     p.add_argument("--niter", type=int, default=10)
     p.add_argument("--seed", type=int, default=20241)
