@@ -75,6 +75,11 @@ struct Options {
     int tile = -1;  // grouped: HBS_TILE_* for every problem, -1 = tuning file / default
     int split = 1;
     std::string tuning;
+    // host golden only: add `perturb_delta` to golden element `perturb_index`
+    // after host_mm, a mistake planted on the CPU side that the SSE rule must
+    // catch (or, below 0.1, tolerate) -- a check of the validation itself
+    int64_t perturb_index = -1;
+    float perturb_delta = 0.0f;
 };
 
 static void usage(const char *prog) {
@@ -82,7 +87,8 @@ static void usage(const char *prog) {
             "usage: %s [--N n] [--NITER k] [--precision fp32|bf16x3]\n"
             "          [--input pattern|uniform|loguniform] [--seed s]\n"
             "          [--golden host|f64|auto] [--reps r] [--max-print m]\n"
-            "          [--mode square|grouped] [--tile auto|0..4] [--split s] [--tuning file]\n",
+            "          [--mode square|grouped] [--tile auto|0..4] [--split s] [--tuning file]\n"
+            "          [--perturb-golden index:delta]\n",
             prog);
 }
 
@@ -103,6 +109,13 @@ static bool parse(int argc, char **argv, Options &o) {
         else if (a == "--tile" && (v = next())) o.tile = strcmp(v, "auto") == 0 ? -1 : atoi(v);
         else if (a == "--split" && (v = next())) o.split = atoi(v);
         else if (a == "--tuning" && (v = next())) o.tuning = v;
+        else if (a == "--perturb-golden" && (v = next())) {
+            const char *colon = strchr(v, ':');
+            if (!colon) return false;
+            o.perturb_index = strtoll(v, nullptr, 10);
+            o.perturb_delta = strtof(colon + 1, nullptr);
+            if (o.perturb_index < 0) return false;
+        }
         else return false;
     }
     if (o.mode != "square" && o.mode != "grouped") return false;
@@ -116,6 +129,9 @@ static bool parse(int argc, char **argv, Options &o) {
     // at sizes host_mm finishes quickly; the FP64 device check otherwise (the
     // absolute SSE < 0.01 rule does not fit inputs of arbitrary magnitude).
     if (o.golden == "auto") o.golden = (o.input == "pattern" && o.N <= 1024) ? "host" : "f64";
+    if (o.perturb_index >= 0 &&
+        (o.golden != "host" || o.perturb_index >= (int64_t)o.N * o.N * o.NITER))
+        return false;
     return o.golden == "host" || o.golden == "f64";
 }
 
@@ -172,6 +188,11 @@ int main(int argc, char **argv) {
         mat_result.resize(total);
         for (int i = 0; i < NITER; i++)
             host_mm(&mat_result[nn * i], &mat1[nn * i], &mat2[nn * i], N);
+        if (o.perturb_index >= 0) {
+            mat_result[o.perturb_index] += o.perturb_delta;
+            printf("golden[%lld] perturbed by %g\n", (long long)o.perturb_index,
+                   (double)o.perturb_delta);
+        }
     }
 
     // Grouped mode: one plan whose problems are the NITER slices, captured in

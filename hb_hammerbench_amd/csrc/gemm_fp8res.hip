@@ -339,6 +339,7 @@ struct Layout {
     size_t a8, ar, ea, b8, br, eb, total;  // byte offsets
 };
 
+// M, N, K <= hbs::kMaxDim, so the padded sizes fit in int
 Layout layout(int M, int N, int K) {
     Layout L;
     L.Mp = (int)round_up(M, kBM);
@@ -356,34 +357,33 @@ Layout layout(int M, int N, int K) {
     return L;
 }
 
-}  // namespace
-
-extern "C" size_t hbs_gemm_fp8res_workspace_size(int M, int N, int K) {
-    if (M < 1 || N < 1 || K < 1) return 0;
-    return layout(M, N, K).total;
+int check_args(const float *A, const float *B, const float *C, int M, int N, int K, int lda,
+               int ldb, int ldc) {
+    if (M < 1 || N < 1 || K < 1 || lda < K || ldb < N || ldc < N) return HBS_ERR_ARG;
+    if (!hbs::dims_ok(M, N, K, lda, ldb, ldc)) return HBS_ERR_ARG;
+    if (!A || !B || !C) return HBS_ERR_NULL;
+    return HBS_SUCCESS;
 }
 
-extern "C" int hbs_gemm_fp8res(const float *A, const float *B, float *C, int M, int N, int K,
-                               int lda, int ldb, int ldc, void *workspace,
-                               size_t workspace_bytes, void *stream) {
-    if (M < 1 || N < 1 || K < 1 || lda < K || ldb < N || ldc < N) return HBS_ERR_ARG;
-    if (!A || !B || !C) return HBS_ERR_NULL;
-    const Layout L = layout(M, N, K);
-    if (!workspace || workspace_bytes < L.total) return HBS_ERR_WORKSPACE;
-    if (!hbs::aligned16(workspace)) return HBS_ERR_ALIGN;
-    const int st = hbs::check_device();
-    if (st) return st;
-    hipStream_t s = (hipStream_t)stream;
-    char *ws = (char *)workspace;
-
+// The three launches of one product: quantise A, quantise B, GEMM.  `ws`
+// holds L.total bytes, 16-byte aligned; every byte the GEMM reads is written
+// by the quantisers first (padding rows and columns as zeros), so nothing in
+// it needs clearing beforehand.
+int run_fp8res(const float *A, const float *B, float *C, int M, int N, int K, int lda, int ldb,
+               int ldc, const Layout &L, char *ws, hipStream_t s) {
+    const int64_t qa_blocks = (int64_t)(L.Mp / kQT) * (L.Kp / kQT);
+    const int64_t qb_blocks = (int64_t)(L.Np / kQT) * (L.Kp / kQT);
+    const int64_t blocks = (int64_t)(L.Mp / kBM) * (L.Np / kBN);
+    if (qa_blocks > 0x7fffffffLL || qb_blocks > 0x7fffffffLL || blocks > 0x7fffffffLL)
+        return HBS_ERR_ARG;
     QuantArgs qa{A, M, K, lda, hbs::aligned16(A) && lda % 4 == 0, L.Mp, L.Kp,
                  (uint8_t *)(ws + L.a8), (uint16_t *)(ws + L.ar), (int *)(ws + L.ea)};
-    quantize_kernel<false><<<(L.Mp / kQT) * (L.Kp / kQT), 256, 0, s>>>(qa);
+    quantize_kernel<false><<<(unsigned)qa_blocks, 256, 0, s>>>(qa);
     int rc = hbs::launch_status();
     if (rc) return rc;
     QuantArgs qb{B, K, N, ldb, hbs::aligned16(B) && ldb % 4 == 0, L.Np, L.Kp,
                  (uint8_t *)(ws + L.b8), (uint16_t *)(ws + L.br), (int *)(ws + L.eb)};
-    quantize_kernel<true><<<(L.Np / kQT) * (L.Kp / kQT), 256, 0, s>>>(qb);
+    quantize_kernel<true><<<(unsigned)qb_blocks, 256, 0, s>>>(qb);
     rc = hbs::launch_status();
     if (rc) return rc;
 
@@ -391,11 +391,30 @@ extern "C" int hbs_gemm_fp8res(const float *A, const float *B, float *C, int M, 
                (const uint16_t *)(ws + L.ar), (const uint16_t *)(ws + L.br),
                (const int *)(ws + L.ea), (const int *)(ws + L.eb), C, M, N, L.Kp, ldc,
                L.Mp / kBM, L.Np / kBN};
-    const int64_t blocks = (int64_t)g.tiles_m * g.tiles_n;
-    if (blocks > 0x7fffffffLL) return HBS_ERR_ARG;
     if (M % kBM == 0 && N % kBN == 0)
         gemm_fp8res_kernel<false><<<(unsigned)blocks, kThreads, 0, s>>>(g);
     else
         gemm_fp8res_kernel<true><<<(unsigned)blocks, kThreads, 0, s>>>(g);
     return hbs::launch_status();
+}
+
+}  // namespace
+
+extern "C" size_t hbs_gemm_fp8res_workspace_size(int M, int N, int K) {
+    if (M < 1 || N < 1 || K < 1) return 0;
+    if (M > hbs::kMaxDim || N > hbs::kMaxDim || K > hbs::kMaxDim) return 0;
+    return layout(M, N, K).total;
+}
+
+extern "C" int hbs_gemm_fp8res(const float *A, const float *B, float *C, int M, int N, int K,
+                               int lda, int ldb, int ldc, void *workspace,
+                               size_t workspace_bytes, void *stream) {
+    int st = check_args(A, B, C, M, N, K, lda, ldb, ldc);
+    if (st) return st;
+    const Layout L = layout(M, N, K);
+    if (!workspace || workspace_bytes < L.total) return HBS_ERR_WORKSPACE;
+    if (!hbs::aligned16(workspace)) return HBS_ERR_ALIGN;
+    st = hbs::check_device();
+    if (st) return st;
+    return run_fp8res(A, B, C, M, N, K, lda, ldb, ldc, L, (char *)workspace, (hipStream_t)stream);
 }

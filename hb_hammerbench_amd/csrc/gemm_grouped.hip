@@ -35,6 +35,7 @@
 #include <algorithm>
 #include <fstream>
 #include <map>
+#include <mutex>
 #include <queue>
 #include <sstream>
 #include <string>
@@ -68,6 +69,7 @@ struct hbs_grouped_plan {
     const int *d_qoff_big[2];  // item range of each of its nq_big queues
     int nq_big[2];
     int *d_queue_big[2];       // nq_big heads and a finished-workgroup count
+    int graphs;                // executable graphs made from this plan and not yet destroyed
 };
 
 namespace {
@@ -428,6 +430,7 @@ bool tile_supported(int precision, int tile) {
 int validate(const hbs_gemm_problem &q) {
     if (q.M < 1 || q.N < 1 || q.K < 1) return HBS_ERR_ARG;
     if (q.lda < q.K || q.ldb < q.N || q.ldc < q.N) return HBS_ERR_ARG;
+    if (!hbs::dims_ok(q.M, q.N, q.K, q.lda, q.ldb, q.ldc)) return HBS_ERR_ARG;
     if (!q.A || !q.B || !q.C) return HBS_ERR_NULL;
     if (!hbs::aligned16(q.A) || !hbs::aligned16(q.B)) return HBS_ERR_ALIGN;
     if (q.N % 4 || q.K % 4 || q.lda % 4 || q.ldb % 4) return HBS_ERR_ALIGN;
@@ -521,9 +524,9 @@ int plan_host(const hbs_gemm_problem *problems, int count, int precision, const 
         const int BM = kTileBM[c.tile], BN = kTileBN[c.tile];
         const int BK = precision == HBS_PREC_FP32 ? 16 : 32;
         const int tm_n = (q.M + BM - 1) / BM, tn_n = (q.N + BN - 1) / BN;
-        int chunk = (q.K + c.split - 1) / c.split;
-        chunk = (chunk + BK - 1) / BK * BK;
-        const int ns = (q.K + chunk - 1) / chunk;
+        // K range per split, a whole number of stages (int64: 2K may exceed int)
+        const int chunk = (int)(hbs::ceil_div(hbs::ceil_div(q.K, c.split), BK) * BK);
+        const int ns = (int)hbs::ceil_div(q.K, chunk);
         // Work-item cost in ns of one workgroup slot: the autotuner's measured
         // time when the tuning file has this class, else a throughput model
         // (FP32: 157.3 TFLOP/s, BF16x3: 2516.6 / 3 TFLOP/s-equivalent, shared
@@ -540,7 +543,7 @@ int plan_host(const hbs_gemm_problem *problems, int count, int precision, const 
                     it.tm = tm;
                     it.tn = tn;
                     it.k0 = s * chunk;
-                    it.k1 = std::min(q.K, (s + 1) * chunk);
+                    it.k1 = (int)std::min<int64_t>(q.K, (int64_t)(s + 1) * chunk);
                     it.cfg = c.tile;
                     it.nsplit = ns;
                     it.split = s;
@@ -863,8 +866,18 @@ extern "C" int hbs_grouped_run(const hbs_grouped_plan *plan, void *stream) {
     return HBS_SUCCESS;
 }
 
+// Executable graphs made by hbs_grouped_graph_create and the plan each one
+// replays: a plan cannot be destroyed while one of them exists (its queues,
+// counters and partials are baked into the graph's kernel arguments).
+std::mutex graph_mu;
+std::map<void *, hbs_grouped_plan *> graph_plans;
+
 extern "C" int hbs_grouped_plan_destroy(hbs_grouped_plan *plan) {
     if (!plan) return HBS_SUCCESS;
+    {
+        std::lock_guard<std::mutex> lk(graph_mu);
+        if (plan->graphs > 0) return HBS_ERR_IN_USE;
+    }
     hipError_t e = hipFree(plan->dmem);
     delete plan;
     return e == hipSuccess ? HBS_SUCCESS : (int)e;
@@ -889,6 +902,11 @@ extern "C" int hbs_grouped_graph_create(const hbs_grouped_plan *plan, void **gra
     (void)hipStreamDestroy(s);
     if (rc != HBS_SUCCESS) return rc;
     if (e != hipSuccess) return (int)e;
+    {
+        std::lock_guard<std::mutex> lk(graph_mu);
+        graph_plans[(void *)ex] = const_cast<hbs_grouped_plan *>(plan);
+        ++const_cast<hbs_grouped_plan *>(plan)->graphs;
+    }
     *graph_exec = (void *)ex;
     return HBS_SUCCESS;
 }
@@ -902,5 +920,11 @@ extern "C" int hbs_graph_launch(void *graph_exec, void *stream) {
 extern "C" int hbs_graph_destroy(void *graph_exec) {
     if (!graph_exec) return HBS_SUCCESS;
     HBS_HIP_TRY(hipGraphExecDestroy((hipGraphExec_t)graph_exec));
+    std::lock_guard<std::mutex> lk(graph_mu);
+    auto f = graph_plans.find(graph_exec);
+    if (f != graph_plans.end()) {
+        --f->second->graphs;
+        graph_plans.erase(f);
+    }
     return HBS_SUCCESS;
 }

@@ -49,13 +49,14 @@ extern "C" const char *hbs_strerror(int code) {
         case HBS_ERR_WORKSPACE: return "workspace missing or too small";
         case HBS_ERR_DEVICE: return "current device is not a gfx950 GPU";
         case HBS_ERR_UNSUPPORTED: return "request not supported by this build";
+        case HBS_ERR_IN_USE: return "grouped plan still has executable graphs";
         default: break;
     }
     if (code > 0) return hipGetErrorString((hipError_t)code);
     return "unknown status code";
 }
 
-extern "C" int hbs_version(void) { return 300; }
+extern "C" int hbs_version(void) { return 400; }
 
 // ---------------------------------------------------------------------------
 // apps/sgemm/main.cpp:46-52 -- mat1[i] = i % 7, mat2[i] = i % 3.

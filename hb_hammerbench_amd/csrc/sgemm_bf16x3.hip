@@ -84,6 +84,7 @@ int run_bf16x3(const float *A, const float *B, float *C, int M, int N, int K, in
 extern "C" int hbs_gemm_bf16x3(const float *A, const float *B, float *C, int M, int N, int K,
                                int lda, int ldb, int ldc, void *stream) {
     if (M < 1 || N < 1 || K < 1 || lda < K || ldb < N || ldc < N) return HBS_ERR_ARG;
+    if (!hbs::dims_ok(M, N, K, lda, ldb, ldc)) return HBS_ERR_ARG;
     if (!A || !B || !C) return HBS_ERR_NULL;
     const int st = hbs::check_device();
     if (st) return st;
@@ -92,7 +93,7 @@ extern "C" int hbs_gemm_bf16x3(const float *A, const float *B, float *C, int M, 
 
 extern "C" int hbs_sgemm_bf16x3(const float *A, const float *B, float *C, int N, int niter,
                                 void *stream) {
-    if (N < 1 || niter < 1) return HBS_ERR_ARG;
+    if (N < 1 || niter < 1 || N > hbs::kMaxDim) return HBS_ERR_ARG;
     const int64_t nn = (int64_t)N * N;
     if (nn > ((int64_t)1 << 62) / niter) return HBS_ERR_ARG;
     if (!A || !B || !C) return HBS_ERR_NULL;

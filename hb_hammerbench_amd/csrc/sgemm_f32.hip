@@ -80,6 +80,7 @@ extern "C" int hbs_sgemm_strided(const float *A, const float *B, float *C, int M
                                  int64_t strideC, int batch, void *stream) {
     if (M < 1 || N < 1 || K < 1 || batch < 1) return HBS_ERR_ARG;
     if (lda < K || ldb < N || ldc < N) return HBS_ERR_ARG;
+    if (!hbs::dims_ok(M, N, K, lda, ldb, ldc)) return HBS_ERR_ARG;
     if (strideA < 0 || strideB < 0 || strideC < 0) return HBS_ERR_ARG;
     if (!A || !B || !C) return HBS_ERR_NULL;
     const int st = hbs::check_device();

@@ -114,6 +114,7 @@ extern "C" int hbs_verify_f64(const float *A, const float *B, const float *C, in
                               int lda, int ldb, int ldc, int row_begin, int row_end,
                               hbs_errstats *out, void *stream) {
     if (M < 1 || N < 1 || K < 1 || lda < K || ldb < N || ldc < N) return HBS_ERR_ARG;
+    if (!hbs::dims_ok(M, N, K, lda, ldb, ldc)) return HBS_ERR_ARG;
     if (row_begin < 0 || row_end > M || row_begin >= row_end) return HBS_ERR_ARG;
     if (!A || !B || !C || !out) return HBS_ERR_NULL;
     const int st = hbs::check_device();

@@ -19,7 +19,8 @@
  *    invalid argument (nothing is launched), or a positive HIP runtime error
  *    code -- the counterpart of HB_MC_SUCCESS / BSG_CUDA_CALL (main.cpp:61-148);
  *  - no entry point allocates device memory on the GEMM path; scratch space
- *    is passed in as a workspace whose size the *_workspace_size call returns;
+ *    is passed in as a workspace whose size the *_workspace_size call returns
+ *    (grouped plans allocate once, when they are created);
  *  - calls on distinct streams are independent; calls sharing a workspace
  *    must be ordered by the caller.
  */
@@ -40,8 +41,13 @@ enum {
     HBS_ERR_ALIGN = -3,       /* pointer or leading dimension misaligned */
     HBS_ERR_WORKSPACE = -4,   /* workspace missing or too small */
     HBS_ERR_DEVICE = -5,      /* current device is not a gfx950 GPU */
-    HBS_ERR_UNSUPPORTED = -6  /* valid request this build does not handle */
+    HBS_ERR_UNSUPPORTED = -6, /* valid request this build does not handle */
+    HBS_ERR_IN_USE = -8       /* grouped plan still replayed by a graph */
 };
+
+/* Every entry point rejects M, N, K or a leading dimension above
+ * HBS_MAX_DIM with HBS_ERR_ARG (tile arithmetic is done in 32-bit ints). */
+#define HBS_MAX_DIM (0x7fffffff - 4096)
 
 /* Human-readable text for any status code (never NULL). */
 const char *hbs_strerror(int code);
@@ -158,10 +164,17 @@ int hbs_grouped_plan_describe(const hbs_gemm_problem *problems, int count, int p
  * read the operands in place.  Deterministic; runs of one plan must not
  * overlap (they share the queues, split-K counters and partial buffers). */
 int hbs_grouped_run(const hbs_grouped_plan *plan, void *stream);
+/* Frees the plan's device buffers (synchronous).  Returns HBS_ERR_IN_USE and
+ * frees nothing while a graph made from the plan by hbs_grouped_graph_create
+ * has not been destroyed with hbs_graph_destroy. */
 int hbs_grouped_plan_destroy(hbs_grouped_plan *plan);
 
 /* Captures one hbs_grouped_run of `plan` into an executable hipGraph
- * (returned as hipGraphExec_t in *graph_exec); replay with hbs_graph_launch. */
+ * (returned as hipGraphExec_t in *graph_exec); replay with hbs_graph_launch.
+ * The graph holds the plan's device buffers (problem table, queues, split-K
+ * counters and partials), so the plan must outlive it: destroy every graph of
+ * a plan before the plan.  Replays of one plan's graphs, and hbs_grouped_run
+ * of that plan, must not overlap. */
 int hbs_grouped_graph_create(const hbs_grouped_plan *plan, void **graph_exec);
 int hbs_graph_launch(void *graph_exec, void *stream);
 int hbs_graph_destroy(void *graph_exec);

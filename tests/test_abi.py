@@ -39,8 +39,8 @@ def test_python_binding_covers_every_declared_symbol():
 
 def test_strerror_and_version():
     L = hb.lib()
-    assert L.hbs_version() == 300  # 0.3.0: hbs_grouped_plan_describe
-    for code in [0, -1, -2, -3, -4, -5, -6]:
+    assert L.hbs_version() == 400  # 0.4.0: HBS_MAX_DIM, HBS_ERR_IN_USE
+    for code in [0, -1, -2, -3, -4, -5, -6, -8]:
         assert L.hbs_strerror(code)
     assert L.hbs_strerror(-99) == b"unknown status code"
 
@@ -146,3 +146,19 @@ def test_fp8res_argument_checks():
     assert L.hbs_gemm_fp8res(P, P, P, 8, 8, 8, 8, 8, 8, None, 1 << 30, None) == -4
     assert L.hbs_gemm_fp8res(P, P, P, 8, 8, 8, 8, 8, 8, P, 16, None) == -4
     assert L.hbs_gemm_fp8res(P, P, P, 8, 8, 8, 8, 8, 8, ctypes.c_void_p(8), 1 << 30, None) == -3
+
+
+def test_square_and_rectangular_entries_reject_sizes_beyond_the_int_limit():
+    """M, N, K or a leading dimension above HBS_MAX_DIM would overflow the
+    kernels' 32-bit tile arithmetic: HBS_ERR_ARG before any device call."""
+    L = hb.lib()
+    big = 0x7fffffff - 4096 + 1
+    assert L.hbs_sgemm(P, P, P, big, 1, None) == -1
+    assert L.hbs_sgemm_strided(P, P, P, big, 8, 8, 8, 8, 8, 0, 0, 0, 1, None) == -1
+    assert L.hbs_sgemm_strided(P, P, P, 8, 8, 8, 8, 8, big, 0, 0, 0, 1, None) == -1
+    assert L.hbs_sgemm_bf16x3(P, P, P, big, 1, None) == -1
+    assert L.hbs_gemm_bf16x3(P, P, P, 8, 8, big, big, 8, 8, None) == -1
+    assert L.hbs_gemm_fp8res(P, P, P, 8, big, 8, 8, big, big, P, 1 << 30, None) == -1
+    assert L.hbs_gemm_fp8res_workspace_size(8, 8, big) == 0
+    out = hb.ErrStats()
+    assert L.hbs_verify_f64(P, P, P, big, 8, 8, 8, 8, 8, 0, 8, ctypes.byref(out), None) == -1

@@ -231,3 +231,62 @@ def test_bf16x3_fp32_level_error_at_N8192_loguniform():
           f"e_F fp32={e32['e_F']:.3e} bf16x3={e3['e_F']:.3e}", flush=True)
     assert e32["nonfinite"] == 0 and e3["nonfinite"] == 0
     assert e3["e_cw"] <= 2.0 * e32["e_cw"]
+
+
+def pattern_closed_form_gpu(N, it):
+    """Closed form of the main.cpp:46-52 product as a rank-21 product of small
+    integer matrices (exact in float64): C = P . Q with P[y][r] = cnt(r) *
+    ((o + N y + r) mod 7) and Q[r][x] = ((o + x) + (N mod 3) r) mod 3."""
+    o = it * N * N
+    r = torch.arange(21, dtype=torch.int64, device="cuda")
+    cnt = N // 21 + (r < N % 21).to(torch.int64)
+    y = torch.arange(N, dtype=torch.int64, device="cuda")
+    P = cnt[None, :] * ((o + N * y[:, None] + r[None, :]) % 7)
+    Q = ((o + y[None, :]) + (N % 3) * r[:, None]) % 3
+    return (P.double() @ Q.double()).float()
+
+
+def test_bf16x3_full_size_launch_pattern_and_loguniform():
+    """The launch the bench times: N = 8192, NITER = 10 slices in one call.
+    Reference pattern: every slice bit-exact against the closed form (values
+    0..6 and 0..2 are exact in bf16, so lo = 0 and every product is exact).
+    Log-uniform inputs (|x| in [1e-3, 1e3], random sign): the last slice of
+    the same 10-slice launch within the split bound of the FP64 oracle on a
+    row band, and its e_cw over the whole slice no worse than 2x the FP32
+    kernel's on the same batch."""
+    N, NITER = 8192, 10
+    n = NITER * N * N
+    a = dev_empty(n); b = dev_empty(n); c = dev_empty(n)
+    hb.fill_pattern(a, b, n)
+    hb.sgemm_bf16x3(a, b, c, N, NITER)
+    torch.cuda.synchronize()
+    C = c.view(NITER, N, N)
+    for it in range(NITER):
+        assert torch.equal(C[it], pattern_closed_form_gpu(N, it)), it
+    for (it, y, x) in [(0, 0, 0), (9, 8191, 8191), (6, 4321, 777)]:
+        assert C[it, y, x].item() == oracle.pattern_value(N, it, y, x)
+
+    hb.fill_random(a, n, hb.GEN_LOGUNIFORM, 51)
+    hb.fill_random(b, n, hb.GEN_LOGUNIFORM, 52)
+    hb.sgemm_bf16x3(a, b, c, N, NITER)
+    torch.cuda.synchronize()
+    it, r0, r1 = NITER - 1, 6000, 6032
+    Av, Bv = a.view(NITER, N, N)[it], b.view(NITER, N, N)[it]
+    A = Av[r0:r1].cpu().numpy()
+    B = Bv.cpu().numpy()
+    R, S = oracle.gemm_f64(A, B)
+    Cs = C[it, r0:r1].cpu().numpy()
+    assert np.all(np.isfinite(Cs))
+    assert np.all(np.abs(Cs - R) <= (3.1 * 2.0 ** -16 + gamma(3 * N + 2)) * S)
+    band = hb.verify_f64(Av, Bv, C[it], N, N, N, row_begin=r0, row_end=r1)
+    assert band["e_cw"] == pytest.approx(oracle.errors(Cs, R, S)["e_cw"], rel=1e-9)
+    e3 = hb.verify_f64(Av, Bv, C[it], N, N, N)  # the whole last slice
+    # the FP32 kernel on the same 10-slice batch, same last slice
+    hb.sgemm(a, b, c, N, NITER)
+    torch.cuda.synchronize()
+    e32 = hb.verify_f64(Av, Bv, C[it], N, N, N)
+    print(f"\nN=8192 NITER=10 last slice: e_cw bf16x3={e3['e_cw']:.3e} fp32={e32['e_cw']:.3e} "
+          f"e_nw {e3['e_nw']:.3e}/{e32['e_nw']:.3e} e_F {e3['e_F']:.3e}/{e32['e_F']:.3e}",
+          flush=True)
+    assert e3["nonfinite"] == 0 and e32["nonfinite"] == 0
+    assert e3["e_cw"] <= 2.0 * e32["e_cw"]

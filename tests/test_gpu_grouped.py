@@ -421,3 +421,28 @@ def test_grouped_chained_tiles_match_problems_run_alone(prec, ks):
         else:
             bound = (3.1 * 2.0 ** -16 + gamma(3 * K + 2)) * S
         assert np.all(np.abs(C.cpu().numpy() - R) <= bound), (M, N, K)
+
+
+def test_plan_outlives_its_graphs():
+    """A plan with a live graph refuses to be destroyed (HBS_ERR_IN_USE, its
+    device buffers stay valid and the graph still replays correctly); once the
+    graph is destroyed the plan can be."""
+    import ctypes
+    shapes = [(128, 96, 64), (64, 256, 96)]
+    probs, views, keep = make_group(shapes, "pattern")
+    plan = hb.GroupedPlan(probs, hb.PREC_FP32)
+    g = plan.graph()
+    L = hb.lib()
+    assert L.hbs_grouped_plan_destroy(plan.handle) == -8
+    g.launch()
+    torch.cuda.synchronize()
+    for (A, B, C), (M, N, K) in zip(views, shapes):
+        R = A.double().cpu().numpy() @ B.double().cpu().numpy()
+        assert np.array_equal(C.cpu().numpy(), R.astype(np.float32))
+    h2 = ctypes.c_void_p()
+    assert L.hbs_grouped_graph_create(plan.handle, ctypes.byref(h2)) == 0
+    g.close()
+    assert L.hbs_grouped_plan_destroy(plan.handle) == -8  # the second graph is live
+    assert L.hbs_graph_destroy(h2) == 0
+    plan.close()
+    assert plan.handle is None

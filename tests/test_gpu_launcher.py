@@ -61,3 +61,25 @@ def test_make_generate_and_exec_reference_test(tmp_path):
     log = (tmp_path / name / "exec.log").read_text()
     assert "mode=grouped" in log and "tile=0 split=2" in log
     assert "HBS REGRESSION TEST PASSED" in log
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+def test_launcher_fails_on_a_planted_golden_mismatch(precision):
+    """The validation of main.cpp:120-134 itself: one golden element moved by
+    0.5 on the host gives SSE = 0.25 >= 0.01, so the launcher must print the
+    mismatch, end in HBS REGRESSION TEST FAILED and exit with status 1; moved by
+    0.05 (SSE = 0.0025) the element still differs but the rule passes."""
+    base = [LAUNCHER, "--N", "256", "--NITER", "2", "--precision", precision]
+    r = subprocess.run(base + ["--perturb-golden", "70000:0.5"], capture_output=True, text=True,
+                       timeout=240)
+    print(r.stdout[-600:])
+    assert r.returncode == 1, r.stdout[-2000:] + r.stderr[-2000:]
+    out = r.stdout.strip().splitlines()
+    assert out[-1] == "HBS REGRESSION TEST FAILED"
+    assert "elements differing: 1, SSE= 0.250000" in r.stdout
+    assert "Matrix Mismatch" in r.stdout and "[70000] actual=" in r.stdout
+    r = subprocess.run(base + ["--perturb-golden", "70000:0.05"], capture_output=True, text=True,
+                       timeout=240)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "elements differing: 1," in r.stdout
+    assert r.stdout.strip().splitlines()[-1] == "HBS REGRESSION TEST PASSED"

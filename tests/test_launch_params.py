@@ -74,3 +74,20 @@ LAUNCHER = os.path.join(APP, "sgemm")
 def test_launcher_rejects_bad_arguments(args):
     r = subprocess.run([LAUNCHER] + args, capture_output=True, text=True, timeout=60)
     assert r.returncode == 2 and "usage" in r.stderr
+
+
+def test_launcher_rejects_bad_golden_perturbation():
+    """--perturb-golden needs index:delta, the host golden and an index inside
+    the NITER slices; anything else is a usage error (status 2) before any
+    device call."""
+    import subprocess
+    launcher = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                            "hb_hammerbench_amd", "apps", "sgemm", "sgemm")
+    if not os.path.exists(launcher):
+        pytest.skip("launcher not built")
+    for extra in (["--perturb-golden", "5"], ["--golden", "f64", "--perturb-golden", "5:1"],
+                  ["--N", "4", "--NITER", "1", "--perturb-golden", "16:1"],
+                  ["--perturb-golden", "-1:1"]):
+        r = subprocess.run([launcher] + extra, capture_output=True, text=True, timeout=60)
+        assert r.returncode == 2, extra
+        assert "usage:" in r.stderr

@@ -114,3 +114,30 @@ def test_describe_random_groups(seed):
         split = int(rng.integers(1, 9)) if tile < 4 else 1
         check(hb.describe_plan(shapes, prec, tile=tile, split_k=split, cus=cus),
               expected(shapes, prec, cus, tile=tile, split=split), len(shapes))
+
+
+MAX_DIM = 0x7fffffff - 4096  # HBS_MAX_DIM
+
+
+@pytest.mark.parametrize("shape", [(MAX_DIM + 1, 4, 4), (4, MAX_DIM + 4, 4),
+                                   (4, 4, 0x7fffffff - 3), (0x7fffffff, 4, 4)])
+def test_sizes_beyond_the_int_limit_are_rejected(shape):
+    """Sizes whose tile or split arithmetic would overflow 32-bit ints get
+    HBS_ERR_ARG instead of a plan with too few tiles."""
+    for tile in (None, 0, 4):
+        with pytest.raises(hb.HbsError) as ei:
+            hb.describe_plan([shape], tile=tile, split_k=1 if tile in (None, 4) else 4)
+        assert ei.value.code == -1, (shape, tile)
+
+
+@pytest.mark.parametrize("split", [1, 2, 4, 64])
+def test_longest_accepted_k_splits_into_whole_ranges(split):
+    """K at the limit (a multiple of 4): the split ranges are computed without
+    overflow, so a thin problem still gets exactly its split count of items."""
+    K = MAX_DIM // 4 * 4
+    for prec in (hb.PREC_FP32, hb.PREC_BF16X3):
+        info = hb.describe_plan([(64, 64, K)], precision=prec, tile=1, split_k=split)
+        exp = expected([(64, 64, K)], prec, tile=1, split=split)
+        assert info["tiles"] == 1
+        assert info["work_items"] == exp["work_items"] == split
+        assert info["split_tiles"] == (1 if split > 1 else 0)
