@@ -18,6 +18,7 @@ __all__ = [
     "fill_pattern", "fill_random", "verify_f64",
     "GEN_UNIFORM", "GEN_LOGUNIFORM", "GEN_NORMAL_OUTLIER", "PREC_FP32", "PREC_BF16X3",
     "GroupedPlan", "Graph", "gemm_fp8res", "fp8res_workspace_size", "gemm_bf16x3",
+    "gemm_fp8res_noalloc", "FP8RES_NOALLOC_SCRATCH_BYTES",
     "describe_plan",
 ]
 
@@ -91,6 +92,7 @@ SIGNATURES = {
     "hbs_gemm_bf16x3": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _vp]),
     "hbs_gemm_fp8res_workspace_size": (_sz, [_i, _i, _i]),
     "hbs_gemm_fp8res": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _vp, _sz, _vp]),
+    "hbs_gemm_fp8res_noalloc": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _vp]),
     "hbs_fill_pattern": (_i, [_vp, _vp, _i64, _i64, _vp]),
     "hbs_fill_random": (_i, [_vp, _i64, _i, _u64, _u64, _vp]),
     "hbs_verify_f64": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _i,
@@ -180,6 +182,21 @@ def gemm_fp8res(A, B, C, M, N, K, lda=None, ldb=None, ldc=None, workspace=None, 
                                  _stream(stream)),
            "hbs_gemm_fp8res")
     return workspace
+
+
+# Bytes of the fixed scratch of hbs_gemm_fp8res_noalloc (HBS_FP8RES_NOALLOC_SCRATCH_BYTES).
+FP8RES_NOALLOC_SCRATCH_BYTES = 64 << 20
+
+
+def gemm_fp8res_noalloc(A, B, C, M, N, K, lda=None, ldb=None, ldc=None, stream=None):
+    """gemm_fp8res with the workspace in the library's fixed scratch: no device
+    allocation (problems whose workspace exceeds FP8RES_NOALLOC_SCRATCH_BYTES
+    raise HbsError with HBS_ERR_SCRATCH).  Calls on one GPU share that scratch
+    and must not overlap."""
+    _check(lib().hbs_gemm_fp8res_noalloc(_ptr(A), _ptr(B), _ptr(C), M, N, K,
+                                         K if lda is None else lda, N if ldb is None else ldb,
+                                         N if ldc is None else ldc, _stream(stream)),
+           "hbs_gemm_fp8res_noalloc")
 
 
 def fill_pattern(mat1, mat2, n, first=0, stream=None):

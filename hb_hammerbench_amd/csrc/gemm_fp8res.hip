@@ -20,6 +20,8 @@
 // bytes per element.  The bf16 copy of q that the residual products need is
 // rebuilt from the e4m3 bytes in registers (v_cvt_scalef32_pk_bf16_fp8, exact:
 // q has four significant bits and the scale is a power of two).
+#include <mutex>
+
 #include "hbs_internal.h"
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -398,6 +400,33 @@ int run_fp8res(const float *A, const float *B, float *C, int M, int N, int K, in
     return hbs::launch_status();
 }
 
+// Scratch of the allocation-free entry point: the workspace of one product of
+// up to HBS_FP8RES_NOALLOC_SCRATCH_BYTES, held by each GPU the library runs
+// on until the process ends.  In host code `scratch` names a host-side
+// stand-in, not device memory: kernels get its device address, looked up once
+// per device with hipGetSymbolAddress.
+static __device__ __align__(256) uint8_t scratch[HBS_FP8RES_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+char *scratch_ptr[kMaxDevices];
+
+// Device address of the current device's scratch (looked up on first use).
+int scratch_address(char **out) {
+    int dev = 0;
+    HBS_HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return HBS_ERR_DEVICE;
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    if (!scratch_ptr[dev]) {
+        void *p = nullptr;
+        HBS_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p || !hbs::aligned16(p)) return HBS_ERR_DEVICE;
+        scratch_ptr[dev] = (char *)p;
+    }
+    *out = scratch_ptr[dev];
+    return HBS_SUCCESS;
+}
+
 }  // namespace
 
 extern "C" size_t hbs_gemm_fp8res_workspace_size(int M, int N, int K) {
@@ -417,4 +446,18 @@ extern "C" int hbs_gemm_fp8res(const float *A, const float *B, float *C, int M, 
     st = hbs::check_device();
     if (st) return st;
     return run_fp8res(A, B, C, M, N, K, lda, ldb, ldc, L, (char *)workspace, (hipStream_t)stream);
+}
+
+extern "C" int hbs_gemm_fp8res_noalloc(const float *A, const float *B, float *C, int M, int N,
+                                       int K, int lda, int ldb, int ldc, void *stream) {
+    int st = check_args(A, B, C, M, N, K, lda, ldb, ldc);
+    if (st) return st;
+    const Layout L = layout(M, N, K);
+    if (L.total > (size_t)HBS_FP8RES_NOALLOC_SCRATCH_BYTES) return HBS_ERR_SCRATCH;
+    st = hbs::check_device();
+    if (st) return st;
+    char *ws = nullptr;
+    st = scratch_address(&ws);
+    if (st) return st;
+    return run_fp8res(A, B, C, M, N, K, lda, ldb, ldc, L, ws, (hipStream_t)stream);
 }

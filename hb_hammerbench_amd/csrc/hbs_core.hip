@@ -49,6 +49,8 @@ extern "C" const char *hbs_strerror(int code) {
         case HBS_ERR_WORKSPACE: return "workspace missing or too small";
         case HBS_ERR_DEVICE: return "current device is not a gfx950 GPU";
         case HBS_ERR_UNSUPPORTED: return "request not supported by this build";
+        case HBS_ERR_SCRATCH:
+            return "problem exceeds the fixed scratch of an allocation-free entry point";
         case HBS_ERR_IN_USE: return "grouped plan still has executable graphs";
         default: break;
     }

@@ -20,7 +20,8 @@
  *    code -- the counterpart of HB_MC_SUCCESS / BSG_CUDA_CALL (main.cpp:61-148);
  *  - no entry point allocates device memory on the GEMM path; scratch space
  *    is passed in as a workspace whose size the *_workspace_size call returns
- *    (grouped plans allocate once, when they are created);
+ *    (grouped plans allocate once, when they are created, and
+ *    hbs_gemm_fp8res_noalloc uses a fixed scratch array in the library);
  *  - calls on distinct streams are independent; calls sharing a workspace
  *    must be ordered by the caller.
  */
@@ -42,6 +43,8 @@ enum {
     HBS_ERR_WORKSPACE = -4,   /* workspace missing or too small */
     HBS_ERR_DEVICE = -5,      /* current device is not a gfx950 GPU */
     HBS_ERR_UNSUPPORTED = -6, /* valid request this build does not handle */
+    HBS_ERR_SCRATCH = -7,     /* problem needs more than an allocation-free entry
+                                 point's fixed scratch */
     HBS_ERR_IN_USE = -8       /* grouped plan still replayed by a graph */
 };
 
@@ -198,6 +201,21 @@ size_t hbs_gemm_fp8res_workspace_size(int M, int N, int K);
  * ldb >= N, ldc >= N) by the scheme above. */
 int hbs_gemm_fp8res(const float *A, const float *B, float *C, int M, int N, int K, int lda,
                     int ldb, int ldc, void *workspace, size_t workspace_bytes, void *stream);
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): hbs_gemm_fp8res with its workspace in a fixed scratch
+ * array inside the library instead of a caller buffer.  Takes problems whose
+ * hbs_gemm_fp8res_workspace_size(M, N, K) is at most
+ * HBS_FP8RES_NOALLOC_SCRATCH_BYTES (64 MiB, e.g. M = N = 4096 with K <= 2688)
+ * and returns HBS_ERR_SCRATCH for larger ones.  Calls no allocating routine;
+ * each GPU the library runs on holds its own copy of the scratch until the
+ * process ends.  Every call on one GPU uses that GPU's one scratch, so calls
+ * that overlap on one GPU (different streams, or host threads) race and must be
+ * ordered by the caller.  Results are identical to hbs_gemm_fp8res's. */
+#define HBS_FP8RES_NOALLOC_SCRATCH_BYTES (64u << 20)
+int hbs_gemm_fp8res_noalloc(const float *A, const float *B, float *C, int M, int N, int K,
+                            int lda, int ldb, int ldc, void *stream);
 
 /* ------------------------------------------------------------------------ */
 /* Input generation on the device.                                            */

@@ -39,8 +39,8 @@ def test_python_binding_covers_every_declared_symbol():
 
 def test_strerror_and_version():
     L = hb.lib()
-    assert L.hbs_version() == 400  # 0.4.0: HBS_MAX_DIM, HBS_ERR_IN_USE
-    for code in [0, -1, -2, -3, -4, -5, -6, -8]:
+    assert L.hbs_version() == 400  # 0.4.0: hbs_gemm_fp8res_noalloc
+    for code in [0, -1, -2, -3, -4, -5, -6, -7, -8]:
         assert L.hbs_strerror(code)
     assert L.hbs_strerror(-99) == b"unknown status code"
 
@@ -146,6 +146,21 @@ def test_fp8res_argument_checks():
     assert L.hbs_gemm_fp8res(P, P, P, 8, 8, 8, 8, 8, 8, None, 1 << 30, None) == -4
     assert L.hbs_gemm_fp8res(P, P, P, 8, 8, 8, 8, 8, 8, P, 16, None) == -4
     assert L.hbs_gemm_fp8res(P, P, P, 8, 8, 8, 8, 8, 8, ctypes.c_void_p(8), 1 << 30, None) == -3
+
+
+def test_fp8res_noalloc_argument_checks():
+    """Argument and size errors of the allocation-free entry point come before
+    any device call: problems whose workspace exceeds the fixed scratch get
+    HBS_ERR_SCRATCH (-7)."""
+    L = hb.lib()
+    cap = hb.FP8RES_NOALLOC_SCRATCH_BYTES
+    assert L.hbs_gemm_fp8res_workspace_size(4096, 4096, 2688) <= cap
+    assert L.hbs_gemm_fp8res_workspace_size(4096, 4096, 2816) > cap
+    assert L.hbs_gemm_fp8res_noalloc(P, P, P, 4096, 4096, 2816, 2816, 4096, 4096, None) == -7
+    assert L.hbs_gemm_fp8res_noalloc(P, P, P, 0, 8, 8, 8, 8, 8, None) == -1
+    assert L.hbs_gemm_fp8res_noalloc(P, P, P, 8, 8, 8, 7, 8, 8, None) == -1
+    assert L.hbs_gemm_fp8res_noalloc(P, None, P, 8, 8, 8, 8, 8, 8, None) == -2
+    assert b"scratch" in L.hbs_strerror(-7)
 
 
 def test_square_and_rectangular_entries_reject_sizes_beyond_the_int_limit():
