@@ -106,6 +106,93 @@ __device__ __forceinline__ TileChain<Publish, Peek> make_chain(Publish p, Peek k
     return TileChain<Publish, Peek>{p, k, false, 0, false, TileArgs{}};
 }
 
+// Wave arrangement of a 256 x 256 tile of 8 waves (4 x 2 sub-tiles of 64 x 128;
+// waves w and w + 4 share a SIMD).  Interior tiles, and edge tiles whose valid
+// rows and columns still reach all four 64-row and all four 64-column units,
+// use the fixed arrangement of wave_pos (the tile configs in tile_f32.h and
+// tile_bf16x3.h).  Any other edge tile is covered by the mu x nu units of
+// 64 x 64 that hold valid rows and columns: each wave takes a piece of one
+// unit row -- two units (64 x 128, kWaveFull), one unit (64 x 64, kWaveHalf)
+// or nothing (kWaveNone) -- so that no SIMD holds more than ceil(mu nu / 4)
+// units, the least possible, with the fewest pieces.  The arrangements below
+// came from an exhaustive search over every split of the unit rows and every
+// assignment of the pieces to the 8 waves; edge_arrangements_valid() checks
+// them at compile time.
+enum WaveKind { kWaveNone = 0, kWaveFull = 1, kWaveHalf = 2 };
+struct WaveTile {
+    int om, on;  // origin of the wave's sub-tile in the tile (elements)
+    int kind;    // WaveKind
+};
+
+// mu x nu valid units (1..4 each, not both 4): byte w = the piece of wave w,
+// (unit row << 4) | (first unit column << 2) | WaveKind.
+__host__ __device__ constexpr uint64_t edge_arrangement(int mu, int nu) {
+    switch ((mu - 1) * 4 + nu - 1) {
+        case 0: return 0x0000000000000002ull;   // 1 x 1
+        case 1: return 0x0000000000000602ull;   // 1 x 2
+        case 2: return 0x00000000000a0602ull;   // 1 x 3
+        case 3: return 0x000000000e0a0602ull;   // 1 x 4
+        case 4: return 0x0000000000001202ull;   // 2 x 1
+        case 5: return 0x0000000016120602ull;   // 2 x 2
+        case 6: return 0x0000001200150502ull;   // 2 x 3
+        case 7: return 0x0000000019110901ull;   // 2 x 4
+        case 8: return 0x0000000000221202ull;   // 3 x 1
+        case 9: return 0x0000000000211101ull;   // 3 x 2
+        case 10: return 0x0025150500221202ull;  // 3 x 3
+        case 11: return 0x292119110e0a0602ull;  // 3 x 4
+        case 12: return 0x0000000032221202ull;  // 4 x 1
+        case 13: return 0x0000000031211101ull;  // 4 x 2
+        case 14: return 0x3525150532221202ull;  // 4 x 3
+        default: return 0;
+    }
+}
+
+// Every valid unit covered exactly once, every piece inside the valid units,
+// and no SIMD with more than ceil(mu nu / 4) units.
+constexpr bool edge_arrangements_valid() {
+    for (int mu = 1; mu <= 4; ++mu) {
+        for (int nu = 1; nu <= 4; ++nu) {
+            if (mu == 4 && nu == 4) continue;
+            int cover[4][4] = {};
+            int load[4] = {};
+            const uint64_t a = edge_arrangement(mu, nu);
+            for (int w = 0; w < 8; ++w) {
+                const int b = (int)((a >> (8 * w)) & 0xff);
+                const int kind = b & 3, um = b >> 4, un = (b >> 2) & 3;
+                if (kind == kWaveNone) {
+                    if (b != 0) return false;
+                    continue;
+                }
+                if (kind != kWaveFull && kind != kWaveHalf) return false;
+                const int n = kind == kWaveFull ? 2 : 1;
+                if (um >= mu || un + n > nu) return false;
+                for (int j = 0; j < n; ++j) ++cover[um][un + j];
+                load[w % 4] += n;
+            }
+            for (int r = 0; r < mu; ++r)
+                for (int c = 0; c < nu; ++c)
+                    if (cover[r][c] != 1) return false;
+            for (int q = 0; q < 4; ++q)
+                if (load[q] > (mu * nu + 3) / 4) return false;
+        }
+    }
+    return true;
+}
+static_assert(edge_arrangements_valid(), "edge-tile wave arrangements");
+
+// The sub-tile of wave `wave` (wave-uniform) in the 256 x 256 tile at
+// (m0, n0) of an M x N product, for a tile that may cross M or N.
+__device__ __forceinline__ WaveTile big_wave_tile(int M, int N, int m0, int n0, int wave) {
+    const int vm = min(M - m0, 256), vn = min(N - n0, 256);
+    const int mu = (vm + 63) >> 6, nu = (vn + 63) >> 6;
+    if (mu == 4 && nu == 4) {
+        const int wn = wave >> 2, wm = (wave + 2 * wn) & 3;
+        return WaveTile{wm * 64, wn * 128, kWaveFull};
+    }
+    const int b = (int)(edge_arrangement(mu, nu) >> (8 * wave)) & 0xff;
+    return WaveTile{(b >> 4) * 64, ((b >> 2) & 3) * 64, b & 3};
+}
+
 // Checks that the current device is a gfx950 GPU (cached per device).
 int check_device();
 

@@ -85,16 +85,20 @@ __device__ __forceinline__ void split_pk(float x0, float x1, uint32_t &hi, uint3
 // predicates.
 // smem holds 2 * G::kStage bf16; regs is the stage in flight.  Every thread of
 // the workgroup must call it.  ch: NoChain, or a TileChain to pipeline the
-// next tile's first stages into this tile's last steps.
-template <class G, int MODE, bool COMPUTE, class Chain>
+// next tile's first stages into this tile's last steps.  The wave computes
+// the sub-tile at (om, on) of the tile: kFM x kFN blocks (CK = kWaveFull),
+// kFM x kFN / 2 blocks (kWaveHalf) or none (kWaveNone: it still loads and
+// splits its share of each stage and meets every barrier).
+template <class G, int MODE, int CK, class Chain>
 __device__ __forceinline__ void bf16x3_block_mma_impl(uint16_t *smem, const float *__restrict__ Ag,
                                                       const float *__restrict__ Bg, int M, int N,
                                                       int K, int lda, int ldb, int m0, int n0,
-                                                      int wm, int wn,
+                                                      int om, int on,
                                                       floatx16 (&acc)[G::kFM][G::kFN],
                                                       typename G::Regs &regs, Chain &ch) {
     constexpr int kBM = G::BM, kBN = G::BN, kBK = G::BK, kThreads = G::kThreads;
-    constexpr int kWTM = G::kWTM, kWTN = G::kWTN, kFM = G::kFM, kFN = G::kFN;
+    constexpr int kFM = G::kFM, kFN = G::kFN;
+    constexpr int kFNc = CK == kWaveHalf ? kFN / 2 : kFN;  // column blocks computed
     constexpr int kLdsRow = G::kLdsRow, kStage = G::kStage;
     constexpr int kGroups = G::kGroups, kItemsA = G::kItemsA, kItemsB = G::kItemsB;
 
@@ -209,18 +213,18 @@ __device__ __forceinline__ void bf16x3_block_mma_impl(uint16_t *smem, const floa
     };
     // one 16-deep substep of the stage in buffer `buf`
     auto compute = [&](int buf, int st) {
-        if (!COMPUTE) return;
-        const uint16_t *as = smem + buf * kStage + (wm * kWTM + l32) * kLdsRow;
-        const uint16_t *bsm = smem + buf * kStage + kBM * kLdsRow + (wn * kWTN + l32) * kLdsRow;
+        if (CK == kWaveNone) return;
+        const uint16_t *as = smem + buf * kStage + (om + l32) * kLdsRow;
+        const uint16_t *bsm = smem + buf * kStage + kBM * kLdsRow + (on + l32) * kLdsRow;
         const int off = (2 * st + h) * 16;  // this lane half's 8-k group
-        bf16x8 ahi[kFM], alo[kFM], bhi[kFN], blo[kFN];
+        bf16x8 ahi[kFM], alo[kFM], bhi[kFNc], blo[kFNc];
 #pragma unroll
         for (int r = 0; r < kFM; ++r) {
             ahi[r] = *(const bf16x8 *)(as + r * 32 * kLdsRow + off);
             alo[r] = *(const bf16x8 *)(as + r * 32 * kLdsRow + off + 8);
         }
 #pragma unroll
-        for (int c = 0; c < kFN; ++c) {
+        for (int c = 0; c < kFNc; ++c) {
             bhi[c] = *(const bf16x8 *)(bsm + c * 32 * kLdsRow + off);
             blo[c] = *(const bf16x8 *)(bsm + c * 32 * kLdsRow + off + 8);
         }
@@ -231,7 +235,7 @@ __device__ __forceinline__ void bf16x3_block_mma_impl(uint16_t *smem, const floa
         // sharing the first operand instead measured no gain.  Each
         // accumulator still receives lo.hi, hi.lo, hi.hi in that order.
 #pragma unroll
-        for (int c = 0; c < kFN; ++c) {
+        for (int c = 0; c < kFNc; ++c) {
 #pragma unroll
             for (int r = 0; r < kFM; ++r)
                 acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[r], bhi[c], acc[r][c], 0, 0, 0);
@@ -318,23 +322,42 @@ __device__ __forceinline__ void bf16x3_block_mma_impl(uint16_t *smem, const floa
     }
 }
 
-// With 8 waves as 4 x 2, a wave whose sub-tile lies wholly outside M x N
-// (edge tiles outside kTileInterior) runs the same loop without fragment reads
-// and MFMAs: it still loads and splits its share of each stage and meets every
-// barrier.
+// The sub-tile of wave `wave` in the tile t: for the 256 x 256 tile of 4 x 2
+// waves outside kTileInterior, the edge-tile arrangement (big_wave_tile, so
+// that the valid units are spread evenly over the SIMDs); else wave_pos.
+template <class G, int MODE>
+__device__ __forceinline__ WaveTile bf16x3_wave_tile(int M, int N, int m0, int n0, int wave) {
+    if constexpr (MODE != kTileInterior && G::BM == 256 && G::BN == 256 && G::WM == 4 &&
+                  G::WN == 2) {
+        return big_wave_tile(M, N, m0, n0, wave);
+    } else {
+        int wm, wn;
+        G::wave_pos(wave, wm, wn);
+        return WaveTile{wm * G::kWTM, wn * G::kWTN, kWaveFull};
+    }
+}
+
+// One tile's main loop; each wave runs the loop copy of its sub-tile kind
+// (whole functions, so that the main loop stays one scheduling region).
 template <class G, int MODE, class Chain>
 __device__ __forceinline__ void bf16x3_block_mma(uint16_t *smem, const TileArgs &t,
                                                  floatx16 (&acc)[G::kFM][G::kFN],
                                                  typename G::Regs &regs, Chain &ch) {
-    int wm, wn;
-    G::wave_pos(__builtin_amdgcn_readfirstlane(threadIdx.x >> 6), wm, wn);
-    constexpr bool kSkip = MODE != kTileInterior && G::WM == 4 && G::WN == 2;
-    if (!kSkip || (t.m0 + wm * G::kWTM < t.M && t.n0 + wn * G::kWTN < t.N))
-        bf16x3_block_mma_impl<G, MODE, true>(smem, t.Ag, t.Bg, t.M, t.N, t.K, t.lda, t.ldb, t.m0,
-                                             t.n0, wm, wn, acc, regs, ch);
-    else
-        bf16x3_block_mma_impl<G, MODE, false>(smem, t.Ag, t.Bg, t.M, t.N, t.K, t.lda, t.ldb, t.m0,
-                                              t.n0, wm, wn, acc, regs, ch);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const WaveTile wt = bf16x3_wave_tile<G, MODE>(t.M, t.N, t.m0, t.n0, wave);
+    if (wt.kind == kWaveFull)
+        bf16x3_block_mma_impl<G, MODE, kWaveFull>(smem, t.Ag, t.Bg, t.M, t.N, t.K, t.lda, t.ldb,
+                                                  t.m0, t.n0, wt.om, wt.on, acc, regs, ch);
+    else if constexpr (MODE != kTileInterior && G::BM == 256 && G::BN == 256) {
+        if (wt.kind == kWaveHalf)
+            bf16x3_block_mma_impl<G, MODE, kWaveHalf>(smem, t.Ag, t.Bg, t.M, t.N, t.K, t.lda,
+                                                      t.ldb, t.m0, t.n0, wt.om, wt.on, acc, regs,
+                                                      ch);
+        else
+            bf16x3_block_mma_impl<G, MODE, kWaveNone>(smem, t.Ag, t.Bg, t.M, t.N, t.K, t.lda,
+                                                      t.ldb, t.m0, t.n0, wt.om, wt.on, acc, regs,
+                                                      ch);
+    }
 }
 template <class G, int MODE>
 __device__ __forceinline__ void bf16x3_block_mma(uint16_t *smem, const float *__restrict__ Ag,
@@ -347,25 +370,29 @@ __device__ __forceinline__ void bf16x3_block_mma(uint16_t *smem, const float *__
     bf16x3_block_mma<G, MODE>(smem, t, acc, regs, ch);
 }
 
-// Epilogue: stores acc to C (Cg = the matrix) at the tile (m0, n0); outside
-// kTileInterior the stores are predicated on M x N.
+// Epilogue: stores acc to C (Cg = the matrix) at the tile (m0, n0), each wave
+// its sub-tile (bf16x3_wave_tile); outside kTileInterior the stores are
+// predicated on M x N.
 template <class G, int MODE>
 __device__ __forceinline__ void bf16x3_block_store(float *__restrict__ Cg, int M, int N, int ldc,
                                                    int m0, int n0,
                                                    const floatx16 (&acc)[G::kFM][G::kFN]) {
-    constexpr int kWTM = G::kWTM, kWTN = G::kWTN, kFM = G::kFM, kFN = G::kFN;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int wm, wn;
-    G::wave_pos(wave, wm, wn);
+    constexpr int kFM = G::kFM, kFN = G::kFN;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const WaveTile wt = bf16x3_wave_tile<G, MODE>(M, N, m0, n0, wave);
+    if (wt.kind == kWaveNone) return;
+    const int ncols = wt.kind == kWaveHalf ? kFN / 2 : kFN;
     const int l32 = lane & 31, h = lane >> 5;
 #pragma unroll
     for (int r = 0; r < kFM; ++r) {
 #pragma unroll
         for (int c = 0; c < kFN; ++c) {
-            const int col = n0 + wn * kWTN + c * 32 + l32;
+            if (c >= ncols) continue;
+            const int col = n0 + wt.on + c * 32 + l32;
 #pragma unroll
             for (int q = 0; q < 16; ++q) {
-                const int row = m0 + wm * kWTM + r * 32 + 8 * (q >> 2) + 4 * h + (q & 3);
+                const int row = m0 + wt.om + r * 32 + 8 * (q >> 2) + 4 * h + (q & 3);
                 if (MODE == kTileInterior || (row < M && col < N))
                     stg(Cg + (int64_t)row * ldc + col, acc[r][c][q]);
             }

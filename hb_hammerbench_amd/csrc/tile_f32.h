@@ -68,14 +68,18 @@ struct F32Cfg {
 // shape or alignment), kTileClamp (ragged M or N; K a multiple of BK, N of 4,
 // 16-byte loadable operands) or kTileClampK (the same with K a multiple of 4).
 // regs is the stage in flight.  ch: NoChain, or a TileChain to pipeline the
-// next tile's first stages into this tile's last steps.
-template <int BM, int BN, int BK, int WM, int WN, int MODE, bool COMPUTE, class Chain>
+// next tile's first stages into this tile's last steps.  The wave computes the
+// sub-tile at (om, on) of the tile: kFM x kFN blocks (CK = kWaveFull),
+// kFM x kFN / 2 blocks (kWaveHalf) or none (kWaveNone: it still loads and
+// stores its share of each stage and meets every barrier).
+template <int BM, int BN, int BK, int WM, int WN, int MODE, int CK, class Chain>
 __device__ __forceinline__ void f32_block_mma_impl(
     float *smem, const float *__restrict__ Ag, const float *__restrict__ Bg, int M, int N, int K,
-    int lda, int ldb, int m0, int n0, int wm, int wn,
+    int lda, int ldb, int m0, int n0, int om, int on,
     floatx16 (&acc)[F32Cfg<BM, BN, BK, WM, WN>::kFM][F32Cfg<BM, BN, BK, WM, WN>::kFN],
     typename F32Cfg<BM, BN, BK, WM, WN>::Regs &regs, Chain &ch) {
     using G = F32Cfg<BM, BN, BK, WM, WN>;
+    constexpr int kFNc = CK == kWaveHalf ? G::kFN / 2 : G::kFN;  // column blocks computed
     float *const As = smem;
     float *const Bs = smem + 2 * G::kStageA;
 
@@ -183,28 +187,28 @@ __device__ __forceinline__ void f32_block_mma_impl(
     // kc + 1 are issued before the MFMAs of chunk kc so LDS latency overlaps
     // matrix-core work.
     auto load_frag = [&](const float *as, const float *bs, int kc, floatx4 (&a)[G::kFM],
-                         float (&b)[G::kFN][4]) {
+                         float (&b)[kFNc][4]) {
 #pragma unroll
         for (int r = 0; r < G::kFM; ++r) a[r] = *(const floatx4 *)(as + r * 32 * G::kLdA + kc * 8);
 #pragma unroll
-        for (int c = 0; c < G::kFN; ++c)
+        for (int c = 0; c < kFNc; ++c)
 #pragma unroll
             for (int q = 0; q < 4; ++q) b[c][q] = bs[(kc * 8 + q) * G::kLdB + c * 32];
     };
-    auto mfma_chunk = [&](const floatx4 (&a)[G::kFM], const float (&b)[G::kFN][4]) {
+    auto mfma_chunk = [&](const floatx4 (&a)[G::kFM], const float (&b)[kFNc][4]) {
 #pragma unroll
         for (int q = 0; q < 4; ++q)
 #pragma unroll
             for (int r = 0; r < G::kFM; ++r)
 #pragma unroll
-                for (int c = 0; c < G::kFN; ++c)
+                for (int c = 0; c < kFNc; ++c)
                     acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[r][q], b[c][q], acc[r][c], 0, 0, 0);
     };
     // Scheduling hint: spread the next chunk's LDS reads (kFM 128-bit and
     // 2 kFN paired 32-bit reads) between this chunk's 4 kFM kFN MFMAs.
     auto interleave = [&]() {
-        constexpr int kReads = G::kFM + 2 * G::kFN;
-        constexpr int kMfma = 4 * G::kFM * G::kFN;
+        constexpr int kReads = G::kFM + 2 * kFNc;
+        constexpr int kMfma = 4 * G::kFM * kFNc;
         constexpr int kPer = kMfma / kReads > 0 ? kMfma / kReads : 1;
 #pragma unroll
         for (int i = 0; i < kReads; ++i) {
@@ -218,14 +222,14 @@ __device__ __forceinline__ void f32_block_mma_impl(
     // flight, so the matrix core is never left waiting on a fresh LDS read.
     constexpr int kChunks = BK / 8, kHook = kChunks / 2;
     auto compute = [&](int buf, auto &&hook) {
-        if constexpr (!COMPUTE) {
+        if constexpr (CK == kWaveNone) {
             hook();
             return;
         }
-        const float *as = As + buf * G::kStageA + (wm * G::kWTM + l32) * G::kLdA + 4 * h;
-        const float *bs = Bs + buf * G::kStageB + 4 * h * G::kLdB + wn * G::kWTN + l32;
+        const float *as = As + buf * G::kStageA + (om + l32) * G::kLdA + 4 * h;
+        const float *bs = Bs + buf * G::kStageB + 4 * h * G::kLdB + on + l32;
         floatx4 a0[G::kFM], a1[G::kFM];
-        float b0[G::kFN][4], b1[G::kFN][4];
+        float b0[kFNc][4], b1[kFNc][4];
         load_frag(as, bs, 0, a0, b0);
 #pragma unroll
         for (int kc = 0; kc < kChunks; kc += 2) {
@@ -312,25 +316,44 @@ __device__ __forceinline__ void f32_block_mma_impl(
     }
 }
 
-// With 8 waves as 4 x 2, a wave whose sub-tile lies wholly outside M x N
-// (edge tiles outside kTileInterior) runs the same loop without fragment reads
-// and MFMAs: it still loads and stores its share of each stage and meets every
-// barrier.
+// The sub-tile of wave `wave` in the tile at (m0, n0): for the 256 x 256 tile
+// of 4 x 2 waves outside kTileInterior, the edge-tile arrangement
+// (big_wave_tile, so that the valid units are spread evenly over the SIMDs);
+// else wave_pos.
+template <int BM, int BN, int BK, int WM, int WN, int MODE>
+__device__ __forceinline__ WaveTile f32_wave_tile(int M, int N, int m0, int n0, int wave) {
+    using G = F32Cfg<BM, BN, BK, WM, WN>;
+    if constexpr (MODE != kTileInterior && BM == 256 && BN == 256 && WM == 4 && WN == 2) {
+        return big_wave_tile(M, N, m0, n0, wave);
+    } else {
+        int wm, wn;
+        G::wave_pos(wave, wm, wn);
+        return WaveTile{wm * G::kWTM, wn * G::kWTN, kWaveFull};
+    }
+}
+
+// One tile's main loop; each wave runs the loop copy of its sub-tile kind
+// (whole functions, so that the main loop stays one scheduling region).
 template <int BM, int BN, int BK, int WM, int WN, int MODE, class Chain>
 __device__ __forceinline__ void f32_block_mma(
     float *smem, const TileArgs &t,
     floatx16 (&acc)[F32Cfg<BM, BN, BK, WM, WN>::kFM][F32Cfg<BM, BN, BK, WM, WN>::kFN],
     typename F32Cfg<BM, BN, BK, WM, WN>::Regs &regs, Chain &ch) {
-    using G = F32Cfg<BM, BN, BK, WM, WN>;
-    int wm, wn;
-    G::wave_pos(__builtin_amdgcn_readfirstlane(threadIdx.x >> 6), wm, wn);
-    constexpr bool kSkip = MODE != kTileInterior && WM == 4 && WN == 2;
-    if (!kSkip || (t.m0 + wm * G::kWTM < t.M && t.n0 + wn * G::kWTN < t.N))
-        f32_block_mma_impl<BM, BN, BK, WM, WN, MODE, true>(
-            smem, t.Ag, t.Bg, t.M, t.N, t.K, t.lda, t.ldb, t.m0, t.n0, wm, wn, acc, regs, ch);
-    else
-        f32_block_mma_impl<BM, BN, BK, WM, WN, MODE, false>(
-            smem, t.Ag, t.Bg, t.M, t.N, t.K, t.lda, t.ldb, t.m0, t.n0, wm, wn, acc, regs, ch);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const WaveTile wt = f32_wave_tile<BM, BN, BK, WM, WN, MODE>(t.M, t.N, t.m0, t.n0, wave);
+    if (wt.kind == kWaveFull)
+        f32_block_mma_impl<BM, BN, BK, WM, WN, MODE, kWaveFull>(
+            smem, t.Ag, t.Bg, t.M, t.N, t.K, t.lda, t.ldb, t.m0, t.n0, wt.om, wt.on, acc, regs, ch);
+    else if constexpr (MODE != kTileInterior && BM == 256 && BN == 256) {
+        if (wt.kind == kWaveHalf)
+            f32_block_mma_impl<BM, BN, BK, WM, WN, MODE, kWaveHalf>(
+                smem, t.Ag, t.Bg, t.M, t.N, t.K, t.lda, t.ldb, t.m0, t.n0, wt.om, wt.on, acc, regs,
+                ch);
+        else
+            f32_block_mma_impl<BM, BN, BK, WM, WN, MODE, kWaveNone>(
+                smem, t.Ag, t.Bg, t.M, t.N, t.K, t.lda, t.ldb, t.m0, t.n0, wt.om, wt.on, acc, regs,
+                ch);
+    }
 }
 template <int BM, int BN, int BK, int WM, int WN, int MODE>
 __device__ __forceinline__ void f32_block_mma(
@@ -343,7 +366,8 @@ __device__ __forceinline__ void f32_block_mma(
     f32_block_mma<BM, BN, BK, WM, WN, MODE>(smem, t, acc, regs, ch);
 }
 
-// Epilogue: stores acc to C (Cg = the matrix) at the tile (m0, n0).  32x32
+// Epilogue: stores acc to C (Cg = the matrix) at the tile (m0, n0), each wave
+// its sub-tile (f32_wave_tile).  32x32
 // accumulator layout: register q of lane l holds row 8*(q/4) + 4*(l/32) + q%4,
 // col l%32 of its block.  Outside kTileInterior the stores are predicated on
 // M x N.
@@ -352,18 +376,21 @@ __device__ __forceinline__ void f32_block_store(
     float *__restrict__ Cg, int M, int N, int ldc, int m0, int n0,
     const floatx16 (&acc)[F32Cfg<BM, BN, BK, WM, WN>::kFM][F32Cfg<BM, BN, BK, WM, WN>::kFN]) {
     using G = F32Cfg<BM, BN, BK, WM, WN>;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int wm, wn;
-    G::wave_pos(wave, wm, wn);
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const WaveTile wt = f32_wave_tile<BM, BN, BK, WM, WN, MODE>(M, N, m0, n0, wave);
+    if (wt.kind == kWaveNone) return;
+    const int ncols = wt.kind == kWaveHalf ? G::kFN / 2 : G::kFN;
     const int l32 = lane & 31, h = lane >> 5;
 #pragma unroll
     for (int r = 0; r < G::kFM; ++r) {
 #pragma unroll
         for (int c = 0; c < G::kFN; ++c) {
-            const int col = n0 + wn * G::kWTN + c * 32 + l32;
+            if (c >= ncols) continue;
+            const int col = n0 + wt.on + c * 32 + l32;
 #pragma unroll
             for (int q = 0; q < 16; ++q) {
-                const int row = m0 + wm * G::kWTM + r * 32 + 8 * (q >> 2) + 4 * h + (q & 3);
+                const int row = m0 + wt.om + r * 32 + 8 * (q >> 2) + 4 * h + (q & 3);
                 if (MODE == kTileInterior || (row < M && col < N))
                     stg(Cg + (int64_t)row * ldc + col, acc[r][c][q]);
             }

@@ -446,3 +446,45 @@ def test_plan_outlives_its_graphs():
     assert L.hbs_graph_destroy(h2) == 0
     plan.close()
     assert plan.handle is None
+
+
+EDGE_REMS = (40, 64, 100, 128, 148, 192, 228, 256)
+
+
+@pytest.mark.parametrize("prec", [hb.PREC_FP32, hb.PREC_BF16X3])
+@pytest.mark.parametrize("K", [64, 100])
+def test_edge_tile_arrangements_exact(prec, K):
+    """Every valid region an edge tile of the large-tile launch can have: M and
+    N remainders falling into each count of 64-row and 64-column units (1..4),
+    so that every wave arrangement of the edge tiles -- pieces of 64 x 128 and
+    64 x 64 spread over the SIMDs, or none -- computes and stores its part.
+    Small-integer inputs: the result is the exact integer product."""
+    shapes = []
+    for i, rm in enumerate(EDGE_REMS):
+        for j, rn in enumerate(EDGE_REMS):
+            shapes.append((256 * ((i + j) % 2) + rm, 256 * (j % 2) + rn, K))
+    probs, views, keep = make_group(shapes, "pattern")
+    plan = hb.GroupedPlan(probs, prec)
+    plan.run()
+    torch.cuda.synchronize()
+    for (A, B, C), (M, N, KK) in zip(views, shapes):
+        R = A.double() @ B.double()
+        assert torch.equal(C, R.float()), (M, N, KK)
+
+
+@pytest.mark.parametrize("N", [3900, 4000, 3830])
+def test_square_edge_tiles_exact(N):
+    """Square kernels take the same edge arrangement on their last tile row and
+    column when they run 256 x 256 tiles: remainders of 60 (K not a multiple
+    of 32: zero-filling loads), 160 (clamped loads) and 246 (N not a multiple
+    of 4: predicated loads)."""
+    n = N * N
+    a = torch.empty(n, dtype=torch.float32, device="cuda"); b = torch.empty_like(a)
+    c = torch.empty_like(a)
+    hb.fill_pattern(a, b, n)
+    expect = oracle.pattern_slice(N, 0)
+    for fn in (hb.sgemm, hb.sgemm_bf16x3):
+        c.fill_(float("nan"))
+        fn(a, b, c, N, 1)
+        torch.cuda.synchronize()
+        assert np.array_equal(c.view(N, N).cpu().numpy(), expect), (fn.__name__, N)
