@@ -188,56 +188,67 @@ __global__ __launch_bounds__(kThreads) void gemm_fp8res_kernel(GemmArgs p) {
 #pragma unroll
         for (int c = 0; c < kFN; ++c) acc[r][c] = floatx16{0};
 
-    floatx4 ra8[kCA8], rb8[kCB8], rar[kCAR], rbr[kCBR];
-    int rexp = 0;
+    // One stage in registers between its global loads and its store to LDS.
+    // Two of them are in flight, so a stage's loads are issued two steps
+    // before its data is stored to LDS: one step of MFMAs does not cover the
+    // loads' latency (72 KB per workgroup per stage), two do.
+    struct Stage {
+        floatx4 a8[kCA8], b8[kCB8], ar[kCAR], br[kCBR];
+        int ex;
+    };
+    const int KT = p.Kp / kBK;
     // lanes 0, 1: exponents of the two 128-row A tiles; lane 2: the B column tile
     const int *exp_src = tid < 2 ? p.eA + (int64_t)(m0 / kQT + tid) * ktiles
                                  : p.eB + (int64_t)(n0 / kQT) * ktiles;
-    auto gload = [&](int k0) {
-        if (tid < 3) rexp = exp_src[k0 / kQT];
+    // Loads are unconditional (a stage past the end re-reads the last one,
+    // and every lane reads an exponent; lanes 0..2 keep theirs), so that the
+    // register sets stay in registers.
+    auto gload = [&](Stage &r, int kt) {
+        const int k0 = min(kt, KT - 1) * kBK;
+        r.ex = exp_src[k0 / kQT];
 #pragma unroll
         for (int i = 0; i < kCA8; ++i) {
             const int f = tid + i * kThreads, row = f >> 2, c = f & 3;
-            ra8[i] = *(const floatx4 *)(p.A8 + (int64_t)(m0 + row) * p.Kp + k0 + c * 16);
+            r.a8[i] = *(const floatx4 *)(p.A8 + (int64_t)(m0 + row) * p.Kp + k0 + c * 16);
         }
 #pragma unroll
         for (int i = 0; i < kCB8; ++i) {
             const int f = tid + i * kThreads, row = f >> 2, c = f & 3;
-            rb8[i] = *(const floatx4 *)(p.B8 + (int64_t)(n0 + row) * p.Kp + k0 + c * 16);
+            r.b8[i] = *(const floatx4 *)(p.B8 + (int64_t)(n0 + row) * p.Kp + k0 + c * 16);
         }
 #pragma unroll
         for (int i = 0; i < kCAR; ++i) {
             const int f = tid + i * kThreads, row = f >> 3, c = f & 7;
-            rar[i] = *(const floatx4 *)(p.Ar + (int64_t)(m0 + row) * p.Kp + k0 + c * 8);
+            r.ar[i] = *(const floatx4 *)(p.Ar + (int64_t)(m0 + row) * p.Kp + k0 + c * 8);
         }
 #pragma unroll
         for (int i = 0; i < kCBR; ++i) {
             const int f = tid + i * kThreads, row = f >> 3, c = f & 7;
-            rbr[i] = *(const floatx4 *)(p.Br + (int64_t)(n0 + row) * p.Kp + k0 + c * 8);
+            r.br[i] = *(const floatx4 *)(p.Br + (int64_t)(n0 + row) * p.Kp + k0 + c * 8);
         }
     };
-    auto sstore = [&](int buf) {
+    auto sstore = [&](const Stage &r, int buf) {
         uint8_t *st = smem + buf * kStage;
-        if (tid < 3) ((int *)(st + kExp))[tid] = rexp;
+        if (tid < 3) ((int *)(st + kExp))[tid] = r.ex;
 #pragma unroll
         for (int i = 0; i < kCA8; ++i) {
             const int f = tid + i * kThreads, row = f >> 2, c = f & 3;
-            *(floatx4 *)(st + kA8 + sw8(row, c)) = ra8[i];
+            *(floatx4 *)(st + kA8 + sw8(row, c)) = r.a8[i];
         }
 #pragma unroll
         for (int i = 0; i < kCB8; ++i) {
             const int f = tid + i * kThreads, row = f >> 2, c = f & 3;
-            *(floatx4 *)(st + kB8 + sw8(row, c)) = rb8[i];
+            *(floatx4 *)(st + kB8 + sw8(row, c)) = r.b8[i];
         }
 #pragma unroll
         for (int i = 0; i < kCAR; ++i) {
             const int f = tid + i * kThreads, row = f >> 3, c = f & 7;
-            *(floatx4 *)(st + kAR + swr(row, c)) = rar[i];
+            *(floatx4 *)(st + kAR + swr(row, c)) = r.ar[i];
         }
 #pragma unroll
         for (int i = 0; i < kCBR; ++i) {
             const int f = tid + i * kThreads, row = f >> 3, c = f & 7;
-            *(floatx4 *)(st + kBR + swr(row, c)) = rbr[i];
+            *(floatx4 *)(st + kBR + swr(row, c)) = r.br[i];
         }
     };
 
@@ -308,16 +319,28 @@ __global__ __launch_bounds__(kThreads) void gemm_fp8res_kernel(GemmArgs p) {
                     a8[r], b8[c], acc[r][c], 0, 0, 0, ea + 127, 0, eb + 127);
     };
 
-    const int KT = p.Kp / kBK;
-    gload(0);
-    sstore(0);
+    // Pipeline: stage kt is in LDS buffer kt & 1; at the start of step kt the
+    // register sets hold stages kt + 1 and kt + 2, in flight.  Step kt computes
+    // from LDS, stores stage kt + 1 to the other buffer (waiting for its
+    // loads, issued two steps earlier), meets the barrier and issues the loads
+    // of stage kt + 3 into the set just stored.  The loop is unrolled by two so
+    // that each register set has a fixed role.
+    Stage s0, s1;
+    gload(s0, 0);
+    gload(s1, 1);
+    sstore(s0, 0);
     __syncthreads();
-    for (int kt = 0; kt < KT; ++kt) {
-        const int cur = kt & 1;
-        if (kt + 1 < KT) gload((kt + 1) * kBK);
-        compute(cur);
-        if (kt + 1 < KT) sstore(cur ^ 1);
+    gload(s0, 2);
+    for (int kt = 0; kt < KT; kt += 2) {
+        compute(0);  // stage kt; s1 = stage kt + 1, s0 = stage kt + 2
+        if (kt + 1 < KT) sstore(s1, 1);
         __syncthreads();
+        if (kt + 1 >= KT) break;
+        gload(s1, kt + 3);
+        compute(1);  // stage kt + 1; s0 = stage kt + 2, s1 = stage kt + 3
+        if (kt + 2 < KT) sstore(s0, 0);
+        __syncthreads();
+        gload(s0, kt + 4);
     }
 
 #pragma unroll
