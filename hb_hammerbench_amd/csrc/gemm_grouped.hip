@@ -45,6 +45,8 @@
 #include "tile_bf16x3.h"
 #include "tile_f32.h"
 
+constexpr int kBigLaunches = 3;  // large-tile launches: clamped, zero fill past K, predicated
+
 struct hbs_grouped_plan {
     int device;
     int precision;
@@ -62,13 +64,14 @@ struct hbs_grouped_plan {
     float *d_partials;
     // large-tile launches (whole problems in 256 x 256 tiles): [0] problems
     // whose K is a multiple of the 32-deep stage (kTileClamp), [1] problems
-    // with any other K (kTileClampK)
-    int nblocks_big[2];
-    int64_t items_big[2];
-    const void *d_items_big[2];
-    const int *d_qoff_big[2];  // item range of each of its nq_big queues
-    int nq_big[2];
-    int *d_queue_big[2];       // nq_big heads and a finished-workgroup count
+    // with any other K (kTileClampK), [2] problems whose operands do not allow
+    // 16-byte loads (kTileEdge)
+    int nblocks_big[kBigLaunches];
+    int64_t items_big[kBigLaunches];
+    const void *d_items_big[kBigLaunches];
+    const int *d_qoff_big[kBigLaunches];  // item range of each of its nq_big queues
+    int nq_big[kBigLaunches];
+    int *d_queue_big[kBigLaunches];       // nq_big heads and a finished-workgroup count
     int graphs;                // executable graphs made from this plan and not yet destroyed
 };
 
@@ -280,7 +283,10 @@ __global__ __launch_bounds__(kThreads, 2) void grouped_kernel(const Prob *__rest
 // is covered whole: tiles at its ragged bottom and right edges read clamped
 // rows and columns (kTileClamp) and store only what lies inside M x N.  No
 // split-K.  MODE kTileClamp takes problems whose K is a multiple of the
-// 32-deep stage; kTileClampK (a launch of its own) the others, zero-filling
+// 32-deep stage; kTileEdge (a launch of its own) problems whose A or B is not
+// 16-byte aligned or whose N, K, lda or ldb is not a multiple of 4, with
+// per-element predicated loads; kTileClampK (a launch of its own) the others,
+// zero-filling
 // past K in the last stage.  One kernel holding both modes ran out of
 // registers and cost the first kind 5%.
 // ---------------------------------------------------------------------------
@@ -295,7 +301,8 @@ __global__ __launch_bounds__(kBigThreads) void grouped_big_kernel(const Prob *__
                                                                   const int *__restrict__ qoff,
                                                                   int nq, int *queue) {
     constexpr bool kF32 = PREC == HBS_PREC_FP32;
-    static_assert(MODE == hbs::kTileClamp || MODE == hbs::kTileClampK, "mode");
+    static_assert(MODE == hbs::kTileClamp || MODE == hbs::kTileClampK || MODE == hbs::kTileEdge,
+                  "mode");
     constexpr int LDS = kF32 ? BigF32::kLdsFloats * 4 : 2 * BigB3::kStage * 2;
     using Regs = std::conditional_t<kF32, BigF32::Regs, BigB3::Regs>;
     static_assert(BigF32::kFM == BigB3::kFM && BigF32::kFN == BigB3::kFN, "accumulators");
@@ -432,9 +439,15 @@ int validate(const hbs_gemm_problem &q) {
     if (q.lda < q.K || q.ldb < q.N || q.ldc < q.N) return HBS_ERR_ARG;
     if (!hbs::dims_ok(q.M, q.N, q.K, q.lda, q.ldb, q.ldc)) return HBS_ERR_ARG;
     if (!q.A || !q.B || !q.C) return HBS_ERR_NULL;
-    if (!hbs::aligned16(q.A) || !hbs::aligned16(q.B)) return HBS_ERR_ALIGN;
-    if (q.N % 4 || q.K % 4 || q.lda % 4 || q.ldb % 4) return HBS_ERR_ALIGN;
     return HBS_SUCCESS;
+}
+
+// Whether the problem's operands allow the 16-byte loads of the clamped tile
+// modes (A and B 16-byte aligned; N, K, lda and ldb multiples of 4).  Other
+// problems run on the large-tile launch with per-element predicated loads.
+bool vec_loads(const hbs_gemm_problem &q) {
+    return hbs::aligned16(q.A) && hbs::aligned16(q.B) && q.N % 4 == 0 && q.K % 4 == 0 &&
+           q.lda % 4 == 0 && q.ldb % 4 == 0;
 }
 
 int validate_all(const hbs_gemm_problem *problems, int count, int precision) {
@@ -453,14 +466,14 @@ int validate_all(const hbs_gemm_problem *problems, int count, int precision) {
 struct HostPlan {
     std::vector<Prob> probs;
     std::vector<Item> sorted;         // 256-thread list
-    std::vector<Item> sorted_big[2];  // large-tile launches, in queue order
-    std::vector<int> qoff_big[2];
-    int nq_big[2] = {1, 1}, nblocks = 0, nblocks_big[2] = {0, 0};
+    std::vector<Item> sorted_big[kBigLaunches];  // large-tile launches, in queue order
+    std::vector<int> qoff_big[kBigLaunches];
+    int nq_big[kBigLaunches] = {1, 1, 1}, nblocks = 0, nblocks_big[kBigLaunches] = {0, 0, 0};
     int64_t slots = 0, poff = 0, tiles_total = 0;
-    double imb = 1.0, imb_big[2] = {1.0, 1.0};
+    double imb = 1.0, imb_big[kBigLaunches] = {1.0, 1.0, 1.0};
     double est_imbalance() const {
         double e = nblocks > 0 ? imb : 1.0;
-        for (int g = 0; g < 2; ++g)
+        for (int g = 0; g < kBigLaunches; ++g)
             if (nblocks_big[g] > 0) e = std::max(e, imb_big[g]);
         return e;
     }
@@ -479,8 +492,9 @@ int plan_host(const hbs_gemm_problem *problems, int count, int precision, const 
     };
     std::vector<Prob> probs(count);
     std::vector<Small> small;
-    std::vector<Item> items_big[2];  // by launch: K a multiple of 32, other K
-    std::vector<double> cost_big[2];
+    // by launch: K a multiple of 32, other K, no 16-byte loads
+    std::vector<Item> items_big[kBigLaunches];
+    std::vector<double> cost_big[kBigLaunches];
     auto lookup = [&](int M, int N, int K, bool &big) {
         Choice c{0, 1, 0.0};  // the 256-thread tile, when one is chosen
         big = true;           // the default (see above)
@@ -501,12 +515,15 @@ int plan_host(const hbs_gemm_problem *problems, int count, int precision, const 
         probs[i] = Prob{q.A, q.B, q.C, q.M, q.N, q.K, q.lda, q.ldb, q.ldc};
         bool big = false;
         const Choice c = lookup(q.M, q.N, q.K, big);
-        if (!big) {
+        const bool vec = vec_loads(q);
+        if (!big && vec) {
             small.push_back(Small{i, c});
             continue;
         }
-        // the whole problem in 256 x 256 tiles; edge tiles clamp their loads
-        const int g = q.K % 32 == 0 ? 0 : 1;
+        // the whole problem in 256 x 256 tiles; edge tiles clamp their loads,
+        // or predicate them per element when the operands allow no 16-byte
+        // loads (such problems take this launch whatever the tuning says)
+        const int g = !vec ? 2 : q.K % 32 == 0 ? 0 : 1;
         for (int tm = 0; tm < (q.M + kBig - 1) / kBig; ++tm)
             for (int tn = 0; tn < (q.N + kBig - 1) / kBig; ++tn) {
                 items_big[g].push_back(Item{i, tm, tn, 0, q.K, kTileBig, 1, 0, -1, 0});
@@ -517,7 +534,8 @@ int plan_host(const hbs_gemm_problem *problems, int count, int precision, const 
     std::vector<Item> items;
     std::vector<double> cost;
     int64_t slots = 0, poff = 0,
-            tiles_total = (int64_t)(items_big[0].size() + items_big[1].size());
+            tiles_total = (int64_t)(items_big[0].size() + items_big[1].size() +
+                                    items_big[2].size());
     for (const Small &sp : small) {
         const Prob &q = probs[sp.prob];
         const Choice &c = sp.c;
@@ -563,9 +581,8 @@ int plan_host(const hbs_gemm_problem *problems, int count, int precision, const 
     }
     if (items.size() > 0x7fffffffULL) return HBS_ERR_ARG;
 
-    if (items.size() > 0x7fffffffULL || items_big[0].size() > 0x7fffffffULL ||
-        items_big[1].size() > 0x7fffffffULL)
-        return HBS_ERR_ARG;
+    for (int g = 0; g < kBigLaunches; ++g)
+        if (items_big[g].size() > 0x7fffffffULL) return HBS_ERR_ARG;
 
     // 256-thread launch: items in decreasing cost, handed out in that order to
     // whichever workgroup is free first.  With the modelled costs this is
@@ -661,7 +678,7 @@ int plan_host(const hbs_gemm_problem *problems, int count, int precision, const 
     std::vector<Item> sorted;
     double imb = 1.0;
     lpt(items, cost, nblocks, sorted, imb);
-    for (int g = 0; g < 2; ++g) {
+    for (int g = 0; g < kBigLaunches; ++g) {
         hp.nblocks_big[g] = (int)std::min<int64_t>(cus, (int64_t)items_big[g].size());
         xcd_queues(items_big[g], cost_big[g], hp.nblocks_big[g], hp.sorted_big[g],
                    hp.qoff_big[g], hp.nq_big[g], hp.imb_big[g]);
@@ -703,12 +720,12 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
     const size_t b_items = align(std::max<size_t>(sorted.size(), 1) * sizeof(Item));
     // split counters, the 256-thread launch's queue (2), each large-tile
     // launch's queues (kXcdQueues heads + 1)
-    const size_t b_cnt = align((slots + 2 + 2 * (kXcdQueues + 1)) * sizeof(int));
+    const size_t b_cnt = align((slots + 2 + kBigLaunches * (kXcdQueues + 1)) * sizeof(int));
     const size_t b_part = align(std::max<int64_t>(poff, 1) * sizeof(float));
     const size_t o_items = b_probs, o_cnt = o_items + b_items;
     const size_t o_part = o_cnt + b_cnt;
-    size_t o_items_big[2], o_qoff_big[2], total = o_part + b_part;
-    for (int g = 0; g < 2; ++g) {
+    size_t o_items_big[kBigLaunches], o_qoff_big[kBigLaunches], total = o_part + b_part;
+    for (int g = 0; g < kBigLaunches; ++g) {
         o_items_big[g] = total;
         total += align(std::max<size_t>(hp.sorted_big[g].size(), 1) * sizeof(Item));
         o_qoff_big[g] = total;
@@ -731,7 +748,7 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
     plan->d_counters = (int *)(dmem + o_cnt);
     plan->d_queue = plan->d_counters + slots;
     plan->d_partials = (float *)(dmem + o_part);
-    for (int g = 0; g < 2; ++g) {
+    for (int g = 0; g < kBigLaunches; ++g) {
         plan->d_queue_big[g] = plan->d_queue + 2 + g * (kXcdQueues + 1);
         plan->nblocks_big[g] = hp.nblocks_big[g];
         plan->items_big[g] = (int64_t)hp.sorted_big[g].size();
@@ -743,7 +760,7 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
     if (e == hipSuccess && !sorted.empty())
         e = hipMemcpy(dmem + o_items, sorted.data(), sorted.size() * sizeof(Item),
                       hipMemcpyHostToDevice);
-    for (int g = 0; g < 2; ++g) {
+    for (int g = 0; g < kBigLaunches; ++g) {
         const std::vector<Item> &sb = hp.sorted_big[g];
         const std::vector<int> &qo = hp.qoff_big[g];
         if (e == hipSuccess && !sb.empty())
@@ -806,10 +823,10 @@ extern "C" int hbs_grouped_plan_describe(const hbs_gemm_problem *problems, int c
                              tuned ? &table : nullptr, cus, hp);
     if (st) return st;
     info->problems = count;
-    info->workgroups = hp.nblocks + hp.nblocks_big[0] + hp.nblocks_big[1];
+    info->workgroups = hp.nblocks + hp.nblocks_big[0] + hp.nblocks_big[1] + hp.nblocks_big[2];
     info->tiles = hp.tiles_total;
     info->work_items = (int64_t)(hp.sorted.size() + hp.sorted_big[0].size() +
-                                 hp.sorted_big[1].size());
+                                 hp.sorted_big[1].size() + hp.sorted_big[2].size());
     info->split_tiles = hp.slots;
     info->est_imbalance = hp.est_imbalance();
     return HBS_SUCCESS;
@@ -818,9 +835,10 @@ extern "C" int hbs_grouped_plan_describe(const hbs_gemm_problem *problems, int c
 extern "C" int hbs_grouped_plan_info(const hbs_grouped_plan *plan, hbs_grouped_info *info) {
     if (!plan || !info) return HBS_ERR_NULL;
     info->problems = plan->count;
-    info->workgroups = plan->nblocks + plan->nblocks_big[0] + plan->nblocks_big[1];
+    info->workgroups =
+        plan->nblocks + plan->nblocks_big[0] + plan->nblocks_big[1] + plan->nblocks_big[2];
     info->tiles = plan->tiles;
-    info->work_items = plan->items + plan->items_big[0] + plan->items_big[1];
+    info->work_items = plan->items + plan->items_big[0] + plan->items_big[1] + plan->items_big[2];
     info->split_tiles = plan->split_tiles;
     info->est_imbalance = plan->est_imbalance;
     return HBS_SUCCESS;
@@ -835,22 +853,23 @@ extern "C" int hbs_grouped_run(const hbs_grouped_plan *plan, void *stream) {
     const Prob *pr = (const Prob *)plan->d_probs;
     const Item *it = (const Item *)plan->d_items;
     const bool fp32 = plan->precision == HBS_PREC_FP32;
-    for (int g = 0; g < 2; ++g) {
+    for (int g = 0; g < kBigLaunches; ++g) {
         if (plan->nblocks_big[g] == 0) continue;
         const Item *itb = (const Item *)plan->d_items_big[g];
         const dim3 grid(plan->nblocks_big[g]);
-        if (fp32 && g == 0)
-            grouped_big_kernel<HBS_PREC_FP32, hbs::kTileClamp><<<grid, kBigThreads, 0, s>>>(
-                pr, itb, plan->d_qoff_big[g], plan->nq_big[g], plan->d_queue_big[g]);
-        else if (!fp32 && g == 0)
-            grouped_big_kernel<HBS_PREC_BF16X3, hbs::kTileClamp><<<grid, kBigThreads, 0, s>>>(
-                pr, itb, plan->d_qoff_big[g], plan->nq_big[g], plan->d_queue_big[g]);
-        else if (!fp32)
-            grouped_big_kernel<HBS_PREC_BF16X3, hbs::kTileClampK><<<grid, kBigThreads, 0, s>>>(
-                pr, itb, plan->d_qoff_big[g], plan->nq_big[g], plan->d_queue_big[g]);
-        else
-            grouped_big_kernel<HBS_PREC_FP32, hbs::kTileClampK><<<grid, kBigThreads, 0, s>>>(
-                pr, itb, plan->d_qoff_big[g], plan->nq_big[g], plan->d_queue_big[g]);
+        auto launch = [&](auto kernel) {
+            kernel<<<grid, kBigThreads, 0, s>>>(pr, itb, plan->d_qoff_big[g], plan->nq_big[g],
+                                                plan->d_queue_big[g]);
+        };
+        if (fp32) {
+            if (g == 0) launch(grouped_big_kernel<HBS_PREC_FP32, hbs::kTileClamp>);
+            else if (g == 1) launch(grouped_big_kernel<HBS_PREC_FP32, hbs::kTileClampK>);
+            else launch(grouped_big_kernel<HBS_PREC_FP32, hbs::kTileEdge>);
+        } else {
+            if (g == 0) launch(grouped_big_kernel<HBS_PREC_BF16X3, hbs::kTileClamp>);
+            else if (g == 1) launch(grouped_big_kernel<HBS_PREC_BF16X3, hbs::kTileClampK>);
+            else launch(grouped_big_kernel<HBS_PREC_BF16X3, hbs::kTileEdge>);
+        }
         const int rc = hbs::launch_status();
         if (rc) return rc;
     }

@@ -113,7 +113,8 @@ enum { HBS_PREC_FP32 = 0, HBS_PREC_BF16X3 = 1 };
  * persistent launch (with optional split-K).  HBS_TILE_256x256 runs the whole
  * problem on a 512-thread large-tile launch with the square kernels' tile (no
  * split-K; edge tiles clamp their loads; problems whose K is not a multiple of
- * 32 run on a second such launch that zero-fills past K). */
+ * 32 run on a second such launch that zero-fills past K, and problems whose
+ * operands allow no 16-byte loads on a third with per-element predicates). */
 enum {
     HBS_TILE_128x128 = 0,
     HBS_TILE_64x64 = 1,
@@ -135,12 +136,14 @@ typedef struct {
 
 /* Builds a plan for `count` problems (host array; the matrices are device
  * pointers that must stay valid while the plan is used).  Requirements per
- * problem: M, N, K >= 1, lda >= K, ldb >= N, ldc >= N; A and B 16-byte aligned
- * with N, K, lda, ldb multiples of 4 (HBS_ERR_ALIGN otherwise).  Tile shape
- * and split-K come from `tuning_file` (written by the autotuner) when it is
- * non-NULL -- HBS_ERR_ARG if it cannot be read -- for the size classes it
- * names; every other problem takes HBS_TILE_256x256.  Allocates and uploads
- * the plan's device buffers (synchronous). */
+ * problem: 1 <= M, N, K <= HBS_MAX_DIM, lda >= K, ldb >= N, ldc >= N; any
+ * alignment.  Tile shape and split-K come from `tuning_file` (written by the
+ * autotuner) when it is non-NULL -- HBS_ERR_ARG if it cannot be read -- for
+ * the size classes it names; every other problem takes HBS_TILE_256x256.
+ * Problems whose A or B is not 16-byte aligned, or whose N, K, lda or ldb is
+ * not a multiple of 4, always take HBS_TILE_256x256 (on the launch with
+ * per-element predicated loads).  Allocates and uploads the plan's device
+ * buffers (synchronous). */
 int hbs_grouped_plan_create(const hbs_gemm_problem *problems, int count, int precision,
                             const char *tuning_file, hbs_grouped_plan **plan);
 /* Same with one tile shape and split-K factor (1..64) for every problem (the

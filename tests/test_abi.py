@@ -116,10 +116,13 @@ def test_grouped_plan_argument_checks():
     assert L.hbs_grouped_plan_create(None, 1, 0, None, ctypes.byref(h)) == -2
     assert L.hbs_grouped_plan_create(arr, 1, 7, None, ctypes.byref(h)) == -1
     assert L.hbs_grouped_plan_create(arr, 1, 0, None, None) == -2
-    bad = (hb.Problem * 1)(hb.Problem(16, 32, 48, 8, 6, 8, 8, 6, 8))    # N % 4
-    assert L.hbs_grouped_plan_create(bad, 1, 0, None, ctypes.byref(h)) == -3
-    bad = (hb.Problem * 1)(hb.Problem(20, 32, 48, 8, 8, 8, 8, 8, 8))    # A misaligned
-    assert L.hbs_grouped_plan_create(bad, 1, 0, None, ctypes.byref(h)) == -3
+    # any shape and alignment passes validation (the dry run needs no device)
+    info = hb.GroupedInfo()
+    for odd in (hb.Problem(16, 32, 48, 8, 6, 8, 8, 6, 8),      # N % 4
+                hb.Problem(20, 32, 48, 8, 8, 8, 8, 8, 8)):     # A misaligned
+        arr1 = (hb.Problem * 1)(odd)
+        assert L.hbs_grouped_plan_describe(arr1, 1, 0, None, -1, 1, 256, ctypes.byref(info)) == 0
+        assert info.tiles == 1 and info.workgroups == 1
     bad = (hb.Problem * 1)(hb.Problem(16, 32, 48, 8, 8, 8, 4, 8, 8))    # lda < K
     assert L.hbs_grouped_plan_create(bad, 1, 0, None, ctypes.byref(h)) == -1
     bad = (hb.Problem * 1)(hb.Problem(None, 32, 48, 8, 8, 8, 8, 8, 8))

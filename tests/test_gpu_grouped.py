@@ -488,3 +488,78 @@ def test_square_edge_tiles_exact(N):
         fn(a, b, c, N, 1)
         torch.cuda.synchronize()
         assert np.array_equal(c.view(N, N).cpu().numpy(), expect), (fn.__name__, N)
+
+
+# (M, N, K, lda, ldb, ldc, float offset of A, of B): operands that allow no
+# 16-byte loads (N, K, lda or ldb not a multiple of 4, or A / B one float off
+# a 16-byte boundary) next to ones that do, in one plan
+ODD_PROBLEMS = [
+    (300, 301, 257, 257, 301, 301, 0, 0),
+    (129, 67, 130, 131, 70, 69, 1, 3),
+    (256, 256, 256, 256, 256, 256, 1, 0),       # aligned shape, A misaligned
+    (64, 128, 96, 96, 128, 128, 0, 2),          # aligned shape, B misaligned
+    (1, 1, 1, 1, 1, 1, 0, 0),
+    (513, 260, 64, 64, 260, 260, 0, 0),         # aligned: the clamped launch
+    (70, 90, 100, 100, 92, 90, 0, 0),           # aligned: zero fill past K
+]
+
+
+def odd_group(kind, seed=9):
+    offs, ta, tb, tc = [], 0, 0, 0
+    for (M, N, K, lda, ldb, ldc, oa, ob) in ODD_PROBLEMS:
+        offs.append((ta + oa, tb + ob, tc))
+        ta += (M * lda + oa + 63) // 64 * 64
+        tb += (K * ldb + ob + 63) // 64 * 64
+        tc += (M * ldc + 63) // 64 * 64
+    a = torch.empty(ta, dtype=torch.float32, device="cuda")
+    b = torch.empty(tb, dtype=torch.float32, device="cuda")
+    c = torch.full((tc,), -3.0, dtype=torch.float32, device="cuda")
+    if kind == "pattern":
+        hb.fill_pattern(a, torch.empty(ta, device="cuda"), ta)
+        hb.fill_pattern(torch.empty(tb, device="cuda"), b, tb)
+    else:
+        hb.fill_random(a, ta, hb.GEN_UNIFORM, seed)
+        hb.fill_random(b, tb, hb.GEN_UNIFORM, seed + 1)
+    probs, views = [], []
+    for (M, N, K, lda, ldb, ldc, _, _), (oa, ob, oc) in zip(ODD_PROBLEMS, offs):
+        A = a[oa:oa + M * lda].view(M, lda)
+        B = b[ob:ob + K * ldb].view(K, ldb)
+        C = c[oc:oc + M * ldc].view(M, ldc)
+        probs.append((A, B, C, M, N, K, lda, ldb, ldc))
+        views.append((A[:, :K], B[:, :N], C, M, N, K))
+    return probs, views, (a, b, c)
+
+
+@pytest.mark.parametrize("prec", [hb.PREC_FP32, hb.PREC_BF16X3])
+def test_grouped_any_shape_and_alignment(prec):
+    """Problems of any shape, leading dimensions and operand alignment in one
+    plan (three large-tile launches: clamped, zero fill past K, predicated):
+    exact on small-integer inputs, within the FP64 bounds on uniform ones, the
+    columns between N and ldc untouched, and a graph replay equal to the eager
+    run."""
+    probs, views, keep = odd_group("pattern")
+    plan = hb.GroupedPlan(probs, prec)
+    plan.run()
+    torch.cuda.synchronize()
+    for (A, B, C, M, N, K) in views:
+        R = A.double() @ B.double()
+        assert torch.equal(C[:, :N], R.float()), (M, N, K)
+        assert bool((C[:, N:] == -3.0).all()), (M, N, K)
+    probs, views, keep = odd_group("uniform")
+    plan = hb.GroupedPlan(probs, prec)
+    plan.run()
+    torch.cuda.synchronize()
+    eager = keep[2].clone()
+    for (A, B, C, M, N, K) in views:
+        R, S = oracle.gemm_f64(A.cpu().numpy(), B.cpu().numpy())
+        Cn = C[:, :N].cpu().numpy()
+        bound = (gamma(K + 2) if prec == hb.PREC_FP32
+                 else 3.1 * 2.0 ** -16 + gamma(3 * K + 2)) * S
+        assert np.all(np.abs(Cn - R) <= bound), (M, N, K)
+    g = plan.graph()
+    for _ in range(2):
+        keep[2].fill_(-3.0)
+        g.launch()
+        torch.cuda.synchronize()
+        assert torch.equal(keep[2], eager)
+    g.close()

@@ -31,6 +31,10 @@ pytestmark = pytest.mark.gpu
      "--split", "2"],
     ["--N", "300", "--NITER", "2", "--mode", "grouped", "--tile", "1", "--split", "3",
      "--input", "uniform", "--golden", "f64", "--reps", "2"],
+    # N = 301: slices start off 16-byte boundaries and N is odd, so the
+    # grouped plan runs them on its predicated large-tile launch
+    ["--N", "301", "--NITER", "3", "--mode", "grouped", "--input", "uniform", "--golden", "f64"],
+    ["--N", "301", "--NITER", "2", "--mode", "grouped", "--precision", "bf16x3"],
     ["--N", "512", "--NITER", "2", "--mode", "grouped", "--tuning",
      os.path.join(ROOT, "hb_hammerbench_amd", "tuning", "mi355x_grouped.txt")],
 ])

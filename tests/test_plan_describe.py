@@ -9,6 +9,9 @@ Expected counts follow from the planning rules:
   * with a forced tile 0..3 a problem takes the 256-thread launch with that
     tile and split-K s: a K range is ceil(K / s) rounded up to the stage depth
     (16 for FP32, 32 for BF16x3), giving ceil(K / range) work items per tile;
+  * a problem whose operands allow no 16-byte loads (here: N or K not a
+    multiple of 4, with lda = K and ldb = N) takes a third large-tile launch
+    with per-element predicated loads, whatever the tuning file or forced tile;
   * workgroups: one per CU on each large-tile launch, two per CU on the other,
     never more than the launch's work items.
 """
@@ -24,11 +27,13 @@ TILE = {0: (128, 128), 1: (64, 64), 2: (128, 64), 3: (64, 128)}
 
 def expected(shapes, prec, cus=256, tile=None, split=1):
     stage = 16 if prec == hb.PREC_FP32 else 32
-    big_tiles = [0, 0]  # large-tile launches: K a multiple of 32, other K
+    big_tiles = [0, 0, 0]  # large-tile launches: K a multiple of 32, other K, predicated
     small_tiles = small_items = split_tiles = 0
     for M, N, K in shapes:
-        if tile is None or tile == 4:
-            big_tiles[0 if K % 32 == 0 else 1] += math.ceil(M / 256) * math.ceil(N / 256)
+        vec = N % 4 == 0 and K % 4 == 0
+        if tile is None or tile == 4 or not vec:
+            g = 2 if not vec else 0 if K % 32 == 0 else 1
+            big_tiles[g] += math.ceil(M / 256) * math.ceil(N / 256)
             continue
         BM, BN = TILE[tile]
         n = math.ceil(M / BM) * math.ceil(N / BN)
@@ -57,7 +62,7 @@ def check(info, want, count):
 def test_describe_default_plan(prec, cus):
     info = hb.describe_plan(MIXED, prec, cus=cus)
     want = expected(MIXED, prec, cus)
-    # every problem on the two large-tile launches, no split-K
+    # every problem on the large-tile launches, no split-K
     assert want["tiles"] > 20 and want["split_tiles"] == 0
     assert want["workgroups"] == min(cus, 35) + min(cus, 8)  # tiles with K % 32 == 0, others
     check(info, want, len(MIXED))
@@ -89,7 +94,6 @@ def test_describe_rejects_bad_arguments(tmp_path):
             hb.describe_plan(*a, **k)
         return e.value.code
     assert code([]) == -1                                   # no problems
-    assert code([(64, 62, 64)]) == -3                      # N not a multiple of 4
     assert code([(0, 64, 64)]) == -1
     assert code([(64, 64, 64)], precision=7) == -1
     assert code([(64, 64, 64)], tile=5) == -1
@@ -97,6 +101,19 @@ def test_describe_rejects_bad_arguments(tmp_path):
     assert code([(64, 64, 64)], tile=0, tuning_file=str(tmp_path / "t.txt")) == -1
     assert code([(64, 64, 64)], tuning_file=str(tmp_path / "missing.txt")) == -1
     assert code([(64, 64, 64)], cus=0) == -1
+
+
+@pytest.mark.parametrize("prec", [hb.PREC_FP32, hb.PREC_BF16X3])
+def test_describe_any_shape_takes_the_predicated_launch(prec):
+    """N or K not a multiple of 4 is accepted: such problems run whole on the
+    third large-tile launch (per-element predicated loads), also when a tile
+    shape and split-K are forced for the others."""
+    shapes = [(64, 62, 64), (300, 301, 257), (1, 1, 1), (512, 512, 512), (256, 255, 64)]
+    for tile, split in ((None, 1), (0, 4), (4, 1)):
+        info = hb.describe_plan(shapes, prec, tile=tile, split_k=split)
+        check(info, expected(shapes, prec, tile=tile, split=split), len(shapes))
+    info = hb.describe_plan(shapes[:2], prec, tile=1, split_k=8)
+    assert info["split_tiles"] == 0 and info["tiles"] == 1 + 2 * 2
 
 
 @pytest.mark.parametrize("seed", range(6))
@@ -107,6 +124,9 @@ def test_describe_random_groups(seed):
     rng = np.random.default_rng(seed)
     shapes = [(int(rng.integers(1, 3000)), 4 * int(rng.integers(1, 700)),
                4 * int(rng.integers(1, 1100))) for _ in range(int(rng.integers(1, 60)))]
+    # a few shapes whose N or K is not a multiple of 4
+    shapes += [(int(rng.integers(1, 600)), int(rng.integers(1, 700)), int(rng.integers(1, 900)))
+               for _ in range(int(rng.integers(0, 6)))]
     cus = int(rng.choice([8, 24, 104, 256]))
     for prec in (hb.PREC_FP32, hb.PREC_BF16X3):
         check(hb.describe_plan(shapes, prec, cus=cus), expected(shapes, prec, cus), len(shapes))
