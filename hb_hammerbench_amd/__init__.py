@@ -119,6 +119,27 @@ def lib():
     return _lib
 
 
+_planner = None
+
+
+def planner():
+    """The library that runs the grouped planner's dry run: libhbsgemm.so, or
+    the host-only build named by HBS_PLANNER_LIB (the sanitizer build of the
+    planner, hb_hammerbench_amd/_asan/libhbsplan_asan.so)."""
+    global _planner
+    if _planner is None:
+        path = os.environ.get("HBS_PLANNER_LIB")
+        if path:
+            P = ctypes.CDLL(path)
+            res, args = SIGNATURES["hbs_grouped_plan_describe"]
+            P.hbs_grouped_plan_describe.restype = res
+            P.hbs_grouped_plan_describe.argtypes = args
+            _planner = P
+        else:
+            _planner = lib()
+    return _planner
+
+
 def _check(code, where):
     if code != 0:
         raise HbsError(code, where)
@@ -234,7 +255,7 @@ def describe_plan(shapes, precision=PREC_FP32, tuning_file=None, tile=None, spli
         arr[i] = Problem(fake, fake, fake, M, N, K, K, N, N)
     out = GroupedInfo()
     path = tuning_file.encode() if tuning_file else None
-    _check(lib().hbs_grouped_plan_describe(arr, len(shapes), precision, path,
+    _check(planner().hbs_grouped_plan_describe(arr, len(shapes), precision, path,
                                            -1 if tile is None else tile, split_k, cus,
                                            ctypes.byref(out)), "hbs_grouped_plan_describe")
     return {name: getattr(out, name) for name, _ in GroupedInfo._fields_}

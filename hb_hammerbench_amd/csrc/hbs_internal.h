@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "../../include/hbsgemm.h"
+#include "hbs_host.h"
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
@@ -16,20 +17,6 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
     } while (0)
 
 namespace hbs {
-
-// Largest M, N, K and leading dimension any entry point accepts.  Tile and
-// stage arithmetic runs on int (a tile origin plus up to 256 rows, a K range
-// rounded up to a 32-deep stage or a 128-deep quantisation tile, split-K chunk
-// ends), which must not overflow; larger values get HBS_ERR_ARG.
-constexpr int kMaxDim = 0x7fffffff - 4096;
-
-inline int64_t ceil_div(int64_t x, int64_t b) { return (x + b - 1) / b; }
-
-// Whether every size and leading dimension is at most kMaxDim.
-inline bool dims_ok(int M, int N, int K, int lda, int ldb, int ldc) {
-    return M <= kMaxDim && N <= kMaxDim && K <= kMaxDim && lda <= kMaxDim && ldb <= kMaxDim &&
-           ldc <= kMaxDim;
-}
 
 // Load modes of a block tile (tile_f32.h, tile_bf16x3.h):
 //   kTileInterior  the tile lies inside M x N, K is a multiple of the stage
@@ -214,8 +201,6 @@ inline int launch_status() {
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? HBS_SUCCESS : (int)e;
 }
-
-inline bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
 
 // Workgroup id remap so that each XCD (workgroups are dealt to the 8 XCDs
 // round-robin by id) receives one contiguous range of logical tile ids, which

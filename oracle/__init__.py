@@ -11,7 +11,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# HBS_ORACLE_LIB selects another build of the same source (the sanitizer
+# build, oracle/_asan/liboracle.so, in tests/test_sanitizers.py)
+_LIB_PATH = os.environ.get("HBS_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
