@@ -28,6 +28,7 @@ _LIB_NAME = "libhbsgemm.so"
 GEN_UNIFORM, GEN_LOGUNIFORM, GEN_NORMAL_OUTLIER = 0, 1, 2
 PREC_FP32, PREC_BF16X3 = 0, 1
 TILE_128x128, TILE_64x64, TILE_128x64, TILE_64x128, TILE_256x256 = 0, 1, 2, 3, 4
+TILE_128x128_4x1, TILE_128x128_BK32, TILE_128x128_8W, TILE_256x128, TILE_128x256 = 5, 6, 7, 8, 9
 
 _lib = None
 

@@ -3,8 +3,16 @@
 // No reference counterpart: the reference fixes its tile-group shape at build
 // time (apps/sgemm/template.mk:7-9) and sweeps shapes by hand in legacy test
 // lists (apps/legacy/gemm/sgemm_512/tests.mk:1-6).  This program measures every
-// candidate (tile shape x split-K) per size class on the device and writes the
-// tuning file hbs_grouped_plan_create reads:
+// candidate per size class on the device and writes the tuning file
+// hbs_grouped_plan_create reads.  A candidate is a tile configuration -- tile
+// shape, wave count and arrangement, K stage depth and launch, the HBS_TILE_*
+// ids of include/hbsgemm.h (those the precision has) -- times a split-K factor
+// (1, 2, 4; the 256-thread tiles only).  The configurations span
+//   tile shape  256 x 256, 256 x 128, 128 x 256, 128 x 128, 128 x 64, 64 x 128, 64 x 64;
+//   waves       8 (large-tile launch, one workgroup per CU) or 4 (two per CU),
+//               4 waves as 2 x 2 or 4 x 1 (128 x 128, FP32);
+//   K stage     16 or 32 (FP32 128 x 128 on the 256-thread launch).
+// Tuning file lines:
 //   <fp32|bf16x3> <M class> <N class> <K class> <tile id> <split> <ns per work item>
 // A size class is the power of two nearest the size in log scale (sizes in
 // [c / sqrt2, c * sqrt2) map to c), clamped to [64, 2048].
@@ -17,7 +25,7 @@
 //  * --group FILE (lines "M N K"): tunes for that actual problem set.  Starting
 //    from the plan's default it visits the size classes in decreasing
 //    share of the flops, times the whole group with each candidate override of
-//    that class (tile ids 0..4, split-K 1, 2, 4), and keeps the best override
+//    that class (every configuration x split-K), and keeps the best override
 //    only when it runs >= 2% faster and a confirming re-time with twice the
 //    repetitions still shows >= 1% against the table accepted so far, so run
 //    to run noise does not turn into overrides.
@@ -70,6 +78,24 @@ static int size_class(int x) {  // same rule as the plan
 struct Override {
     int mc, nc, kc, tile, split;
 };
+
+// Tile configurations (HBS_TILE_*) and whether they run on the 256-thread
+// launch, where split-K applies.
+constexpr int kTileIds = 10;
+static bool small_tile(int tile) { return tile <= 3 || tile == 5 || tile == 6; }
+static const char *tile_name(int tile) {
+    static const char *names[kTileIds] = {
+        "128x128 4w 2x2", "64x64 4w 2x2", "128x64 4w 2x2", "64x128 4w 2x2", "256x256 8w",
+        "128x128 4w 4x1", "128x128 4w 2x2 BK32", "128x128 8w", "256x128 8w", "128x256 8w"};
+    return tile >= 0 && tile < kTileIds ? names[tile] : "?";
+}
+// Whether the precision has the configuration (the planner's own check, on a
+// dry run of one small problem).
+static bool tile_supported(int prec, int tile) {
+    hbs_gemm_problem p{(const float *)256, (const float *)256, (float *)256, 64, 64, 64, 64, 64, 64};
+    hbs_grouped_info info;
+    return hbs_grouped_plan_describe(&p, 1, prec, nullptr, tile, 1, 256, &info) == HBS_SUCCESS;
+}
 
 static void write_table(const char *path, const char *pname, const std::vector<Override> &ov,
                         const char *comment) {
@@ -126,7 +152,7 @@ static int group_mode(const char *shapes_path, const char *out_path, const std::
     }
     int m, n, k;
     while (fscanf(f, "%d %d %d", &m, &n, &k) == 3) {
-        if (m < 1 || n < 1 || k < 1 || n % 4 || k % 4) {
+        if (m < 1 || n < 1 || k < 1) {
             fprintf(stderr, "bad shape %d %d %d\n", m, n, k);
             return 2;
         }
@@ -174,6 +200,7 @@ static int group_mode(const char *shapes_path, const char *out_path, const std::
     for (int prec : precs) {
         const char *pname = prec == HBS_PREC_FP32 ? "fp32" : "bf16x3";
         std::vector<Override> ov;
+        std::string margins;  // per class: the fastest candidate against the table
         const double base = time_group(probs, prec, "", reps, e0, e1);
         double best = base;
         printf("%s group of %d: default %.3f ms (%.1f TFLOP/s)\n", pname, count, base,
@@ -184,13 +211,21 @@ static int group_mode(const char *shapes_path, const char *out_path, const std::
             const int mc = std::get<0>(cl.second), nc = std::get<1>(cl.second),
                       kc = std::get<2>(cl.second);
             Override keep{mc, nc, kc, -1, 0};
-            for (int tile = 0; tile <= HBS_TILE_256x256; ++tile) {
+            double class_best = 1e30;
+            int class_tile = -1, class_split = 0;
+            for (int tile = 0; tile < kTileIds; ++tile) {
+                if (!tile_supported(prec, tile)) continue;
                 for (int split : {1, 2, 4}) {
-                    if (split > 1 && (kc < 256 || tile == HBS_TILE_256x256)) continue;
+                    if (split > 1 && (kc < 256 || !small_tile(tile))) continue;
                     std::vector<Override> trial = ov;
                     trial.push_back(Override{mc, nc, kc, tile, split});
                     write_table(tmp.c_str(), pname, trial, nullptr);
                     const double t = time_group(probs, prec, tmp.c_str(), reps, e0, e1);
+                    if (t < class_best) {
+                        class_best = t;
+                        class_tile = tile;
+                        class_split = split;
+                    }
                     if (t < best * 0.98) {
                         best = t;
                         keep.tile = tile;
@@ -214,9 +249,19 @@ static int group_mode(const char *shapes_path, const char *out_path, const std::
                     best = t_cur;
                 }
             }
-            printf("%s class M%d N%d K%d (%.1f%% of flops): %s -> %.3f ms\n", pname, mc, nc, kc,
-                   100.0 * cl.first, keep.tile >= 0 ? "override" : "default", best);
+            printf("%s class M%d N%d K%d (%.1f%% of flops): %s -> %.3f ms; fastest candidate "
+                   "tile %d (%s) split %d at %.3f ms\n",
+                   pname, mc, nc, kc, 100.0 * cl.first, keep.tile >= 0 ? "override" : "default",
+                   best, class_tile, tile_name(class_tile), class_split, class_best);
             fflush(stdout);
+            char cline[320];
+            snprintf(cline, sizeof cline,
+                     "#   %s class %d %d %d (%.1f%% of flops): %s; fastest candidate tile %d "
+                     "(%s) split %d %.3f ms against %.3f ms\n",
+                     pname, mc, nc, kc, 100.0 * cl.first,
+                     keep.tile >= 0 ? "override kept" : "default kept", class_tile,
+                     tile_name(class_tile), class_split, class_best, best);
+            margins += cline;
         }
         if (ov.empty()) best = base;  // no override: the re-times above were noise
         printf("%s tuned: %.3f ms (%.1f TFLOP/s), default %.3f ms, %zu overrides\n", pname, best,
@@ -226,6 +271,7 @@ static int group_mode(const char *shapes_path, const char *out_path, const std::
         snprintf(line, sizeof line, "# %s: group of %d problems from %s: default %.3f ms, tuned %.3f ms\n",
                  pname, count, shapes_path, base, best);
         merged += line;
+        merged += margins;
         for (const Override &o : ov) {
             snprintf(line, sizeof line, "%s %d %d %d %d %d 0\n", pname, o.mc, o.nc, o.kc, o.tile, o.split);
             merged += line;
@@ -306,7 +352,7 @@ int main(int argc, char **argv) {
     fprintf(out, "# hbsgemm grouped tuning file: prec Mclass Nclass Kclass tile split ns_per_item\n");
     for (int prec : precs) {
         const char *pname = prec == HBS_PREC_FP32 ? "fp32" : "bf16x3";
-        const int ntiles = HBS_TILE_256x256 + 1;
+        const int ntiles = kTileIds;
         for (int mc : classes) {
             for (int nc : classes) {
                 for (int kc : classes) {
@@ -322,10 +368,9 @@ int main(int argc, char **argv) {
                     }
                     Probe best{-1, 0, 1e30, 0.0};
                     for (int tile = 0; tile < ntiles; ++tile) {
-                        // the large-tile path needs a 256 x 256 core (no split-K)
-                        if (tile == HBS_TILE_256x256 && (mc < 256 || nc < 256)) continue;
+                        if (!tile_supported(prec, tile)) continue;
                         for (int split : {1, 2, 4}) {
-                            if (split > 1 && (kc < 512 || tile == HBS_TILE_256x256)) continue;
+                            if (split > 1 && (kc < 512 || !small_tile(tile))) continue;
                             hbs_grouped_plan *plan = nullptr;
                             int rc = hbs_grouped_plan_create_forced(probs.data(), count, prec, tile,
                                                                     split, &plan);
@@ -357,8 +402,9 @@ int main(int argc, char **argv) {
                     const double tflops = 2.0 * mc * nc * kc * count / (best.ms * 1e-3) / 1e12;
                     fprintf(out, "%s %d %d %d %d %d %.1f\n", pname, mc, nc, kc, best.tile,
                             best.split, best.ns_item);
-                    printf("%s M%-5d N%-5d K%-5d -> tile %d split %d  %.3f ms  %.1f TFLOP/s\n",
-                           pname, mc, nc, kc, best.tile, best.split, best.ms, tflops);
+                    printf("%s M%-5d N%-5d K%-5d -> tile %d (%s) split %d  %.3f ms  %.1f TFLOP/s\n",
+                           pname, mc, nc, kc, best.tile, tile_name(best.tile), best.split, best.ms,
+                           tflops);
                     fflush(stdout);
                 }
             }

@@ -75,16 +75,17 @@ constexpr int kBigThreads = 512;
 // ---------------------------------------------------------------------------
 // shared helpers
 // ---------------------------------------------------------------------------
-// Epilogue for a 2x2-wave tile of FMxFN 32x32 accumulator blocks per wave.
+// Epilogue for a tile of WM x WN waves, each with FM x FN 32x32 accumulator
+// blocks (wave w at wm = w % WM, wn = w / WM, as the tile configs place it).
 // Direct tiles store C; split tiles store a partial, and the last split to
 // arrive sums all partials in split order and stores C.
-template <int BM, int BN>
-__device__ __forceinline__ void finish_tile(floatx16 (&acc)[BM / 64][BN / 64], const Prob &p,
-                                            const Item &it, float *partials, int *counters,
-                                            int *s_flag) {
-    constexpr int FM = BM / 64, FN = BN / 64;
+template <int BM, int BN, int WM, int WN>
+__device__ __forceinline__ void finish_tile(floatx16 (&acc)[BM / WM / 32][BN / WN / 32],
+                                            const Prob &p, const Item &it, float *partials,
+                                            int *counters, int *s_flag) {
+    constexpr int FM = BM / WM / 32, FN = BN / WN / 32;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
+    const int wm = wave % WM, wn = wave / WM;
     const int l32 = lane & 31, h = lane >> 5;
     const int m0 = it.tm * BM, n0 = it.tn * BN;
 
@@ -111,8 +112,8 @@ __device__ __forceinline__ void finish_tile(floatx16 (&acc)[BM / 64][BN / 64], c
 #pragma unroll 1
         for (int b = 0; b < FM * FN; ++b) {
             const int r = b / FN, c = b % FN;
-            const int row0 = m0 + wm * (BM / 2) + r * 32 + 4 * h;
-            const int col = n0 + wn * (BN / 2) + c * 32 + l32;
+            const int row0 = m0 + wm * (BM / WM) + r * 32 + 4 * h;
+            const int col = n0 + wn * (BN / WN) + c * 32 + l32;
             float *cp = p.C + (int64_t)row0 * p.ldc + col;
             const float *src0 = partials + it.poff + wave * (FM * FN * 16 * 64) + b * 16 * 64 + lane;
             float v[16];
@@ -138,10 +139,10 @@ __device__ __forceinline__ void finish_tile(floatx16 (&acc)[BM / 64][BN / 64], c
     // wave-uniform multiples of ldc
 #pragma unroll
     for (int r = 0; r < FM; ++r) {
-        const int row0 = m0 + wm * (BM / 2) + r * 32 + 4 * h;
+        const int row0 = m0 + wm * (BM / WM) + r * 32 + 4 * h;
 #pragma unroll
         for (int c = 0; c < FN; ++c) {
-            const int col = n0 + wn * (BN / 2) + c * 32 + l32;
+            const int col = n0 + wn * (BN / WN) + c * 32 + l32;
             float *cp = p.C + (int64_t)row0 * p.ldc + col;
             const bool colok = col < p.N;
 #pragma unroll
@@ -160,10 +161,10 @@ __device__ __forceinline__ void finish_tile(floatx16 (&acc)[BM / 64][BN / 64], c
 // 16-byte-loadable A and B and N, K multiples of 4; split ranges start on a
 // stage).
 // ---------------------------------------------------------------------------
-template <int BM, int BN>
+template <int BM, int BN, int BK, int WM, int WN>
 __device__ void f32_tile(float *smem, const Prob &p, const Item &it, float *partials,
                          int *counters, int *s_flag) {
-    using G = hbs::F32Cfg<BM, BN, 16, 2, 2>;
+    using G = hbs::F32Cfg<BM, BN, BK, WM, WN>;
     floatx16 acc[G::kFM][G::kFN];
     const int m0 = it.tm * BM, n0 = it.tn * BN, K = it.k1 - it.k0;
     const float *Ag = p.A + (int64_t)m0 * p.lda + it.k0;
@@ -171,14 +172,14 @@ __device__ void f32_tile(float *smem, const Prob &p, const Item &it, float *part
     const hbs::TileArgs t{Ag, Bg, p.M, p.N, K, p.lda, p.ldb, m0, n0};
     typename G::Regs regs;
     hbs::NoChainGlobal ch;  // operands from the problem table: global loads
-    hbs::f32_block_mma<BM, BN, 16, 2, 2, hbs::kTileClampK>(smem, t, acc, regs, ch);
-    finish_tile<BM, BN>(acc, p, it, partials, counters, s_flag);
+    hbs::f32_block_mma<BM, BN, BK, WM, WN, hbs::kTileClampK>(smem, t, acc, regs, ch);
+    finish_tile<BM, BN, WM, WN>(acc, p, it, partials, counters, s_flag);
 }
 
-template <int BM, int BN>
+template <int BM, int BN, int WM, int WN>
 __device__ void b3_tile(uint16_t *smem, const Prob &p, const Item &it, float *partials,
                         int *counters, int *s_flag) {
-    using G = hbs::B3Cfg<BM, BN, 32, 2, 2>;
+    using G = hbs::B3Cfg<BM, BN, 32, WM, WN>;
     floatx16 acc[G::kFM][G::kFN];
     const int m0 = it.tm * BM, n0 = it.tn * BN, K = it.k1 - it.k0;
     const float *Ag = p.A + (int64_t)m0 * p.lda + it.k0;
@@ -187,11 +188,14 @@ __device__ void b3_tile(uint16_t *smem, const Prob &p, const Item &it, float *pa
     typename G::Regs regs;
     hbs::NoChainGlobal ch;  // operands from the problem table: global loads
     hbs::bf16x3_block_mma<G, hbs::kTileClampK>(smem, t, acc, regs, ch);
-    finish_tile<BM, BN>(acc, p, it, partials, counters, s_flag);
+    finish_tile<BM, BN, WM, WN>(acc, p, it, partials, counters, s_flag);
 }
 
-constexpr int kLdsF32 = hbs::F32Cfg<128, 128, 16, 2, 2>::kLdsFloats * 4;
+// LDS of the largest configuration of each precision (FP32: 128 x 128 with
+// BK = 32, still two workgroups per CU)
+constexpr int kLdsF32 = hbs::F32Cfg<128, 128, 32, 2, 2>::kLdsFloats * 4;
 constexpr int kLdsB3 = 2 * hbs::B3Cfg<128, 128, 32, 2, 2>::kStage * 2;
+static_assert(2 * kLdsF32 <= 160 * 1024 && 2 * kLdsB3 <= 160 * 1024, "two workgroups per CU");
 
 template <int PREC>
 __global__ __launch_bounds__(kThreads, 2) void grouped_kernel(const Prob *__restrict__ probs,
@@ -216,19 +220,22 @@ __global__ __launch_bounds__(kThreads, 2) void grouped_kernel(const Prob *__rest
         const Prob p = probs[it.prob];
         if constexpr (PREC == HBS_PREC_FP32) {
             float *s = (float *)smem;
+            // tile ids of kTileCfg (grouped_plan.h) on this launch
             switch (it.cfg) {
-                case 0: f32_tile<128, 128>(s, p, it, partials, counters, &s_flag); break;
-                case 1: f32_tile<64, 64>(s, p, it, partials, counters, &s_flag); break;
-                case 2: f32_tile<128, 64>(s, p, it, partials, counters, &s_flag); break;
-                default: f32_tile<64, 128>(s, p, it, partials, counters, &s_flag); break;
+                case 0: f32_tile<128, 128, 16, 2, 2>(s, p, it, partials, counters, &s_flag); break;
+                case 1: f32_tile<64, 64, 16, 2, 2>(s, p, it, partials, counters, &s_flag); break;
+                case 2: f32_tile<128, 64, 16, 2, 2>(s, p, it, partials, counters, &s_flag); break;
+                case 3: f32_tile<64, 128, 16, 2, 2>(s, p, it, partials, counters, &s_flag); break;
+                case 5: f32_tile<128, 128, 16, 4, 1>(s, p, it, partials, counters, &s_flag); break;
+                default: f32_tile<128, 128, 32, 2, 2>(s, p, it, partials, counters, &s_flag); break;
             }
         } else {
             uint16_t *s = (uint16_t *)smem;
             switch (it.cfg) {
-                case 0: b3_tile<128, 128>(s, p, it, partials, counters, &s_flag); break;
-                case 1: b3_tile<64, 64>(s, p, it, partials, counters, &s_flag); break;
-                case 2: b3_tile<128, 64>(s, p, it, partials, counters, &s_flag); break;
-                default: b3_tile<64, 128>(s, p, it, partials, counters, &s_flag); break;
+                case 0: b3_tile<128, 128, 2, 2>(s, p, it, partials, counters, &s_flag); break;
+                case 1: b3_tile<64, 64, 2, 2>(s, p, it, partials, counters, &s_flag); break;
+                case 2: b3_tile<128, 64, 2, 2>(s, p, it, partials, counters, &s_flag); break;
+                default: b3_tile<64, 128, 2, 2>(s, p, it, partials, counters, &s_flag); break;
             }
         }
         if (threadIdx.x == 0) s_next = next;
@@ -247,24 +254,20 @@ __global__ __launch_bounds__(kThreads, 2) void grouped_kernel(const Prob *__rest
 }
 
 // ---------------------------------------------------------------------------
-// Large-tile launch: one 512-thread workgroup per CU takes 256 x 256 tiles
-// from its XCD's queue and runs them with the square kernels' tile code (waves
-// whose sub-tile lies outside M x N skip their MFMAs).  A problem on this path
-// is covered whole: tiles at its ragged bottom and right edges read clamped
-// rows and columns (kTileClamp) and store only what lies inside M x N.  No
-// split-K.  MODE kTileClamp takes problems whose K is a multiple of the
-// 32-deep stage; kTileEdge (a launch of its own) problems whose A or B is not
-// 16-byte aligned or whose N, K, lda or ldb is not a multiple of 4, with
-// per-element predicated loads; kTileClampK (a launch of its own) the others,
-// zero-filling
-// past K in the last stage.  One kernel holding both modes ran out of
+// Large-tile launch: one 512-thread workgroup per CU takes BM x BN tiles (8
+// waves as WM x WN) from its XCD's queue and runs them with the square
+// kernels' tile code; the default 256 x 256 tile spreads the valid part of an
+// edge tile over the SIMDs.  A problem on this path is covered whole: tiles
+// at its ragged bottom and right edges read clamped rows and columns
+// (kTileClamp) and store only what lies inside M x N.  No split-K.  MODE
+// kTileClamp takes problems whose K is a multiple of the 32-deep stage;
+// kTileClampK (a launch of its own) the others, zero-filling past K in the
+// last stage; kTileEdge (a launch of its own, 256 x 256 only) problems whose A
+// or B is not 16-byte aligned or whose N, K, lda or ldb is not a multiple of 4,
+// with per-element predicated loads.  One kernel holding two modes ran out of
 // registers and cost the first kind 5%.
 // ---------------------------------------------------------------------------
-using BigF32 = hbs::F32Cfg<256, 256, 32, 4, 2>;
-using BigB3 = hbs::B3Main;
-static_assert(BigF32::kThreads == kBigThreads && BigB3::kThreads == kBigThreads, "big tiles");
-
-template <int PREC, int MODE>
+template <int PREC, int MODE, int BM, int BN, int WM, int WN>
 __global__ __launch_bounds__(kBigThreads) void grouped_big_kernel(const Prob *__restrict__ probs,
                                                                   const Item *__restrict__ items,
                                                                   const int *__restrict__ qoff,
@@ -272,9 +275,12 @@ __global__ __launch_bounds__(kBigThreads) void grouped_big_kernel(const Prob *__
     constexpr bool kF32 = PREC == HBS_PREC_FP32;
     static_assert(MODE == hbs::kTileClamp || MODE == hbs::kTileClampK || MODE == hbs::kTileEdge,
                   "mode");
-    constexpr int LDS = kF32 ? BigF32::kLdsFloats * 4 : 2 * BigB3::kStage * 2;
-    using Regs = std::conditional_t<kF32, BigF32::Regs, BigB3::Regs>;
-    static_assert(BigF32::kFM == BigB3::kFM && BigF32::kFN == BigB3::kFN, "accumulators");
+    using GF = hbs::F32Cfg<BM, BN, 32, WM, WN>;
+    using GB = hbs::B3Cfg<BM, BN, 32, WM, WN>;
+    static_assert(GF::kThreads == kBigThreads && GB::kThreads == kBigThreads, "8 waves");
+    constexpr int LDS = kF32 ? GF::kLdsFloats * 4 : 2 * GB::kStage * 2;
+    using Regs = std::conditional_t<kF32, typename GF::Regs, typename GB::Regs>;
+    static_assert(GF::kFM == GB::kFM && GF::kFN == GB::kFN, "accumulators");
     __shared__ __attribute__((aligned(16))) char smem[LDS];
     __shared__ int s_next[2];
     // queue q = blockIdx.x % nq (the workgroup's XCD when nq = 8) hands out
@@ -288,7 +294,7 @@ __global__ __launch_bounds__(kBigThreads) void grouped_big_kernel(const Prob *__
     auto tile = [&](int i, hbs::TileArgs &t, float *&Cg, int &ldc) {
         const Item it = items[i];
         const Prob p = probs[it.prob];
-        const int m0 = it.tm * kBig, n0 = it.tn * kBig;
+        const int m0 = it.tm * BM, n0 = it.tn * BN;
         t = hbs::TileArgs{p.A + (int64_t)m0 * p.lda, p.B + n0, p.M, p.N, p.K, p.lda, p.ldb, m0, n0};
         Cg = p.C;
         ldc = p.ldc;
@@ -317,17 +323,16 @@ __global__ __launch_bounds__(kBigThreads) void grouped_big_kernel(const Prob *__
         };
         auto ch = hbs::make_chain(publish, peek);
         Regs regs;
-        floatx16 acc[BigF32::kFM][BigF32::kFN];
+        floatx16 acc[GF::kFM][GF::kFN];
         for (;;) {
             if (threadIdx.x == 0) nxt = fetch();
             if constexpr (kF32) {
-                hbs::f32_block_mma<256, 256, 32, 4, 2, MODE>((float *)smem, t, acc, regs, ch);
-                hbs::f32_block_store<256, 256, 32, 4, 2, hbs::kTileClamp>(Cg, t.M, t.N, ldc, t.m0,
+                hbs::f32_block_mma<BM, BN, 32, WM, WN, MODE>((float *)smem, t, acc, regs, ch);
+                hbs::f32_block_store<BM, BN, 32, WM, WN, hbs::kTileClamp>(Cg, t.M, t.N, ldc, t.m0,
                                                                           t.n0, acc);
             } else {
-                hbs::bf16x3_block_mma<BigB3, MODE>((uint16_t *)smem, t, acc, regs, ch);
-                hbs::bf16x3_block_store<BigB3, hbs::kTileClamp>(Cg, t.M, t.N, ldc, t.m0, t.n0,
-                                                                acc);
+                hbs::bf16x3_block_mma<GB, MODE>((uint16_t *)smem, t, acc, regs, ch);
+                hbs::bf16x3_block_store<GB, hbs::kTileClamp>(Cg, t.M, t.N, ldc, t.m0, t.n0, acc);
             }
             if (!ch.has_next) break;
             t = ch.next;
@@ -460,14 +465,20 @@ extern "C" int hbs_grouped_plan_create_forced(const hbs_gemm_problem *problems, 
 extern "C" int hbs_grouped_plan_info(const hbs_grouped_plan *plan, hbs_grouped_info *info) {
     if (!plan || !info) return HBS_ERR_NULL;
     info->problems = plan->count;
-    info->workgroups =
-        plan->nblocks + plan->nblocks_big[0] + plan->nblocks_big[1] + plan->nblocks_big[2];
+    info->workgroups = plan->nblocks;
+    info->work_items = plan->items;
+    for (int g = 0; g < kBigLaunches; ++g) {
+        info->workgroups += plan->nblocks_big[g];
+        info->work_items += plan->items_big[g];
+    }
     info->tiles = plan->tiles;
-    info->work_items = plan->items + plan->items_big[0] + plan->items_big[1] + plan->items_big[2];
     info->split_tiles = plan->split_tiles;
     info->est_imbalance = plan->est_imbalance;
     return HBS_SUCCESS;
 }
+
+template <int V>
+using IC = std::integral_constant<int, V>;
 
 extern "C" int hbs_grouped_run(const hbs_grouped_plan *plan, void *stream) {
     if (!plan) return HBS_ERR_NULL;
@@ -486,15 +497,30 @@ extern "C" int hbs_grouped_run(const hbs_grouped_plan *plan, void *stream) {
             kernel<<<grid, kBigThreads, 0, s>>>(pr, itb, plan->d_qoff_big[g], plan->nq_big[g],
                                                 plan->d_queue_big[g]);
         };
-        if (fp32) {
-            if (g == 0) launch(grouped_big_kernel<HBS_PREC_FP32, hbs::kTileClamp>);
-            else if (g == 1) launch(grouped_big_kernel<HBS_PREC_FP32, hbs::kTileClampK>);
-            else launch(grouped_big_kernel<HBS_PREC_FP32, hbs::kTileEdge>);
-        } else {
-            if (g == 0) launch(grouped_big_kernel<HBS_PREC_BF16X3, hbs::kTileClamp>);
-            else if (g == 1) launch(grouped_big_kernel<HBS_PREC_BF16X3, hbs::kTileClampK>);
-            else launch(grouped_big_kernel<HBS_PREC_BF16X3, hbs::kTileEdge>);
-        }
+        // (tile id, mode) of launch g: kTileCfg / big_launch_* in grouped_plan.h
+        auto by_mode = [&](auto prec, auto bm, auto bn, auto wm, auto wn) {
+            constexpr int P = decltype(prec)::value, BM = decltype(bm)::value,
+                          BN = decltype(bn)::value, WM = decltype(wm)::value,
+                          WN = decltype(wn)::value;
+            const int mode = big_launch_mode(g);
+            if (mode == kBigClamp) launch(grouped_big_kernel<P, hbs::kTileClamp, BM, BN, WM, WN>);
+            else if (mode == kBigClampK)
+                launch(grouped_big_kernel<P, hbs::kTileClampK, BM, BN, WM, WN>);
+            else if constexpr (BM == 256 && BN == 256)
+                launch(grouped_big_kernel<P, hbs::kTileEdge, BM, BN, WM, WN>);
+        };
+        auto by_tile = [&](auto prec) {
+            switch (big_launch_tile(g)) {
+                case 7: by_mode(prec, IC<128>{}, IC<128>{}, IC<4>{}, IC<2>{}); break;
+                case 8: by_mode(prec, IC<256>{}, IC<128>{}, IC<4>{}, IC<2>{}); break;
+                case 9: by_mode(prec, IC<128>{}, IC<256>{}, IC<2>{}, IC<4>{}); break;
+                default: by_mode(prec, IC<256>{}, IC<256>{}, IC<4>{}, IC<2>{}); break;
+            }
+        };
+        if (fp32)
+            by_tile(IC<HBS_PREC_FP32>{});
+        else
+            by_tile(IC<HBS_PREC_BF16X3>{});
         const int rc = hbs::launch_status();
         if (rc) return rc;
     }

@@ -33,15 +33,15 @@ bool load_tuning(const char *path, std::map<TuneKey, Choice> &table) {
         if (!(ss >> prec >> mc >> nc >> kc >> tile >> split)) continue;
         ss >> ns;
         const int pr = prec == "fp32" ? HBS_PREC_FP32 : prec == "bf16x3" ? HBS_PREC_BF16X3 : -1;
-        if (pr < 0 || tile < 0 || tile > kTileBig || split < 1 || split > 64) continue;
+        if (pr < 0 || !tile_supported(pr, tile) || split < 1 || split > 64) continue;
         table[TuneKey(pr, mc, nc, kc)] = Choice{tile, split, ns};
     }
     return true;
 }
 
 bool tile_supported(int precision, int tile) {
-    (void)precision;
-    return tile >= 0 && tile <= kTileBig;
+    if (tile < 0 || tile >= kTileIds) return false;
+    return (precision == HBS_PREC_FP32 ? kTileCfg[tile].bk_f32 : kTileCfg[tile].bk_b3) > 0;
 }
 
 // By default every problem takes the large-tile path, whatever its M, N and K:
@@ -96,41 +96,39 @@ int plan_host(const hbs_gemm_problem *problems, int count, int precision, const 
     };
     std::vector<Prob> probs(count);
     std::vector<Small> small;
-    // by launch: K a multiple of 32, other K, no 16-byte loads
     std::vector<Item> items_big[kBigLaunches];
     std::vector<double> cost_big[kBigLaunches];
-    auto lookup = [&](int M, int N, int K, bool &big) {
-        Choice c{0, 1, 0.0};  // the 256-thread tile, when one is chosen
-        big = true;           // the default (see above)
-        if (forced) {
-            big = forced->tile == kTileBig;
-            if (!big) c = *forced;
-        } else if (table) {
+    // the forced choice, else the tuning table's for the size class, else the
+    // large tile (see above)
+    auto lookup = [&](int M, int N, int K) {
+        if (forced) return *forced;
+        if (table) {
             auto f = table->find(TuneKey(precision, size_class(M), size_class(N), size_class(K)));
-            if (f != table->end() && tile_supported(precision, f->second.tile)) {
-                big = f->second.tile == kTileBig;
-                if (!big) c = f->second;
-            }
+            if (f != table->end() && tile_supported(precision, f->second.tile)) return f->second;
         }
-        return c;
+        return Choice{kTileBig, 1, 0.0};
     };
     for (int i = 0; i < count; ++i) {
         const hbs_gemm_problem &q = problems[i];
         probs[i] = Prob{q.A, q.B, q.C, q.M, q.N, q.K, q.lda, q.ldb, q.ldc};
-        bool big = false;
-        const Choice c = lookup(q.M, q.N, q.K, big);
+        Choice c = lookup(q.M, q.N, q.K);
         const bool vec = vec_loads(q);
-        if (!big && vec) {
+        // operands that allow no 16-byte loads: the whole problem on the
+        // 256 x 256 launch with per-element predicated loads, whatever the
+        // tuning says
+        if (!vec) c = Choice{kTileBig, 1, 0.0};
+        const TileCfg &cfg = kTileCfg[c.tile];
+        if (cfg.big < 0) {
             small.push_back(Small{i, c});
             continue;
         }
-        // the whole problem in 256 x 256 tiles; edge tiles clamp their loads,
-        // or predicate them per element when the operands allow no 16-byte
-        // loads (such problems take this launch whatever the tuning says)
-        const int g = !vec ? 2 : q.K % 32 == 0 ? 0 : 1;
-        for (int tm = 0; tm < (q.M + kBig - 1) / kBig; ++tm)
-            for (int tn = 0; tn < (q.N + kBig - 1) / kBig; ++tn) {
-                items_big[g].push_back(Item{i, tm, tn, 0, q.K, kTileBig, 1, 0, -1, 0});
+        // the whole problem in tiles of a large-tile launch; edge tiles clamp
+        // their loads (zero fill past K in the last stage when K is not a
+        // multiple of 32), or predicate them per element
+        const int g = !vec ? kEdgeLaunch : 2 * cfg.big + (q.K % 32 == 0 ? 0 : 1);
+        for (int tm = 0; tm < (q.M + cfg.bm - 1) / cfg.bm; ++tm)
+            for (int tn = 0; tn < (q.N + cfg.bn - 1) / cfg.bn; ++tn) {
+                items_big[g].push_back(Item{i, tm, tn, 0, q.K, c.tile, 1, 0, -1, 0});
                 cost_big[g].push_back((double)((q.K + 31) / 32 * 32));
             }
     }
@@ -138,13 +136,14 @@ int plan_host(const hbs_gemm_problem *problems, int count, int precision, const 
     std::vector<Item> items;
     std::vector<double> cost;
     int64_t slots = 0, poff = 0,
-            tiles_total = (int64_t)(items_big[0].size() + items_big[1].size() +
-                                    items_big[2].size());
+            tiles_total = 0;
+    for (int g = 0; g < kBigLaunches; ++g) tiles_total += (int64_t)items_big[g].size();
     for (const Small &sp : small) {
         const Prob &q = probs[sp.prob];
         const Choice &c = sp.c;
-        const int BM = kTileBM[c.tile], BN = kTileBN[c.tile];
-        const int BK = precision == HBS_PREC_FP32 ? 16 : 32;
+        const TileCfg &cfg = kTileCfg[c.tile];
+        const int BM = cfg.bm, BN = cfg.bn;
+        const int BK = precision == HBS_PREC_FP32 ? cfg.bk_f32 : cfg.bk_b3;
         const int tm_n = (q.M + BM - 1) / BM, tn_n = (q.N + BN - 1) / BN;
         // K range per split, a whole number of stages (int64: 2K may exceed int)
         const int chunk = (int)(hbs::ceil_div(hbs::ceil_div(q.K, c.split), BK) * BK);
@@ -155,7 +154,7 @@ int plan_host(const hbs_gemm_problem *problems, int count, int precision, const 
         // by 2 x CUs slots) with a tile-shape efficiency.
         const double slots_all = 2.0 * cus;
         const double peak = precision == HBS_PREC_FP32 ? 157.3e12 : 2516.6e12 / 3.0;
-        const double eff = (c.tile == 0) ? 0.7 : (c.tile == 1 ? 0.35 : 0.5);
+        const double eff = BM * BN >= 128 * 128 ? 0.7 : (BM * BN <= 64 * 64 ? 0.35 : 0.5);
         for (int tm = 0; tm < tm_n; ++tm) {
             for (int tn = 0; tn < tn_n; ++tn) {
                 ++tiles_total;
@@ -233,7 +232,7 @@ int plan_host(const hbs_gemm_problem *problems, int count, int precision, const 
         for (size_t i = 0; i < order.size(); ++i) order[i] = (int64_t)i;
         auto key = [&](int64_t x) {
             const Item &t = its[x];
-            const int bm = kTileBM[t.cfg], bn = kTileBN[t.cfg];
+            const int bm = kTileCfg[t.cfg].bm, bn = kTileCfg[t.cfg].bn;
             const int gm = 4, rows = (probs[t.prob].M + bm - 1) / bm;
             const int g = t.tm / gm, first = g * gm, h = std::min(gm, rows - first);
             const int64_t pos = ((int64_t)g * ((probs[t.prob].N + bn - 1) / bn) + t.tn) * h +
@@ -321,10 +320,13 @@ extern "C" int hbs_grouped_plan_describe(const hbs_gemm_problem *problems, int c
                              tuned ? &table : nullptr, cus, hp);
     if (st) return st;
     info->problems = count;
-    info->workgroups = hp.nblocks + hp.nblocks_big[0] + hp.nblocks_big[1] + hp.nblocks_big[2];
+    info->workgroups = hp.nblocks;
+    info->work_items = (int64_t)hp.sorted.size();
+    for (int g = 0; g < kBigLaunches; ++g) {
+        info->workgroups += hp.nblocks_big[g];
+        info->work_items += (int64_t)hp.sorted_big[g].size();
+    }
     info->tiles = hp.tiles_total;
-    info->work_items = (int64_t)(hp.sorted.size() + hp.sorted_big[0].size() +
-                                 hp.sorted_big[1].size() + hp.sorted_big[2].size());
     info->split_tiles = hp.slots;
     info->est_imbalance = hp.est_imbalance();
     return HBS_SUCCESS;

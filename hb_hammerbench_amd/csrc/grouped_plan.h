@@ -34,12 +34,42 @@ struct Item {
     int64_t poff;  // float offset of the tile's partial slots (split tiles only)
 };
 
-// Tile shapes: id -> (BM, BN).  Ids 0..3 run on the 256-thread launch as
-// 2 x 2 waves; id 4 (kTileBig) is the 512-thread large-tile launch.
-constexpr int kTileBM[5] = {128, 64, 128, 64, 256};
-constexpr int kTileBN[5] = {128, 64, 64, 128, 256};
-constexpr int kTileBig = 4, kBig = 256;
-constexpr int kBigLaunches = 3;  // large-tile launches: clamped, zero fill past K, predicated
+// Tile configurations by tile id (the HBS_TILE_* ids of include/hbsgemm.h).
+// `big` < 0: the 256-thread launch (4 waves, two workgroups per CU, split-K
+// allowed); else the large-tile slot of a 512-thread launch (8 waves, one
+// workgroup per CU, whole problems, tiles chained through one pipeline).
+// bk_*: K stage depth per precision, 0 where the configuration does not exist.
+struct TileCfg {
+    int bm, bn;          // tile rows x columns
+    int wm, wn;          // waves as wm x wn
+    int bk_f32, bk_b3;   // stage depth: FP32, BF16x3
+    int big;             // large-tile slot, or -1
+};
+constexpr int kTileIds = 10;
+constexpr TileCfg kTileCfg[kTileIds] = {
+    {128, 128, 2, 2, 16, 32, -1},  // 0: HBS_TILE_128x128
+    {64, 64, 2, 2, 16, 32, -1},    // 1: HBS_TILE_64x64
+    {128, 64, 2, 2, 16, 32, -1},   // 2: HBS_TILE_128x64
+    {64, 128, 2, 2, 16, 32, -1},   // 3: HBS_TILE_64x128
+    {256, 256, 4, 2, 32, 32, 0},   // 4: HBS_TILE_256x256 (the default)
+    {128, 128, 4, 1, 16, 0, -1},   // 5: HBS_TILE_128x128_4x1, waves of 32 x 128 (FP32 only)
+    {128, 128, 2, 2, 32, 0, -1},   // 6: HBS_TILE_128x128_BK32 (FP32 only)
+    {128, 128, 4, 2, 32, 32, 1},   // 7: HBS_TILE_128x128_8W, 8 waves of 32 x 64
+    {256, 128, 4, 2, 32, 32, 2},   // 8: HBS_TILE_256x128, 8 waves of 64 x 64
+    {128, 256, 2, 4, 32, 32, 3},   // 9: HBS_TILE_128x256, 8 waves of 64 x 64
+};
+constexpr int kTileBig = 4;  // the default tile
+constexpr int kBigCfgs = 4;  // large-tile slots: tile ids 4, 7, 8, 9
+constexpr int kBigCfgTile[kBigCfgs] = {4, 7, 8, 9};
+// Large-tile launches: 2 slot + 0 for problems whose K is a multiple of the
+// 32-deep stage (clamped loads), 2 slot + 1 for any other K (zero fill past
+// K), and kEdgeLaunch for problems whose operands allow no 16-byte loads (the
+// 256 x 256 tile with per-element predicated loads).
+constexpr int kBigLaunches = 2 * kBigCfgs + 1;
+constexpr int kEdgeLaunch = 2 * kBigCfgs;
+enum BigMode { kBigClamp = 0, kBigClampK = 1, kBigEdge = 2 };
+inline int big_launch_tile(int g) { return g == kEdgeLaunch ? kTileBig : kBigCfgTile[g / 2]; }
+inline int big_launch_mode(int g) { return g == kEdgeLaunch ? kBigEdge : g % 2; }
 constexpr int kXcdQueues = 8;  // one item queue per XCD on each large-tile launch
 
 struct Choice {
@@ -66,9 +96,9 @@ struct HostPlan {
     std::vector<Item> sorted;         // 256-thread list
     std::vector<Item> sorted_big[kBigLaunches];  // large-tile launches, in queue order
     std::vector<int> qoff_big[kBigLaunches];
-    int nq_big[kBigLaunches] = {1, 1, 1}, nblocks = 0, nblocks_big[kBigLaunches] = {0, 0, 0};
+    int nq_big[kBigLaunches] = {}, nblocks = 0, nblocks_big[kBigLaunches] = {};
     int64_t slots = 0, poff = 0, tiles_total = 0;
-    double imb = 1.0, imb_big[kBigLaunches] = {1.0, 1.0, 1.0};
+    double imb = 1.0, imb_big[kBigLaunches] = {};
     double est_imbalance() const {
         double e = nblocks > 0 ? imb : 1.0;
         for (int g = 0; g < kBigLaunches; ++g)

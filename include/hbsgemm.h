@@ -109,18 +109,30 @@ typedef struct {
 } hbs_gemm_problem;
 
 enum { HBS_PREC_FP32 = 0, HBS_PREC_BF16X3 = 1 };
-/* Tile shapes of the grouped kernel (BM x BN).  0..3 run on the 256-thread
- * persistent launch (with optional split-K).  HBS_TILE_256x256 runs the whole
- * problem on a 512-thread large-tile launch with the square kernels' tile (no
- * split-K; edge tiles clamp their loads; problems whose K is not a multiple of
- * 32 run on a second such launch that zero-fills past K, and problems whose
- * operands allow no 16-byte loads on a third with per-element predicates). */
+/* Tile configurations of the grouped kernel (BM x BN, waves, K stage).
+ *  - 256-thread persistent launch, 4 waves, two workgroups per CU, optional
+ *    split-K: 0..3 (2 x 2 waves; K stage 16 for FP32, 32 for BF16x3),
+ *    HBS_TILE_128x128_4x1 (FP32 only: waves of 32 x 128), HBS_TILE_128x128_BK32
+ *    (FP32 only: 2 x 2 waves, K stage 32);
+ *  - 512-thread large-tile launches, 8 waves, one workgroup per CU, whole
+ *    problems (no split-K), tiles chained through one pipeline, K stage 32:
+ *    HBS_TILE_256x256 (the default; edge tiles spread their valid part over
+ *    the SIMDs), HBS_TILE_128x128_8W, HBS_TILE_256x128, HBS_TILE_128x256.
+ * Each large-tile shape has one launch for problems whose K is a multiple of
+ * 32 (clamped loads) and one for other K (zero fill past K); problems whose
+ * operands allow no 16-byte loads run on a 256 x 256 launch with per-element
+ * predicated loads. */
 enum {
     HBS_TILE_128x128 = 0,
     HBS_TILE_64x64 = 1,
     HBS_TILE_128x64 = 2,
     HBS_TILE_64x128 = 3,
-    HBS_TILE_256x256 = 4
+    HBS_TILE_256x256 = 4,
+    HBS_TILE_128x128_4x1 = 5,
+    HBS_TILE_128x128_BK32 = 6,
+    HBS_TILE_128x128_8W = 7,
+    HBS_TILE_256x128 = 8,
+    HBS_TILE_128x256 = 9
 };
 
 typedef struct hbs_grouped_plan hbs_grouped_plan;
@@ -146,8 +158,9 @@ typedef struct {
  * buffers (synchronous). */
 int hbs_grouped_plan_create(const hbs_gemm_problem *problems, int count, int precision,
                             const char *tuning_file, hbs_grouped_plan **plan);
-/* Same with one tile shape and split-K factor (1..64) for every problem (the
- * autotuner's probe; split-K does not apply to HBS_TILE_256x256). */
+/* Same with one tile configuration and split-K factor (1..64) for every
+ * problem (the autotuner's probe; split-K applies to the 256-thread tiles
+ * only).  HBS_ERR_ARG for a configuration the precision does not have. */
 int hbs_grouped_plan_create_forced(const hbs_gemm_problem *problems, int count, int precision,
                                    int tile, int split_k, hbs_grouped_plan **plan);
 int hbs_grouped_plan_info(const hbs_grouped_plan *plan, hbs_grouped_info *info);

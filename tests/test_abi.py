@@ -128,7 +128,9 @@ def test_grouped_plan_argument_checks():
     bad = (hb.Problem * 1)(hb.Problem(None, 32, 48, 8, 8, 8, 8, 8, 8))
     assert L.hbs_grouped_plan_create(bad, 1, 0, None, ctypes.byref(h)) == -2
     assert L.hbs_grouped_plan_create(arr, 1, 0, b"/nonexistent/tuning.txt", ctypes.byref(h)) == -1
-    assert L.hbs_grouped_plan_create_forced(arr, 1, 0, 5, 1, ctypes.byref(h)) == -1  # tile ids 0..4
+    assert L.hbs_grouped_plan_create_forced(arr, 1, 0, 10, 1, ctypes.byref(h)) == -1  # ids 0..9
+    assert L.hbs_grouped_plan_create_forced(arr, 1, 1, 5, 1, ctypes.byref(h)) == -1  # FP32 only
+    assert L.hbs_grouped_plan_create_forced(arr, 1, 1, 6, 1, ctypes.byref(h)) == -1  # FP32 only
     assert L.hbs_grouped_plan_create_forced(arr, 1, 0, 0, 0, ctypes.byref(h)) == -1
     assert L.hbs_grouped_run(None, None) == -2
     assert L.hbs_grouped_plan_destroy(None) == 0

@@ -107,10 +107,23 @@ def test_grouped_bf16x3_identity_reproduces_oracle_split(tile, split):
         assert np.array_equal(c.cpu().numpy(), want), (tile, split)
 
 
+FP32_ONLY_TILES = (hb.TILE_128x128_4x1, hb.TILE_128x128_BK32)
+
+
 @pytest.mark.parametrize("prec", [hb.PREC_FP32, hb.PREC_BF16X3])
-@pytest.mark.parametrize("tile,split", [(0, 1), (1, 1), (2, 1), (3, 1), (0, 3), (1, 4), (2, 2)])
+@pytest.mark.parametrize("tile,split", [(0, 1), (1, 1), (2, 1), (3, 1), (0, 3), (1, 4), (2, 2),
+                                        (5, 1), (5, 3), (6, 1), (6, 2), (7, 1), (8, 1), (9, 1)])
 def test_grouped_forced_configs_within_bound(prec, tile, split):
+    """Every tile configuration (256-thread tiles with and without split-K,
+    the 4 x 1 wave and BK 32 variants, the 8-wave 128 x 128, 256 x 128 and
+    128 x 256 large tiles) on ragged problems: within the FP64 bound, and a
+    second run identical to the first."""
     probs, views, _ = make_group(RAGGED, "uniform", seed=7 + tile)
+    if prec == hb.PREC_BF16X3 and tile in FP32_ONLY_TILES:
+        with pytest.raises(hb.HbsError) as e:
+            hb.GroupedPlan(probs, prec, tile=tile, split_k=split)
+        assert e.value.code == -1
+        return
     plan = hb.GroupedPlan(probs, prec, tile=tile, split_k=split)
     info = plan.info()
     if split > 1:
@@ -563,3 +576,31 @@ def test_grouped_any_shape_and_alignment(prec):
         torch.cuda.synchronize()
         assert torch.equal(keep[2], eager)
     g.close()
+
+
+@pytest.mark.parametrize("prec", [hb.PREC_FP32, hb.PREC_BF16X3])
+@pytest.mark.parametrize("tile", [hb.TILE_128x128_8W, hb.TILE_256x128, hb.TILE_128x256])
+def test_large_tile_shapes_chained_match_alone(prec, tile):
+    """The 8-wave 128 x 128, 256 x 128 and 128 x 256 large tiles chain their
+    tiles through one pipeline like the 256 x 256 one: a group with K of 1, 3,
+    4 and 8 stages and K not a multiple of 32 (the zero-filling launch) gives,
+    bit for bit, what each problem gives alone; the pattern is exact."""
+    rng = np.random.default_rng(5 + tile)
+    ks = (32, 96, 128, 256, 100, 260)
+    shapes = [(int(rng.integers(1, 700)), 4 * int(rng.integers(1, 180)), ks[i % 6])
+              for i in range(60)]
+    probs, views, _ = make_group(shapes, "uniform", seed=13)
+    plan = hb.GroupedPlan(probs, prec, tile=tile)
+    plan.run()
+    torch.cuda.synchronize()
+    together = [C.clone() for (_, _, C) in views]
+    for pr, (_, _, C), t, shape in zip(probs, views, together, shapes):
+        C.fill_(float("nan"))
+        hb.GroupedPlan([pr], prec, tile=tile).run()
+        torch.cuda.synchronize()
+        assert torch.equal(C, t), shape
+    probs, views, _ = make_group(shapes[:20], "pattern")
+    hb.GroupedPlan(probs, prec, tile=tile).run()
+    torch.cuda.synchronize()
+    for (A, B, C), shape in zip(views, shapes[:20]):
+        assert torch.equal(C, (A.double() @ B.double()).float()), shape

@@ -22,20 +22,30 @@ import pytest
 
 import hb_hammerbench_amd as hb
 
-TILE = {0: (128, 128), 1: (64, 64), 2: (128, 64), 3: (64, 128)}
+# tile id -> (BM, BN, large-tile slot or None, K stage FP32, K stage BF16x3 or None)
+TILE = {0: (128, 128, None, 16, 32), 1: (64, 64, None, 16, 32), 2: (128, 64, None, 16, 32),
+        3: (64, 128, None, 16, 32), 4: (256, 256, 0, 32, 32), 5: (128, 128, None, 16, None),
+        6: (128, 128, None, 32, None), 7: (128, 128, 1, 32, 32), 8: (256, 128, 2, 32, 32),
+        9: (128, 256, 3, 32, 32)}
+EDGE_LAUNCH = 8  # the 256 x 256 launch with per-element predicated loads
+
+
+def tiles_for(prec):
+    return [t for t, cfg in TILE.items() if cfg[3 if prec == hb.PREC_FP32 else 4]]
 
 
 def expected(shapes, prec, cus=256, tile=None, split=1):
-    stage = 16 if prec == hb.PREC_FP32 else 32
-    big_tiles = [0, 0, 0]  # large-tile launches: K a multiple of 32, other K, predicated
+    big_tiles = [0] * 9  # large-tile launches: 2 slot + (K not a multiple of 32), predicated
     small_tiles = small_items = split_tiles = 0
     for M, N, K in shapes:
         vec = N % 4 == 0 and K % 4 == 0
-        if tile is None or tile == 4 or not vec:
-            g = 2 if not vec else 0 if K % 32 == 0 else 1
-            big_tiles[g] += math.ceil(M / 256) * math.ceil(N / 256)
+        t = 4 if tile is None or not vec else tile
+        BM, BN, slot, st32, st3 = TILE[t]
+        if slot is not None:
+            g = EDGE_LAUNCH if not vec else 2 * slot + (0 if K % 32 == 0 else 1)
+            big_tiles[g] += math.ceil(M / BM) * math.ceil(N / BN)
             continue
-        BM, BN = TILE[tile]
+        stage = st32 if prec == hb.PREC_FP32 else st3
         n = math.ceil(M / BM) * math.ceil(N / BN)
         chunk = math.ceil(math.ceil(K / split) / stage) * stage
         ns = math.ceil(K / chunk)
@@ -69,10 +79,42 @@ def test_describe_default_plan(prec, cus):
 
 
 @pytest.mark.parametrize("prec", [hb.PREC_FP32, hb.PREC_BF16X3])
-@pytest.mark.parametrize("tile,split", [(0, 1), (1, 3), (2, 2), (3, 5), (4, 1), (4, 2)])
+@pytest.mark.parametrize("tile,split", [(0, 1), (1, 3), (2, 2), (3, 5), (4, 1), (4, 2), (5, 1),
+                                        (5, 4), (6, 1), (6, 3), (7, 1), (8, 1), (9, 2)])
 def test_describe_forced_tiles(prec, tile, split):
+    if tile not in tiles_for(prec):
+        with pytest.raises(hb.HbsError) as e:
+            hb.describe_plan(MIXED, prec, tile=tile, split_k=split)
+        assert e.value.code == -1  # the 4 x 1 and BK 32 tiles are FP32 only
+        return
     info = hb.describe_plan(MIXED, prec, tile=tile, split_k=split)
     check(info, expected(MIXED, prec, tile=tile, split=split), len(MIXED))
+
+
+def test_describe_tuning_file_with_every_tile_id(tmp_path):
+    """A tuning file may name any tile configuration per size class; lines
+    whose tile the precision does not have are ignored."""
+    shapes = [(64, 64, 2048), (512, 512, 512), (300, 1000, 96), (2048, 128, 1024),
+              (100, 1800, 64), (130, 260, 1000)]
+    lines = ["fp32 64 64 2048 5 4 0", "fp32 512 512 512 6 2 0", "fp32 256 1024 128 7 1 0",
+             "fp32 2048 128 1024 8 1 0", "fp32 128 2048 64 9 1 0",
+             "bf16x3 64 64 2048 5 2 0", "bf16x3 512 512 512 8 1 0", "bf16x3 128 256 1024 7 1 0"]
+    path = tmp_path / "t.txt"
+    path.write_text("\n".join(lines) + "\n")
+    fp32 = hb.describe_plan(shapes, hb.PREC_FP32, tuning_file=str(path))
+    # tiles per shape: 128 x 128 (4 x 1) split 4; 128 x 128 (BK 32) split 2 on
+    # 4 x 4 tiles; 8 waves of 128 x 128 on 3 x 8; 256 x 128 on 8 x 1; 128 x 256
+    # on 1 x 8; the class of 130 x 260 x 1000 (128 256 1024) has no fp32 line:
+    # 256 x 256 on 1 x 2
+    assert fp32["tiles"] == 1 + 16 + 24 + 8 + 8 + 2
+    assert fp32["split_tiles"] == 1 + 16
+    assert fp32["work_items"] == 4 + 16 * 2 + 24 + 8 + 8 + 2
+    b3 = hb.describe_plan(shapes, hb.PREC_BF16X3, tuning_file=str(path))
+    # 64 x 64 x 2048 keeps the default (tile 5 is FP32 only); 512^3 on 256 x 128
+    # (2 x 4); 130 x 260 x 1000 on 8 waves of 128 x 128 (2 x 3); the others on
+    # 256 x 256 tiles
+    assert b3["split_tiles"] == 0
+    assert b3["tiles"] == 1 + 8 + 2 * 4 + 8 * 1 + 1 * 8 + 6
 
 
 def test_describe_tuning_file_overrides_the_default(tmp_path):
@@ -96,7 +138,8 @@ def test_describe_rejects_bad_arguments(tmp_path):
     assert code([]) == -1                                   # no problems
     assert code([(0, 64, 64)]) == -1
     assert code([(64, 64, 64)], precision=7) == -1
-    assert code([(64, 64, 64)], tile=5) == -1
+    assert code([(64, 64, 64)], tile=10) == -1
+    assert code([(64, 64, 64)], precision=hb.PREC_BF16X3, tile=6) == -1
     assert code([(64, 64, 64)], tile=0, split_k=0) == -1
     assert code([(64, 64, 64)], tile=0, tuning_file=str(tmp_path / "t.txt")) == -1
     assert code([(64, 64, 64)], tuning_file=str(tmp_path / "missing.txt")) == -1
