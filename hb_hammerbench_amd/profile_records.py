@@ -75,6 +75,36 @@ def read_kernel_stats(path):
     return rows
 
 
+def read_kernel_trace(path):
+    """{kernel: [duration ns of each dispatch, in dispatch order]} from a
+    rocprofv3 kernel_trace.csv (the --kernel-trace output)."""
+    out = {}
+    rows = []
+    with open(find_csv(path, "kernel_trace.csv"), newline="") as f:
+        for r in csv.DictReader(f):
+            rows.append((int(r["Start_Timestamp"]), short_name(r["Kernel_Name"]),
+                         int(r["End_Timestamp"]) - int(r["Start_Timestamp"])))
+    for _, name, dur in sorted(rows):
+        out.setdefault(name, []).append(dur)
+    return out
+
+
+def skip_warmup(stats, trace, warmup):
+    """Kernel statistics with each kernel's first `warmup` dispatches left out
+    (the untimed warm-up launches of a bench run), so that averages cover the
+    same launches as the bench line's HIP-event timing.  Kernels launched no
+    more than `warmup` times keep their figures."""
+    out = []
+    for r in stats:
+        d = trace.get(r["name"], [])
+        if warmup > 0 and len(d) > warmup:
+            t = d[warmup:]
+            r = dict(r, calls=len(t), total_ns=float(sum(t)), avg_ns=sum(t) / len(t),
+                     min_ns=float(min(t)), max_ns=float(max(t)), warmup_skipped=warmup)
+        out.append(r)
+    return out
+
+
 def read_counters(*paths):
     """{(kernel, counter): mean value per dispatch} over rocprofv3
     counter_collection.csv files (one per --pmc run)."""
@@ -177,7 +207,12 @@ def to_markdown(rec, title):
     if rec["commands"]:
         lines += ["Commands (`$REPO` = repository root, one MI355X):", "", "```"]
         lines += rec["commands"] + ["```", ""]
-    lines += ["## Kernel stats (`kernel_stats.csv`, one step = the launches below)", "",
+    skipped = max((r.get("warmup_skipped", 0) for r in rec["kernels"]), default=0)
+    head = "## Kernel stats (`kernel_stats.csv`, one step = the launches below)"
+    if skipped:
+        head = (f"## Kernel stats (`kernel_trace.csv` without each kernel's first {skipped} "
+                "dispatches, the bench's warm-up; one step = the launches below)")
+    lines += [head, "",
               "| kernel | calls | avg ns | min ns | max ns | % |", "|---|---|---|---|---|---|"]
     for r in rec["kernels"]:
         lines.append(f"| `{r['name']}` | {r['calls']} | {r['avg_ns']:,.0f} | {r['min_ns']:,.0f} | "
@@ -229,6 +264,9 @@ def main(argv=None):
     ap.add_argument("--copy-write", help="--pmc WRITE_SIZE output of bench.py --mode copy")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30)
     ap.add_argument("--dominant", help="substring of the dominant kernel's name")
+    ap.add_argument("--warmup", type=int, default=0,
+                    help="leave out each kernel's first N dispatches (the bench's warm-up "
+                         "launches), using the --stats run's kernel_trace.csv")
     ap.add_argument("--json", help="write the record here")
     ap.add_argument("--markdown", help="write the Markdown summary here")
     ap.add_argument("--title", default="profiling record")
@@ -241,8 +279,10 @@ def main(argv=None):
     if a.copy_fetch and a.copy_write:
         scales = copy_scales(read_counters(a.copy_fetch), read_counters(a.copy_write),
                              a.copy_bytes)
-    rec = make_record(bench, read_kernel_stats(a.stats), read_counters(*a.pmc), a.dominant,
-                      scales, a.command)
+    stats = read_kernel_stats(a.stats)
+    if a.warmup > 0:
+        stats = skip_warmup(stats, read_kernel_trace(a.stats), a.warmup)
+    rec = make_record(bench, stats, read_counters(*a.pmc), a.dominant, scales, a.command)
     if a.json:
         with open(a.json, "w") as f:
             json.dump(rec, f, indent=1)

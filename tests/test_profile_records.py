@@ -116,3 +116,34 @@ def test_cli_writes_json_and_markdown(run_dir):
     assert "**125.0 TFLOP/s** = 79.5% of 157.3" in md
     assert "MFMA busy: 90.0%" in md and "clock during the kernel: 2.40 GHz" in md
     assert "HBM traffic per step (all launches): 0.00 GB (FETCH x 2.000 + WRITE x 1.000" in md
+
+
+def test_warmup_dispatches_left_out_of_the_averages(tmp_path):
+    """kernel_trace.csv durations, in start order: the first `warmup`
+    dispatches of each kernel (the bench's untimed warm-up) are dropped from
+    calls, average, minimum and maximum; kernels launched fewer times keep
+    their kernel_stats.csv figures."""
+    d = str(tmp_path / "stats")
+    write_stats(d, [(KERNEL, 5, 7.0e6), (FILL, 2, 1.0e6)])
+    durs = [12.0e6, 9.0e6, 6.0e6, 5.0e6, 4.0e6]  # kernel dispatches in start order
+    with open(os.path.join(d, "run_kernel_trace.csv"), "w", newline="") as f:
+        w = csv.writer(f)
+        w.writerow(["Kind", "Kernel_Name", "Start_Timestamp", "End_Timestamp"])
+        t = 1000
+        rows = []
+        for dur in durs:
+            rows.append(["KERNEL_DISPATCH", KERNEL, t, t + int(dur)])
+            t += int(dur) + 100
+        rows.insert(1, ["KERNEL_DISPATCH", FILL, 5, 5 + 1000000])
+        rows.append(["KERNEL_DISPATCH", FILL, t, t + 1000000])
+        for r in reversed(rows):  # file order must not matter
+            w.writerow(r)
+    trace = pr.read_kernel_trace(d)
+    name = pr.short_name(KERNEL)
+    assert trace[name] == [int(x) for x in durs]
+    stats = pr.skip_warmup(pr.read_kernel_stats(d), trace, 2)
+    dom = [r for r in stats if r["name"] == name][0]
+    assert dom["calls"] == 3 and dom["avg_ns"] == pytest.approx(5.0e6)
+    assert dom["min_ns"] == 4.0e6 and dom["max_ns"] == 6.0e6
+    fill = [r for r in stats if r["name"] == pr.short_name(FILL)][0]
+    assert fill["calls"] == 2 and fill["avg_ns"] == 1.0e6
