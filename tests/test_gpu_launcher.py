@@ -87,3 +87,23 @@ def test_launcher_fails_on_a_planted_golden_mismatch(precision):
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     assert "elements differing: 1," in r.stdout
     assert r.stdout.strip().splitlines()[-1] == "HBS REGRESSION TEST PASSED"
+
+
+def test_launcher_host_code_under_asan_and_ubsan():
+    """The launcher with ASan + UBSan on its host code (`make -C
+    hb_hammerbench_amd/apps asan`): the reference configuration N_512__NITER_2
+    with the host golden and SSE rule (apps/sgemm/main.cpp:15-26,120-134) runs
+    clean, and the planted golden mismatch still fails with status 1 -- the
+    argument parsing, golden, readback and validation code raise no report."""
+    prog = os.path.join(APP, "sgemm_asan")
+    assert os.path.exists(prog), "sanitized launcher not built"
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:halt_on_error=1:exitcode=23",
+               UBSAN_OPTIONS="halt_on_error=1:print_stacktrace=1")
+    for extra, code in (([], 0), (["--perturb-golden", "1000:0.5"], 1),
+                        (["--precision", "bf16x3", "--mode", "grouped"], 0)):
+        r = subprocess.run([prog, "--N", "512", "--NITER", "2"] + extra, capture_output=True,
+                           text=True, timeout=240, env=env)
+        out = r.stdout + r.stderr
+        print(out[-400:])
+        assert "AddressSanitizer" not in out and "runtime error" not in out, out[-3000:]
+        assert r.returncode == code, out[-2000:]
