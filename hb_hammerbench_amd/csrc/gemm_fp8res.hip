@@ -3,7 +3,8 @@
 // No reference counterpart: a precision variant of apps/sgemm's C = A.B
 // (kernel.cpp:279-333 call shape, generalised to M x N x K).
 //
-// Quantisation (per 128 x 128 tile T of A, and of B, with amax = max |x| on T):
+// Quantisation (per 128 x 128 tile T of A, and of B, with amax = max |x| over
+// the finite entries of T):
 //   e    = the power-of-two exponent with 224 < amax / 2^e <= 448
 //   q(x) = 2^e * e4m3_rne(x / 2^e)                 (exact in bf16)
 //   r(x) = bf16_rne(x - q(x))                      (x - q(x) is exact)
@@ -86,8 +87,14 @@ __global__ __launch_bounds__(256) void quantize_kernel(QuantArgs a) {
 #pragma unroll
             for (int j = 0; j < 4; ++j) v[i][j] = (sr < a.rows && sc + j < a.cols) ? x[j] : 0.0f;
         }
+        // amax over the finite entries: a NaN or infinity leaves the tile's
+        // scale alone and makes only its own row (A) or column (B) of C
+        // non-finite
 #pragma unroll
-        for (int j = 0; j < 4; ++j) m = fmaxf(m, fabsf(v[i][j]));
+        for (int j = 0; j < 4; ++j) {
+            const float av = fabsf(v[i][j]);
+            m = fmaxf(m, av < INFINITY ? av : 0.0f);
+        }
     }
     for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
     if (lane == 0) red[wave] = m;

@@ -200,7 +200,8 @@ int hbs_graph_destroy(void *graph_exec);
 
 /* ------------------------------------------------------------------------ */
 /* FP8-e4m3 GEMM with per-tile scaling and a BF16 residual (no reference      */
-/* counterpart).  Per 128 x 128 tile of A and of B, with amax its max |x|:     */
+/* counterpart).  Per 128 x 128 tile of A and of B, with amax its max |x|     */
+/* over the tile's finite entries:                                            */
 /*   e = power-of-two exponent with 224 < amax / 2^e <= 448,                  */
 /*   q = 2^e * e4m3_rne(x / 2^e),  r = bf16_rne(x - q);                       */
 /*   C = Aq.Bq (FP8 matrix cores, E8M0 tile scales) + Aq.Br + Ar.Bq (BF16),  */
@@ -214,7 +215,9 @@ int hbs_graph_destroy(void *graph_exec);
  * holds column n of B).  q = decode(X8) * 2^e is rebuilt on the fly. */
 size_t hbs_gemm_fp8res_workspace_size(int M, int N, int K);
 /* C[M x N] = A[M x K] . B[K x N] (row-major, leading dimensions lda >= K,
- * ldb >= N, ldc >= N) by the scheme above. */
+ * ldb >= N, ldc >= N) by the scheme above.  A NaN or infinity in row i of A
+ * (column j of B) makes every output of row i (column j) of C non-finite and
+ * changes no other output: tile scales ignore non-finite entries. */
 int hbs_gemm_fp8res(const float *A, const float *B, float *C, int M, int N, int K, int lda,
                     int ldb, int ldc, void *workspace, size_t workspace_bytes, void *stream);
 

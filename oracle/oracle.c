@@ -333,7 +333,7 @@ void orc_fp8_quantize(int rows, int cols, const float *X, int ld, int tile,
             for (int i = r0; i < r1; i++)
                 for (int j = c0; j < c1; j++) {
                     float a = fabsf(X[(int64_t)i * ld + j]);
-                    if (a > amax) amax = a;
+                    if (isfinite(a) && a > amax) amax = a; /* finite entries only */
                 }
             int e = orc_fp8_tile_exponent(amax);
             exps[bi * tc + bj] = e;

@@ -211,3 +211,42 @@ def test_fp8res_full_size_outliers_error_report():
     assert e8["nonfinite"] == 0
     # BF16-level: within the unit roundoff of bf16 inputs (2^-8) normwise
     assert e8["e_F"] <= 2.0 ** -8 and e8["e_nw"] <= 2.0 ** -8
+
+
+def test_fp8res_non_finite_inputs_stay_in_their_row_and_column():
+    """A NaN or an infinity in row i of A (column j of B) makes every output
+    of row i (column j) of C non-finite and changes no other output: tile
+    scales come from the finite entries only.  Every other output is within
+    the bound of the algorithm applied to the inputs with the non-finite
+    entries set to zero (the quantisation of the finite entries is the same),
+    and both entry points agree bit for bit."""
+    M = N = K = 384
+    rng = np.random.default_rng(17)
+    A = rng.uniform(-1, 1, (M, K)).astype(np.float32)
+    B = rng.uniform(-1, 1, (K, N)).astype(np.float32)
+    A[5, 300] = np.nan; A[200, 17] = np.inf; A[130, 129] = -np.inf
+    B[40, 7] = np.nan; B[100, 250] = np.inf
+    bad_rows, bad_cols = [5, 130, 200], [7, 250]
+    a = torch.from_numpy(A.ravel()).cuda(); b = torch.from_numpy(B.ravel()).cuda()
+    c = torch.empty(M * N, dtype=torch.float32, device="cuda")
+    c2 = torch.empty_like(c)
+    hb.gemm_fp8res(a, b, c, M, N, K)
+    hb.gemm_fp8res_noalloc(a, b, c2, M, N, K)
+    torch.cuda.synchronize()
+    C = c.cpu().numpy().reshape(M, N)
+    assert np.array_equal(C, c2.cpu().numpy().reshape(M, N), equal_nan=True)
+    fin = np.ones((M, N), bool)
+    fin[bad_rows, :] = False
+    fin[:, bad_cols] = False
+    assert not np.isfinite(C[~fin]).any()
+    assert np.isfinite(C[fin]).all()
+    Az = np.where(np.isfinite(A), A, 0).astype(np.float32)
+    Bz = np.where(np.isfinite(B), B, 0).astype(np.float32)
+    Aq, Ar, _ = oracle.fp8_quantize(Az)
+    BqT, BrT, _ = oracle.fp8_quantize(np.ascontiguousarray(Bz.T))
+    Bq, Br = BqT.T.astype(np.float64), BrT.T.astype(np.float64)
+    Aq, Ar = Aq.astype(np.float64), Ar.astype(np.float64)
+    ideal = Aq @ Bq + Aq @ Br + Ar @ Bq
+    absum = np.abs(Aq) @ np.abs(Bq) + np.abs(Aq) @ np.abs(Br) + np.abs(Ar) @ np.abs(Bq)
+    bound = gamma(3 * K + 2) * absum + fp8_block_term(Aq, Bq)
+    assert np.all(np.abs(C - ideal)[fin] <= bound[fin])

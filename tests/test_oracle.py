@@ -312,3 +312,22 @@ def test_fp8_quantize_pattern_is_exact_and_residual_bound():
         for bj in range(3):
             t = X[bi * 128:(bi + 1) * 128, bj * 128:(bj + 1) * 128]
             assert 224.0 < np.abs(t).max() / 2.0 ** e[bi, bj] <= 448.0
+
+
+def test_fp8_tile_scale_ignores_non_finite_entries():
+    """A NaN or an infinity in a 128 x 128 tile leaves its exponent at what
+    the tile's finite entries give, and every finite entry's q and r at what
+    they are with the non-finite entries replaced by zero."""
+    rng = np.random.default_rng(8)
+    X = rng.normal(0, 3, (256, 256)).astype(np.float32)
+    Y = X.copy()
+    Y[5, 30] = np.inf; Y[140, 200] = -np.inf; Y[70, 150] = np.nan
+    Z = Y.copy()
+    Z[~np.isfinite(Z)] = 0.0
+    qy, ry, ey = oracle.fp8_quantize(Y)
+    qz, rz, ez = oracle.fp8_quantize(Z)
+    assert np.array_equal(ey, ez)
+    fin = np.isfinite(Y)
+    assert np.array_equal(qy[fin], qz[fin]) and np.array_equal(ry[fin], rz[fin])
+    # an infinity alone in its tile would otherwise push the scale to 2^120
+    assert ez.max() < 10
