@@ -211,6 +211,26 @@ def test_fp8res_full_size_outliers_error_report():
     assert e8["nonfinite"] == 0
     # BF16-level: within the unit roundoff of bf16 inputs (2^-8) normwise
     assert e8["e_F"] <= 2.0 ** -8 and e8["e_nw"] <= 2.0 ** -8
+    # componentwise at full size: rows 8192..8207 x columns 0..2047 of the
+    # same launch within the algorithm's bound against the FP64 product, from
+    # the oracle's quantisation of their whole 128 x 128 scaling tiles
+    r0, nr, nc = 8192, 16, 2048
+    At = a.view(M, K)[r0:r0 + 128].cpu().numpy()
+    Bt = b.view(K, N)[:, :nc].cpu().numpy()
+    Aq, Ar, _ = oracle.fp8_quantize(At)
+    BqT, BrT, _ = oracle.fp8_quantize(np.ascontiguousarray(Bt.T))
+    Aq, Ar = Aq[:nr].astype(np.float64), Ar[:nr].astype(np.float64)
+    Bq, Br = BqT.T.astype(np.float64), BrT.T.astype(np.float64)
+    A64, B64 = At[:nr].astype(np.float64), Bt.astype(np.float64)
+    absum = np.abs(Aq) @ np.abs(Bq) + np.abs(Aq) @ np.abs(Br) + np.abs(Ar) @ np.abs(Bq)
+    dA, dB = A64 - Aq, B64 - Bq
+    algo = (2.0 ** -8 * (np.abs(Aq) @ np.abs(dB)) + 2.0 ** -8 * (np.abs(dA) @ np.abs(Bq))
+            + np.abs(dA) @ np.abs(dB))
+    bound = algo + gamma(3 * K + 2) * absum + fp8_block_term(Aq, Bq)
+    Cs = c8.view(M, N)[r0:r0 + nr, :nc].cpu().numpy()
+    R = A64 @ B64
+    assert np.all(np.abs(Cs - R) <= bound)
+    print(f"componentwise: max |C - R| / bound = {(np.abs(Cs - R) / bound).max():.3f}", flush=True)
 
 
 def test_fp8res_non_finite_inputs_stay_in_their_row_and_column():
