@@ -285,11 +285,20 @@ __global__ __launch_bounds__(kBigThreads) void grouped_big_kernel(const Prob *__
     __shared__ int s_next[2];
     // queue q = blockIdx.x % nq (the workgroup's XCD when nq = 8) hands out
     // items[qoff[q] .. qoff[q + 1]) in order; queue[q] is its head and
-    // queue[kXcdQueues] counts the workgroups that have left
-    const int q = blockIdx.x % nq, first = qoff[q], end = qoff[q + 1];
+    // queue[kXcdQueues] counts the workgroups that have left.  A workgroup
+    // whose own queue is empty takes the next item of the other queues in
+    // turn (q + 1, q + 2, ...), so no XCD idles while another still has tiles
+    // (per-workgroup end stamps showed XCDs finishing up to 0.6 ms apart
+    // without it); `total` means no item is left anywhere.
+    const int q = blockIdx.x % nq, total = qoff[nq];
     auto fetch = [&]() {
-        return first +
-               __hip_atomic_fetch_add(&queue[q], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int d = 0; d < nq; ++d) {
+            const int qq = q + d < nq ? q + d : q + d - nq;
+            const int i = qoff[qq] + __hip_atomic_fetch_add(&queue[qq], 1, __ATOMIC_RELAXED,
+                                                             __HIP_MEMORY_SCOPE_AGENT);
+            if (i < qoff[qq + 1]) return i;
+        }
+        return total;
     };
     auto tile = [&](int i, hbs::TileArgs &t, float *&Cg, int &ldc) {
         const Item it = items[i];
@@ -302,7 +311,7 @@ __global__ __launch_bounds__(kBigThreads) void grouped_big_kernel(const Prob *__
     if (threadIdx.x == 0) s_next[0] = fetch();
     __syncthreads();
     const int i0 = __builtin_amdgcn_readfirstlane(s_next[0]);
-    if (i0 < end) {
+    if (i0 < total) {
         // Tiles run back to back through one pipeline: while a tile of three
         // or more K stages finishes, the next one's first two stages are
         // loaded (hbs::TileChain).  Thread 0 fetches the next item at the
@@ -317,7 +326,7 @@ __global__ __launch_bounds__(kBigThreads) void grouped_big_kernel(const Prob *__
         };
         auto peek = [&](hbs::TileArgs &nt) {
             const int j = __builtin_amdgcn_readfirstlane(s_next[pw]);
-            if (j >= end) return false;
+            if (j >= total) return false;
             tile(j, nt, nCg, nldc);
             return true;
         };
