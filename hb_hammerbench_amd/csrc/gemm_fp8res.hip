@@ -364,6 +364,195 @@ __global__ __launch_bounds__(kThreads) void gemm_fp8res_kernel(GemmArgs p) {
     }
 }
 
+// Wide GEMM: 256 x 256 output tile, 8 waves as 4 x 2 of 64 x 128, BK = 64,
+// for padded widths Np that are multiples of 256.  A third fewer bytes are
+// loaded per flop than with the 256 x 128 tile.  Its stages do not fit twice
+// in the 160 KB LDS (96 KB each), so the e4m3 planes have one LDS buffer and
+// the residual planes two:
+//   e4m3 buffer   A8 [256][64 B], B8 [256][64 B]          32 KB
+//   residual x 2  Ar [256][128 B], Br [256][128 B]     2 x 64 KB
+// The operands go from the workspace straight into LDS
+// (global_load_lds_dwordx4: lane l of a load writes bytes 16 l .. 16 l + 15
+// of a 1 KB run), so no registers hold a stage in flight.  Each load picks
+// its source chunk so that the runs land in the swizzled layout above (sw8,
+// swr).  Step kt:
+//   1. loads the residual planes of stage kt + 1 into the other residual
+//      buffer and the exponents of stage kt + 1 into a register;
+//   2. reads its e4m3 fragments from the e4m3 buffer, runs residual step 0;
+//   3. barrier (every wave holds its e4m3 fragments), then loads the e4m3
+//      planes of stage kt + 1 into the e4m3 buffer;
+//   4. runs residual steps 1-3 and the FP8 products;
+//   5. waits for its loads, barrier.
+// The MFMAs of each 32 x 32 block run in the same order as in the 256 x 128
+// kernel, so both give the same bits.
+constexpr int kWBN = 256, kWFN = 4;                  // tile width, column blocks per wave
+constexpr int kWA8 = 0, kWB8 = kWA8 + kBM * kRow8;   // e4m3 buffer
+constexpr int kWRes = kWB8 + kWBN * kRow8;           // first residual buffer
+constexpr int kWAR = 0, kWBR = kWAR + kBM * kRowR;   // within a residual buffer
+constexpr int kWRStage = kWBR + kWBN * kRowR;
+constexpr int kWLds = kWRes + 2 * kWRStage;
+static_assert(kWLds <= 160 * 1024, "LDS");
+
+// one 16-byte lane of a global -> LDS run (the LDS address is the run's start)
+__device__ __forceinline__ void dma16(const void *g, void *lds) {
+    __builtin_amdgcn_global_load_lds((__attribute__((address_space(1))) void *)g,
+                                     (__attribute__((address_space(3))) void *)lds, 16, 0, 0);
+}
+
+template <bool EDGE>
+__global__ __launch_bounds__(kThreads) void gemm_fp8res_wide_kernel(GemmArgs p) {
+    __shared__ __attribute__((aligned(16))) uint8_t smem[kWLds];
+    const int bid = hbs::xcd_remap(blockIdx.x, gridDim.x);
+    int tm, tn;
+    hbs::tile_coords(bid, p.tiles_m, p.tiles_n, 8, tm, tn);
+    const int m0 = tm * kBM, n0 = tn * kWBN;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = wave % 4, wn = wave / 4;
+    const int l32 = lane & 31, h = lane >> 5;
+    const int ktiles = p.Kp / kQT;
+    const int KT = p.Kp / kBK;
+
+    floatx16 acc[kFM][kWFN];
+#pragma unroll
+    for (int r = 0; r < kFM; ++r)
+#pragma unroll
+        for (int c = 0; c < kWFN; ++c) acc[r][c] = floatx16{0};
+
+    // Sources of this lane's loads.  e4m3 planes: load i (0, 1) of wave w
+    // fills rows 16 (2w + i) .. +15 (4 chunks a row); row & ~3 is a multiple of
+    // 4, so the swizzle of lane l's chunk depends on l alone.  Residual planes:
+    // load i (0..3) fills rows 8 (4w + i) .. +7 (8 chunks a row).
+    const uint8_t *a8g = p.A8 + (int64_t)m0 * p.Kp;
+    const uint8_t *b8g = p.B8 + (int64_t)n0 * p.Kp;
+    const uint16_t *arg = p.Ar + (int64_t)m0 * p.Kp;
+    const uint16_t *brg = p.Br + (int64_t)n0 * p.Kp;
+    const int r8 = 32 * wave + (lane >> 2);                    // row of e4m3 load 0
+    const int c8 = ((lane & 3) ^ ((lane >> 4) & 3)) * 16;      // its byte column
+    const int rr = 32 * wave + (lane >> 3);                    // row of residual load 0
+    auto dma_q = [&](int kt) {  // e4m3 planes of stage kt -> e4m3 buffer
+        const int k0 = kt * kBK;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int64_t o = (int64_t)(r8 + 16 * i) * p.Kp + k0 + c8;
+            dma16(a8g + o, smem + kWA8 + (2 * wave + i) * 1024);
+            dma16(b8g + o, smem + kWB8 + (2 * wave + i) * 1024);
+        }
+    };
+    auto dma_r = [&](int kt, int buf) {  // residual planes of stage kt -> buffer buf
+        const int k0 = kt * kBK;
+        uint8_t *rs = smem + kWRes + buf * kWRStage;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int cr = ((lane & 7) ^ ((4 * (i & 1) + (lane >> 4)) & 7)) * 8;
+            const int64_t o = (int64_t)(rr + 8 * i) * p.Kp + k0 + cr;
+            dma16(arg + o, rs + kWAR + (4 * wave + i) * 1024);
+            dma16(brg + o, rs + kWBR + (4 * wave + i) * 1024);
+        }
+    };
+    // lanes 0, 1: exponents of the two 128-row A tiles, lanes 2, 3: of the two
+    // 128-column B tiles; the other lanes repeat lane 0's load
+    const int *exp_src = lane == 1 ? p.eA + (int64_t)(m0 / kQT + 1) * ktiles
+                         : lane == 2 ? p.eB + (int64_t)(n0 / kQT) * ktiles
+                         : lane == 3 ? p.eB + (int64_t)(n0 / kQT + 1) * ktiles
+                                     : p.eA + (int64_t)(m0 / kQT) * ktiles;
+    auto eload = [&](int kt) { return hbs::ldg(exp_src + (kt * kBK) / kQT); };
+
+    const int arow = wm * 64 + l32, brow = wn * 128 + l32;
+    auto compute = [&](int buf, int ex, auto &&mid) {
+        const uint8_t *rs = smem + kWRes + buf * kWRStage;
+        const int ea = __builtin_amdgcn_readlane(ex, wm >> 1);
+        const int eb = __builtin_amdgcn_readlane(ex, 2 + wn);
+        intx8 a8[kFM], b8[kWFN];
+#pragma unroll
+        for (int r = 0; r < kFM; ++r) {
+            const int row = arow + r * 32;
+            const floatx4 lo = *(const floatx4 *)(smem + kWA8 + sw8(row, 2 * h));
+            const floatx4 hi = *(const floatx4 *)(smem + kWA8 + sw8(row, 2 * h + 1));
+            a8[r] = intx8{__float_as_int(lo[0]), __float_as_int(lo[1]), __float_as_int(lo[2]),
+                          __float_as_int(lo[3]), __float_as_int(hi[0]), __float_as_int(hi[1]),
+                          __float_as_int(hi[2]), __float_as_int(hi[3])};
+        }
+#pragma unroll
+        for (int c = 0; c < kWFN; ++c) {
+            const int row = brow + c * 32;
+            const floatx4 lo = *(const floatx4 *)(smem + kWB8 + sw8(row, 2 * h));
+            const floatx4 hi = *(const floatx4 *)(smem + kWB8 + sw8(row, 2 * h + 1));
+            b8[c] = intx8{__float_as_int(lo[0]), __float_as_int(lo[1]), __float_as_int(lo[2]),
+                          __float_as_int(lo[3]), __float_as_int(hi[0]), __float_as_int(hi[1]),
+                          __float_as_int(hi[2]), __float_as_int(hi[3])};
+        }
+        const float sa = ldexpf(1.0f, ea), sb = ldexpf(1.0f, eb);
+#pragma unroll
+        for (int s = 0; s < kBK / 16; ++s) {
+            if (s == 1) mid();
+            bf16x8 aq[kFM], ar[kFM];
+#pragma unroll
+            for (int r = 0; r < kFM; ++r) {
+                ar[r] = *(const bf16x8 *)(rs + kWAR + swr(arow + r * 32, 4 * h + s));
+                aq[r] = fp8x8_to_bf16(a8[r][2 * s], a8[r][2 * s + 1], sa);
+            }
+#pragma unroll
+            for (int c = 0; c < kWFN; ++c) {
+                const bf16x8 br = *(const bf16x8 *)(rs + kWBR + swr(brow + c * 32, 4 * h + s));
+                const bf16x8 bq = fp8x8_to_bf16(b8[c][2 * s], b8[c][2 * s + 1], sb);
+#pragma unroll
+                for (int r = 0; r < kFM; ++r)
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ar[r], bq, acc[r][c], 0, 0, 0);
+#pragma unroll
+                for (int r = 0; r < kFM; ++r)
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aq[r], br, acc[r][c], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < kWFN; ++c)
+#pragma unroll
+            for (int r = 0; r < kFM; ++r)
+                acc[r][c] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
+                    a8[r], b8[c], acc[r][c], 0, 0, 0, ea + 127, 0, eb + 127);
+    };
+
+    dma_q(0);
+    dma_r(0, 0);
+    int ex = eload(0);
+    __builtin_amdgcn_s_waitcnt(0xF70);  // vmcnt(0)
+    __syncthreads();
+    for (int kt = 0; kt < KT; ++kt) {
+        const bool more = kt + 1 < KT;
+        int exn = ex;
+        if (more) {
+            dma_r(kt + 1, (kt + 1) & 1);
+            exn = eload(kt + 1);
+        }
+        compute(kt & 1, ex, [&]() {
+            // every wave holds its e4m3 fragments of stage kt.  A bare
+            // barrier after this wave's LDS reads: __syncthreads() would also
+            // wait for the residual loads in flight (they write LDS).
+            __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            __builtin_amdgcn_s_barrier();
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            if (more) dma_q(kt + 1);
+        });
+        __builtin_amdgcn_s_waitcnt(0xF70);  // vmcnt(0): stage kt + 1 is in LDS
+        __syncthreads();
+        ex = exn;
+    }
+
+#pragma unroll
+    for (int r = 0; r < kFM; ++r) {
+#pragma unroll
+        for (int c = 0; c < kWFN; ++c) {
+            const int col = n0 + wn * 128 + c * 32 + l32;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int row = m0 + wm * 64 + r * 32 + 8 * (q >> 2) + 4 * h + (q & 3);
+                if (!EDGE || (row < p.M && col < p.N)) p.C[(int64_t)row * p.ldc + col] = acc[r][c][q];
+            }
+        }
+    }
+}
+
 inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
 struct Layout {
@@ -423,10 +612,18 @@ int run_fp8res(const float *A, const float *B, float *C, int M, int N, int K, in
                (const uint16_t *)(ws + L.ar), (const uint16_t *)(ws + L.br),
                (const int *)(ws + L.ea), (const int *)(ws + L.eb), C, M, N, L.Kp, ldc,
                L.Mp / kBM, L.Np / kBN};
-    if (M % kBM == 0 && N % kBN == 0)
+    if (L.Np % kWBN == 0) {
+        g.tiles_n = L.Np / kWBN;
+        const unsigned wblocks = (unsigned)(g.tiles_m * g.tiles_n);
+        if (M % kBM == 0 && N % kWBN == 0)
+            gemm_fp8res_wide_kernel<false><<<wblocks, kThreads, 0, s>>>(g);
+        else
+            gemm_fp8res_wide_kernel<true><<<wblocks, kThreads, 0, s>>>(g);
+    } else if (M % kBM == 0 && N % kBN == 0) {
         gemm_fp8res_kernel<false><<<(unsigned)blocks, kThreads, 0, s>>>(g);
-    else
+    } else {
         gemm_fp8res_kernel<true><<<(unsigned)blocks, kThreads, 0, s>>>(g);
+    }
     return hbs::launch_status();
 }
 

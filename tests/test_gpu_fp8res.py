@@ -119,7 +119,12 @@ def test_fp8res_quantiser_matches_oracle(M, N, K):
     assert not A8[M:].any() and not A8[:, K:].any()
 
 
-@pytest.mark.parametrize("M,N,K", [(256, 128, 256), (300, 200, 100), (512, 384, 1024)])
+# Padded widths Np (N rounded up to 128) that are multiples of 256 run the
+# 256 x 256 kernel, the others the 256 x 128 one: (256, 512, 384) and
+# (384, 768, 512) take the wide kernel, (300, 200, 100) and (300, 700, 200)
+# its predicated-store form.
+@pytest.mark.parametrize("M,N,K", [(256, 128, 256), (300, 200, 100), (512, 384, 1024),
+                                   (256, 512, 384), (384, 768, 512), (300, 700, 200)])
 def test_fp8res_matches_algorithm_within_fp32_bound(M, N, K):
     A, B, C, _ = run(M, N, K, seed=M)
     Aq, Ar, _ = oracle.fp8_quantize(A)
@@ -139,6 +144,27 @@ def test_fp8res_matches_algorithm_within_fp32_bound(M, N, K):
     algo = (2.0 ** -8 * (np.abs(Aq) @ np.abs(dB)) + 2.0 ** -8 * (np.abs(dA) @ np.abs(Bq))
             + np.abs(dA) @ np.abs(dB))
     assert np.all(np.abs(C - R) <= algo + gamma(3 * K + 2) * absum + hw)
+
+
+@pytest.mark.parametrize("M,K", [(512, 640), (300, 200)])
+def test_fp8res_wide_tile_equals_narrow_tile(M, K):
+    """N = 512 runs the 256 x 256 kernel and N = 384 the 256 x 128 one.  B's
+    first 384 columns are whole 128-column scaling tiles, so both quantise
+    them alike, and each output element sees the same MFMAs in the same order:
+    the shared columns of C agree bit for bit."""
+    a = torch.empty(M * K, dtype=torch.float32, device="cuda")
+    b = torch.empty(K * 512, dtype=torch.float32, device="cuda")
+    hb.fill_random(a, M * K, hb.GEN_NORMAL_OUTLIER, 11)
+    hb.fill_random(b, K * 512, hb.GEN_NORMAL_OUTLIER, 12)
+    b384 = b.view(K, 512)[:, :384].contiguous().view(-1)
+    c512 = torch.empty(M * 512, dtype=torch.float32, device="cuda")
+    c384 = torch.empty(M * 384, dtype=torch.float32, device="cuda")
+    hb.gemm_fp8res(a, b, c512, M, 512, K)
+    hb.gemm_fp8res(a, b384, c384, M, 384, K)
+    torch.cuda.synchronize()
+    wide = c512.view(M, 512)[:, :384].cpu().numpy()
+    narrow = c384.view(M, 384).cpu().numpy()
+    assert np.array_equal(wide.view(np.uint32), narrow.view(np.uint32))
 
 
 def test_fp8res_identity_b_reproduces_quantised_a():
