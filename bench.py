@@ -10,9 +10,10 @@ Prints one JSON line: whole-job TFLOP/s, the roofline of the dominant kernel
 (timed with HIP events on the stream it is launched on), the FP64-verified
 error of the result, the metric's comparison (the BF16x3 kernel on the same
 inputs, and both kernels on one log-uniform batch with their errors against
-FP64), and the reference's CPU golden (host_mm restated in the oracle, single
-thread) timed on a bounded row sample on this host.  --variant selects the
-other workloads.
+FP64), the same products through torch (hipBLASLt) on this GPU for
+comparison, and the reference's CPU golden (host_mm restated in the oracle,
+single thread) timed on a bounded row sample on this host.  --variant selects
+the other workloads.
 """
 import argparse
 import json
@@ -49,6 +50,8 @@ def parse():
                    help="rows of the row-parallel CPU golden sample")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--no-verify", action="store_true", help="skip the FP64 error check")
+    p.add_argument("--no-vendor", action="store_true",
+                   help="skip the same-box vendor-library comparison (torch / hipBLASLt)")
     p.add_argument("--no-compare", action="store_true",
                    help="fp32: skip the BF16x3 comparison on the same workload")
     p.add_argument("--mode", choices=["gemm", "copy"], default="gemm",
@@ -229,6 +232,21 @@ def setup_square(args, hb, torch):
     extra = {"verify": verify,
              "config": {"N": N, "niter": NITER, "bytes_algorithmic": 12.0 * N * N * NITER},
              "traffic": measured_traffic(args.variant, f"N={N} NITER={NITER}")}
+    def vendor(ours):
+        A3 = a.view(NITER, N, N)
+        B3 = b.view(NITER, N, N)
+        if args.variant == "fp32":
+            C3 = torch.empty_like(A3)
+            return vendor_leg(torch, 2.0 * N ** 3 * NITER, [(lambda: torch.bmm(A3, B3, out=C3), 1)],
+                              "torch.bmm in float32 over the same NITER slices", ours)
+        Ah = A3.bfloat16()
+        Bh = B3.bfloat16()
+        Ch = torch.empty_like(Ah)
+        return vendor_leg(torch, 2.0 * N ** 3 * NITER, [(lambda: torch.bmm(Ah, Bh, out=Ch), 3)],
+                          "the split's three BF16 products at the vendor's rate: torch.bmm in "
+                          "bfloat16 over the same slices, its time x3 (no split counted)", ours)
+
+    extra["vendor"] = vendor
     if args.variant == "fp32" and not args.no_compare:
         extra["compare"] = lambda fp32_tflops: bf16x3_comparison(args, hb, torch, a, b, c,
                                                                  fp32_tflops)
@@ -252,6 +270,21 @@ def time_steps(torch, steps, warmup, fn):
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) / steps
     return wall, sum(s.elapsed_time(e) for s, e in ev) / len(ev)
+
+
+def vendor_leg(torch, flops, products, what, ours):
+    """The same products through torch (hipBLASLt on ROCm) on this GPU:
+    `products` is a list of (fn, times) whose summed time, each call's mean
+    time `times` times over, is the vendor's time per step of `flops`
+    FP32-equivalent flops.  Reported beside the bench line, not used by it."""
+    total = 0.0
+    for fn, times in products:
+        wall, _ = time_steps(torch, 5, 2, fn)
+        total += wall * times
+    value = flops / total / 1e12
+    return {"library": f"torch {torch.__version__} (hipBLASLt)", "what": what,
+            "value": round(value, 3), "unit": "TFLOP/s",
+            "ratio_ours_over_vendor": round(ours / value, 3)}
 
 
 def last_slice_errors(hb, a, b, c, N, NITER):
@@ -358,8 +391,31 @@ def setup_grouped(args, hb, torch):
     cfg = {"problems": args.problems, "plan": info,
            "bytes_algorithmic": float(sum(4 * (M * K + K * N + M * N) for M, N, K in shapes)),
            "tuning": os.path.relpath(tuning, ROOT) if tuning else None}
+    def vendor(ours):
+        # own outputs: the FP64 check that follows reads this kernel's C
+        cv = torch.empty(tc, dtype=torch.float32, device="cuda")
+        mats, o = [], 0
+        for A, B, C, M, N, K in views:
+            mats.append((A.view(M, K), B.view(K, N), cv[o:o + M * N].view(M, N)))
+            o += M * N
+        if prec == hb.PREC_FP32:
+            def run_all():
+                for A2, B2, C2 in mats:
+                    torch.matmul(A2, B2, out=C2)
+            return vendor_leg(torch, flops, [(run_all, 1)],
+                              "torch.matmul in float32, one call per problem", ours)
+        half = [(A2.bfloat16(), B2.bfloat16(), torch.empty(C2.shape, dtype=torch.bfloat16,
+                                                         device="cuda")) for A2, B2, C2 in mats]
+
+        def run_half():
+            for A2, B2, C2 in half:
+                torch.matmul(A2, B2, out=C2)
+        return vendor_leg(torch, flops, [(run_half, 3)],
+                          "the split's three BF16 products at the vendor's rate: torch.matmul "
+                          "in bfloat16, one call per problem, its time x3 (no split counted)", ours)
+
     return step, flops, workload, {
-        "verify": verify, "config": cfg, "keep": (plan, graph),
+        "verify": verify, "config": cfg, "keep": (plan, graph), "vendor": vendor,
         "traffic": measured_traffic(args.variant, f"{args.problems} problems seed {args.seed}")}
 
 
@@ -386,7 +442,25 @@ def setup_fp8res(args, hb, torch):
 
     workload = (f"FP8-e4m3 per-128x128-tile scales + BF16 residual GEMM, M=N=16384, K=4096, "
                 f"normal(0,1) with 1% x1e3 outliers seed {args.seed}")
+    def vendor(ours):
+        A2 = a.view(M, K)
+        B2 = b.view(K, N)
+        # scaled by 1/16 into e4m3's range (the 1e3 outliers reach about 5e3)
+        a8 = (A2 * 0.0625).to(torch.float8_e4m3fn)
+        b8 = (B2.t().contiguous() * 0.0625).to(torch.float8_e4m3fn).t()  # column-major K x N
+        one = torch.tensor(1.0, device="cuda")
+        ah, bh = A2.bfloat16(), B2.bfloat16()
+        ch = torch.empty(M, N, dtype=torch.bfloat16, device="cuda")
+        return vendor_leg(
+            torch, 2.0 * M * N * K,
+            [(lambda: torch._scaled_mm(a8, b8, scale_a=one, scale_b=one, out_dtype=torch.float32), 1),
+             (lambda: torch.matmul(ah, bh, out=ch), 2)],
+            "the three products at the vendor's rates: torch._scaled_mm in float8_e4m3fn "
+            "(per-tensor scale, FP32 out) plus torch.matmul in bfloat16 twice (no quantiser "
+            "counted)", ours)
+
     return step, 2.0 * M * N * K, workload, {
+        "vendor": vendor,
         "verify": verify,
         "config": {"M": M, "N": N, "K": K, "bytes_algorithmic": 4.0 * (M * K + K * N + M * N)},
         "traffic": measured_traffic("fp8res", f"M={M} N={N} K={K}")}
@@ -472,6 +546,8 @@ def main():
             "kernel_ms_avg": round(avg_ms, 4),
         },
     }
+    if not args.no_vendor and "vendor" in extra:
+        line["vendor_same_box"] = extra["vendor"](value)
     if not args.no_verify:
         line["accuracy"] = extra["verify"]()
     if "compare" in extra:
