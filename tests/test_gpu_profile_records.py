@@ -20,7 +20,8 @@ pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 N, NITER = 4096, 2
 BENCH = [sys.executable, os.path.join(ROOT, "bench.py"), "--n", str(N), "--niter", str(NITER),
-         "--steps", "3", "--warmup", "1", "--no-cpu", "--no-compare", "--no-verify"]
+         "--steps", "3", "--warmup", "1", "--no-cpu", "--no-compare", "--no-verify",
+         "--no-vendor"]
 
 
 def rocprof():
