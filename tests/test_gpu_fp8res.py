@@ -122,9 +122,11 @@ def test_fp8res_quantiser_matches_oracle(M, N, K):
 # Padded widths Np (N rounded up to 128) that are multiples of 256 run the
 # 256 x 256 kernel, the others the 256 x 128 one: (256, 512, 384) and
 # (384, 768, 512) take the wide kernel, (300, 200, 100) and (300, 700, 200)
-# its predicated-store form.
+# its predicated-store form; (256, 512, 100) and (512, 256, 128) give it the
+# shortest pipeline (Kp = 128: two 64-deep stages).
 @pytest.mark.parametrize("M,N,K", [(256, 128, 256), (300, 200, 100), (512, 384, 1024),
-                                   (256, 512, 384), (384, 768, 512), (300, 700, 200)])
+                                   (256, 512, 384), (384, 768, 512), (300, 700, 200),
+                                   (256, 512, 100), (512, 256, 128)])
 def test_fp8res_matches_algorithm_within_fp32_bound(M, N, K):
     A, B, C, _ = run(M, N, K, seed=M)
     Aq, Ar, _ = oracle.fp8_quantize(A)
