@@ -547,7 +547,13 @@ def main():
         },
     }
     if not args.no_vendor and "vendor" in extra:
-        line["vendor_same_box"] = extra["vendor"](value)
+        # context only: a torch build without one of these calls loses the
+        # comparison, not the line
+        try:
+            line["vendor_same_box"] = extra["vendor"](value)
+        except Exception as e:  # noqa: BLE001
+            line["vendor_same_box"] = {"error": f"{type(e).__name__}: {e}"[:300]}
+        torch.cuda.synchronize()
     if not args.no_verify:
         line["accuracy"] = extra["verify"]()
     if "compare" in extra:
