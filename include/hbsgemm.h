@@ -22,8 +22,11 @@
  *    is passed in as a workspace whose size the *_workspace_size call returns
  *    (grouped plans allocate once, when they are created, and
  *    hbs_gemm_fp8res_noalloc uses a fixed scratch array in the library);
- *  - calls on distinct streams are independent; calls sharing a workspace
- *    must be ordered by the caller.
+ *  - calls on distinct streams are independent, and the entry points may be
+ *    called from several host threads at once (the library's per-device
+ *    state is locked; grouped graphs are captured in thread-local mode);
+ *    calls sharing a workspace, a grouped plan or the allocation-free
+ *    scratch must be ordered by the caller.
  */
 #ifndef HBSGEMM_H
 #define HBSGEMM_H

@@ -1,0 +1,115 @@
+"""Host threads calling the C-ABI at once, each on its own stream.
+
+Four threads each run, three times over, the FP32 and BF16x3 SGEMMs, the FP8 +
+residual GEMM (with its own workspace) and a grouped BF16x3 plan (created,
+captured into a graph, launched and destroyed in the thread).  ctypes drops
+the GIL during the calls, so the library's entry points, its per-device
+caches and the grouped graph capture run concurrently.  Every kernel is
+deterministic, so each thread's outputs must equal, bit for bit, the same
+calls made one at a time on the default stream.  (The allocation-free FP8
+entry point shares one scratch per GPU and is documented as not safe for
+overlapping calls; it is left out.)
+"""
+import threading
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+import hb_hammerbench_amd as hb  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+THREADS, ROUNDS = 4, 3
+N, NITER = 512, 2
+FM, FN, FK = 384, 512, 320  # FP8 shape: the 256 x 256 kernel with ragged rows
+SHAPES = [(256, 320, 192), (512, 512, 256), (100, 260, 68), (64, 576, 512)]
+
+
+def make_inputs(t):
+    a = torch.empty(N * N * NITER, device="cuda")
+    b = torch.empty_like(a)
+    hb.fill_random(a, a.numel(), hb.GEN_UNIFORM, 100 + t)
+    hb.fill_random(b, b.numel(), hb.GEN_UNIFORM, 200 + t)
+    return a, b
+
+
+def make_outputs():
+    out = {
+        "f32": torch.empty(N * N * NITER, device="cuda"),
+        "b3": torch.empty(N * N * NITER, device="cuda"),
+        "fp8": torch.empty(FM * FN, device="cuda"),
+        "grouped": torch.empty(sum(M * Nn for M, Nn, _ in SHAPES), device="cuda"),
+    }
+    ws = torch.empty(hb.fp8res_workspace_size(FM, FN, FK), dtype=torch.uint8, device="cuda")
+    return out, ws
+
+
+def run_all(a, b, out, ws, stream):
+    """Every entry point once on `stream`; returns the objects that must
+    outlive the launches."""
+    hb.sgemm(a, b, out["f32"], N, NITER, stream=stream)
+    hb.sgemm_bf16x3(a, b, out["b3"], N, NITER, stream=stream)
+    hb.gemm_fp8res(a[:FM * FK], b[:FK * FN], out["fp8"], FM, FN, FK, workspace=ws,
+                   stream=stream)
+    probs, oa, ob, oc = [], 0, 0, 0
+    for M, Nn, K in SHAPES:
+        A, B, C = a[oa:oa + M * K], b[ob:ob + K * Nn], out["grouped"][oc:oc + M * Nn]
+        # every view whole (a short slice would let a kernel read past it)
+        assert (A.numel(), B.numel(), C.numel()) == (M * K, K * Nn, M * Nn)
+        probs.append((A, B, C, M, Nn, K))
+        oa += M * K
+        ob += K * Nn
+        oc += M * Nn
+    plan = hb.GroupedPlan(probs, hb.PREC_BF16X3)
+    graph = plan.graph()
+    graph.launch(stream=stream)
+    return plan, graph
+
+
+def bits(t):
+    return t.cpu().numpy().view(np.uint32)
+
+
+def test_threads_on_own_streams_match_serial_calls():
+    inputs = [make_inputs(t) for t in range(THREADS)]
+    outs = [make_outputs() for _ in range(THREADS)]
+    torch.cuda.synchronize()
+
+    # one at a time on the default stream
+    expect = []
+    for t in range(THREADS):
+        keep = run_all(*inputs[t], *outs[t], torch.cuda.current_stream())
+        torch.cuda.synchronize()
+        expect.append({k: bits(v) for k, v in outs[t][0].items()})
+        keep[1].close()
+        keep[0].close()
+
+    streams = [torch.cuda.Stream() for _ in range(THREADS)]
+    for r in range(ROUNDS):
+        for o, _ in outs:
+            for v in o.values():
+                v.fill_(float("nan"))
+        torch.cuda.synchronize()
+        errors, keeps = [], [None] * THREADS
+        start = threading.Barrier(THREADS)
+
+        def work(t):
+            try:
+                start.wait()
+                keeps[t] = run_all(*inputs[t], *outs[t], streams[t])
+            except Exception as e:  # reported below, in the main thread
+                errors.append((t, repr(e)))
+
+        threads = [threading.Thread(target=work, args=(t,)) for t in range(THREADS)]
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join()
+        torch.cuda.synchronize()
+        assert not errors, errors
+        for t in range(THREADS):
+            for k, v in outs[t][0].items():
+                assert np.array_equal(bits(v), expect[t][k]), (r, t, k)
+            keeps[t][1].close()
+            keeps[t][0].close()
