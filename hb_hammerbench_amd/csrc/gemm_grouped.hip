@@ -360,6 +360,14 @@ __global__ __launch_bounds__(kBigThreads) void grouped_big_kernel(const Prob *__
     }
 }
 
+// Held while a graph is captured and while a plan's device memory is
+// allocated, filled or freed.  A device-wide synchronisation (which hipFree
+// and hipMalloc may do) in one host thread while another thread's stream is
+// capturing fails ("operation not permitted when stream is capturing") and
+// invalidates that capture; a concurrency test met exactly this with plans
+// created in some threads while others captured their graphs.
+std::mutex capture_mu;
+
 int build_plan(const hbs_gemm_problem *problems, int count, int precision, const Choice *forced,
                const std::map<TuneKey, Choice> *table, hbs_grouped_plan **out) {
     if (!out) return HBS_ERR_NULL;
@@ -397,6 +405,7 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
         o_qoff_big[g] = total;
         total += align(hp.qoff_big[g].size() * sizeof(int));
     }
+    std::lock_guard<std::mutex> cap(capture_mu);
     char *dmem = nullptr;
     HBS_HIP_TRY(hipMalloc(&dmem, total));
     hbs_grouped_plan *plan = new hbs_grouped_plan();
@@ -422,22 +431,32 @@ int build_plan(const hbs_gemm_problem *problems, int count, int precision, const
         plan->d_qoff_big[g] = (const int *)(dmem + o_qoff_big[g]);
         plan->nq_big[g] = hp.nq_big[g];
     }
-    hipError_t e = hipMemcpy(dmem, probs.data(), probs.size() * sizeof(Prob), hipMemcpyHostToDevice);
+    // upload on a private stream and wait for that stream only (no device-
+    // wide synchronisation; this stream is never captured)
+    hipStream_t up = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&up, hipStreamNonBlocking);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dmem, probs.data(), probs.size() * sizeof(Prob),
+                           hipMemcpyHostToDevice, up);
     if (e == hipSuccess && !sorted.empty())
-        e = hipMemcpy(dmem + o_items, sorted.data(), sorted.size() * sizeof(Item),
-                      hipMemcpyHostToDevice);
+        e = hipMemcpyAsync(dmem + o_items, sorted.data(), sorted.size() * sizeof(Item),
+                           hipMemcpyHostToDevice, up);
     for (int g = 0; g < kBigLaunches; ++g) {
         const std::vector<Item> &sb = hp.sorted_big[g];
         const std::vector<int> &qo = hp.qoff_big[g];
         if (e == hipSuccess && !sb.empty())
-            e = hipMemcpy(dmem + o_items_big[g], sb.data(), sb.size() * sizeof(Item),
-                          hipMemcpyHostToDevice);
+            e = hipMemcpyAsync(dmem + o_items_big[g], sb.data(), sb.size() * sizeof(Item),
+                               hipMemcpyHostToDevice, up);
         if (e == hipSuccess && !qo.empty())
-            e = hipMemcpy(dmem + o_qoff_big[g], qo.data(), qo.size() * sizeof(int),
-                          hipMemcpyHostToDevice);
+            e = hipMemcpyAsync(dmem + o_qoff_big[g], qo.data(), qo.size() * sizeof(int),
+                               hipMemcpyHostToDevice, up);
     }
-    if (e == hipSuccess) e = hipMemset(plan->d_counters, 0, b_cnt);
-    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemsetAsync(plan->d_counters, 0, b_cnt, up);
+    if (up) {
+        const hipError_t es = hipStreamSynchronize(up);
+        if (e == hipSuccess) e = es;
+        (void)hipStreamDestroy(up);
+    }
     if (e != hipSuccess) {
         (void)hipFree(dmem);
         delete plan;
@@ -557,6 +576,7 @@ extern "C" int hbs_grouped_plan_destroy(hbs_grouped_plan *plan) {
         std::lock_guard<std::mutex> lk(graph_mu);
         if (plan->graphs > 0) return HBS_ERR_IN_USE;
     }
+    std::lock_guard<std::mutex> cap(capture_mu);
     hipError_t e = hipFree(plan->dmem);
     delete plan;
     return e == hipSuccess ? HBS_SUCCESS : (int)e;
@@ -566,6 +586,7 @@ extern "C" int hbs_grouped_plan_destroy(hbs_grouped_plan *plan) {
 extern "C" int hbs_grouped_graph_create(const hbs_grouped_plan *plan, void **graph_exec) {
     if (!plan || !graph_exec) return HBS_ERR_NULL;
     *graph_exec = nullptr;
+    std::lock_guard<std::mutex> cap(capture_mu);
     hipStream_t s;
     HBS_HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     hipGraph_t g = nullptr;

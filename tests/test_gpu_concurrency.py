@@ -20,7 +20,7 @@ import hb_hammerbench_amd as hb  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
-THREADS, ROUNDS = 4, 3
+THREADS, ROUNDS = 4, 8
 N, NITER = 512, 2
 FM, FN, FK = 384, 512, 320  # FP8 shape: the 256 x 256 kernel with ragged rows
 SHAPES = [(256, 320, 192), (512, 512, 256), (100, 260, 68), (64, 576, 512)]
