@@ -350,18 +350,10 @@ __global__ __launch_bounds__(kThreads) void gemm_fp8res_kernel(GemmArgs p) {
         gload(s0, kt + 4);
     }
 
-#pragma unroll
-    for (int r = 0; r < kFM; ++r) {
-#pragma unroll
-        for (int c = 0; c < kFN; ++c) {
-            const int col = n0 + wn * 64 + c * 32 + l32;
-#pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                const int row = m0 + wm * 64 + r * 32 + 8 * (q >> 2) + 4 * h + (q & 3);
-                if (!EDGE || (row < p.M && col < p.N)) p.C[(int64_t)row * p.ldc + col] = acc[r][c][q];
-            }
-        }
-    }
+    if (!EDGE || (m0 + kBM <= p.M && n0 + kBN <= p.N))
+        hbs::store_acc<kFM, kFN, false>(p.C, p.ldc, m0 + wm * 64, n0 + wn * 64, p.M, p.N, kFN, acc);
+    else
+        hbs::store_acc<kFM, kFN, true>(p.C, p.ldc, m0 + wm * 64, n0 + wn * 64, p.M, p.N, kFN, acc);
 }
 
 // Wide GEMM: 256 x 256 output tile, 8 waves as 4 x 2 of 64 x 128, BK = 64,
@@ -539,18 +531,12 @@ __global__ __launch_bounds__(kThreads) void gemm_fp8res_wide_kernel(GemmArgs p) 
         ex = exn;
     }
 
-#pragma unroll
-    for (int r = 0; r < kFM; ++r) {
-#pragma unroll
-        for (int c = 0; c < kWFN; ++c) {
-            const int col = n0 + wn * 128 + c * 32 + l32;
-#pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                const int row = m0 + wm * 64 + r * 32 + 8 * (q >> 2) + 4 * h + (q & 3);
-                if (!EDGE || (row < p.M && col < p.N)) p.C[(int64_t)row * p.ldc + col] = acc[r][c][q];
-            }
-        }
-    }
+    if (!EDGE || (m0 + kBM <= p.M && n0 + kWBN <= p.N))
+        hbs::store_acc<kFM, kWFN, false>(p.C, p.ldc, m0 + wm * 64, n0 + wn * 128, p.M, p.N, kWFN,
+                                         acc);
+    else
+        hbs::store_acc<kFM, kWFN, true>(p.C, p.ldc, m0 + wm * 64, n0 + wn * 128, p.M, p.N, kWFN,
+                                        acc);
 }
 
 inline int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }

@@ -135,23 +135,11 @@ __device__ __forceinline__ void finish_tile(floatx16 (&acc)[BM / WM / 32][BN / W
             __hip_atomic_store(&counters[it.slot], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return;
     }
-    // one base pointer per 32x32 block; the per-register row offsets are
-    // wave-uniform multiples of ldc
-#pragma unroll
-    for (int r = 0; r < FM; ++r) {
-        const int row0 = m0 + wm * (BM / WM) + r * 32 + 4 * h;
-#pragma unroll
-        for (int c = 0; c < FN; ++c) {
-            const int col = n0 + wn * (BN / WN) + c * 32 + l32;
-            float *cp = p.C + (int64_t)row0 * p.ldc + col;
-            const bool colok = col < p.N;
-#pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                const int dr = 8 * (q >> 2) + (q & 3);
-                if (colok && row0 + dr < p.M) hbs::stg(cp + (int64_t)dr * p.ldc, acc[r][c][q]);
-            }
-        }
-    }
+    const int rb = m0 + wm * (BM / WM), cb = n0 + wn * (BN / WN);
+    if (m0 + BM <= p.M && n0 + BN <= p.N)
+        hbs::store_acc<FM, FN, false>(p.C, p.ldc, rb, cb, p.M, p.N, FN, acc);
+    else
+        hbs::store_acc<FM, FN, true>(p.C, p.ldc, rb, cb, p.M, p.N, FN, acc);
 }
 
 // ---------------------------------------------------------------------------

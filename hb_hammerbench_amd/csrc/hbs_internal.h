@@ -50,6 +50,39 @@ __device__ __forceinline__ void stg(T *p, T v) {
     *(__attribute__((address_space(1))) T *)p = v;
 }
 
+// Stores a wave's FM x FN accumulator blocks of 32 x 32 (v_mfma_f32_32x32x*
+// results: in block (r, c) lane l holds column 32 c + (l & 31) and, in
+// register q, row 32 r + 8 (q >> 2) + 4 (l >> 5) + (q & 3)) to C at the
+// wave's sub-tile origin (rb, cb), row stride ldc.  The lane's own address is
+// formed once; each store adds a wave-uniform offset to it, so a store costs
+// one 64-bit add and no 64-bit multiply.  PRED: only rows < M and columns < N
+// are written.  Column blocks from ncols on are skipped (a half-width wave).
+// (A 16-byte-store form, each group of four registers transposed across its
+// lane quad by DPP, was bit-identical but 2.5% slower on the grouped BF16x3
+// group: its 16 extra VALU instructions per four values cost more than the
+// three quarters of the store instructions they save.)
+template <int FM, int FN, bool PRED>
+__device__ __forceinline__ void store_acc(float *__restrict__ C, int ldc, int rb, int cb, int M,
+                                          int N, int ncols, const floatx16 (&acc)[FM][FN]) {
+    const int lane = threadIdx.x & 63;
+    const int l32 = lane & 31, h = lane >> 5;
+    float *lanep = C + ((int64_t)rb + 4 * h) * ldc + cb + l32;
+    const int vm = M - rb - 4 * h, vn = N - cb - l32;  // valid rows / columns from the lane's own
+#pragma unroll
+    for (int r = 0; r < FM; ++r) {
+#pragma unroll
+        for (int c = 0; c < FN; ++c) {
+            if (c >= ncols) continue;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int dr = r * 32 + 8 * (q >> 2) + (q & 3);
+                if (!PRED || (dr < vm && c * 32 < vn))
+                    stg(lanep + ((int64_t)dr * ldc + c * 32), acc[r][c][q]);
+            }
+        }
+    }
+}
+
 // Operands of one block tile's main loop: Ag = A + m0 * lda (the tile's first
 // row), Bg = B + n0 (its first column); M, N, K bound the predicates.
 struct TileArgs {

@@ -371,33 +371,20 @@ __device__ __forceinline__ void bf16x3_block_mma(uint16_t *smem, const float *__
 }
 
 // Epilogue: stores acc to C (Cg = the matrix) at the tile (m0, n0), each wave
-// its sub-tile (bf16x3_wave_tile); outside kTileInterior the stores are
-// predicated on M x N.
+// its sub-tile (bf16x3_wave_tile); outside kTileInterior the stores of a tile
+// that crosses M or N are predicated.
 template <class G, int MODE>
 __device__ __forceinline__ void bf16x3_block_store(float *__restrict__ Cg, int M, int N, int ldc,
                                                    int m0, int n0,
                                                    const floatx16 (&acc)[G::kFM][G::kFN]) {
-    constexpr int kFM = G::kFM, kFN = G::kFN;
-    const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const WaveTile wt = bf16x3_wave_tile<G, MODE>(M, N, m0, n0, wave);
     if (wt.kind == kWaveNone) return;
-    const int ncols = wt.kind == kWaveHalf ? kFN / 2 : kFN;
-    const int l32 = lane & 31, h = lane >> 5;
-#pragma unroll
-    for (int r = 0; r < kFM; ++r) {
-#pragma unroll
-        for (int c = 0; c < kFN; ++c) {
-            if (c >= ncols) continue;
-            const int col = n0 + wt.on + c * 32 + l32;
-#pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                const int row = m0 + wt.om + r * 32 + 8 * (q >> 2) + 4 * h + (q & 3);
-                if (MODE == kTileInterior || (row < M && col < N))
-                    stg(Cg + (int64_t)row * ldc + col, acc[r][c][q]);
-            }
-        }
-    }
+    const int ncols = wt.kind == kWaveHalf ? G::kFN / 2 : G::kFN;
+    if (MODE == kTileInterior || (m0 + G::BM <= M && n0 + G::BN <= N))
+        store_acc<G::kFM, G::kFN, false>(Cg, ldc, m0 + wt.om, n0 + wt.on, M, N, ncols, acc);
+    else
+        store_acc<G::kFM, G::kFN, true>(Cg, ldc, m0 + wt.om, n0 + wt.on, M, N, ncols, acc);
 }
 
 // One whole output tile: main loop and store.

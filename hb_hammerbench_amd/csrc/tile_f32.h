@@ -367,35 +367,22 @@ __device__ __forceinline__ void f32_block_mma(
 }
 
 // Epilogue: stores acc to C (Cg = the matrix) at the tile (m0, n0), each wave
-// its sub-tile (f32_wave_tile).  32x32
-// accumulator layout: register q of lane l holds row 8*(q/4) + 4*(l/32) + q%4,
-// col l%32 of its block.  Outside kTileInterior the stores are predicated on
-// M x N.
+// its sub-tile (f32_wave_tile).  32x32 accumulator layout: register q of lane
+// l holds row 8*(q/4) + 4*(l/32) + q%4, col l%32 of its block.  Outside
+// kTileInterior the stores of a tile that crosses M or N are predicated.
 template <int BM, int BN, int BK, int WM, int WN, int MODE>
 __device__ __forceinline__ void f32_block_store(
     float *__restrict__ Cg, int M, int N, int ldc, int m0, int n0,
     const floatx16 (&acc)[F32Cfg<BM, BN, BK, WM, WN>::kFM][F32Cfg<BM, BN, BK, WM, WN>::kFN]) {
     using G = F32Cfg<BM, BN, BK, WM, WN>;
-    const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const WaveTile wt = f32_wave_tile<BM, BN, BK, WM, WN, MODE>(M, N, m0, n0, wave);
     if (wt.kind == kWaveNone) return;
     const int ncols = wt.kind == kWaveHalf ? G::kFN / 2 : G::kFN;
-    const int l32 = lane & 31, h = lane >> 5;
-#pragma unroll
-    for (int r = 0; r < G::kFM; ++r) {
-#pragma unroll
-        for (int c = 0; c < G::kFN; ++c) {
-            if (c >= ncols) continue;
-            const int col = n0 + wt.on + c * 32 + l32;
-#pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                const int row = m0 + wt.om + r * 32 + 8 * (q >> 2) + 4 * h + (q & 3);
-                if (MODE == kTileInterior || (row < M && col < N))
-                    stg(Cg + (int64_t)row * ldc + col, acc[r][c][q]);
-            }
-        }
-    }
+    if (MODE == kTileInterior || (m0 + BM <= M && n0 + BN <= N))
+        store_acc<G::kFM, G::kFN, false>(Cg, ldc, m0 + wt.om, n0 + wt.on, M, N, ncols, acc);
+    else
+        store_acc<G::kFM, G::kFN, true>(Cg, ldc, m0 + wt.om, n0 + wt.on, M, N, ncols, acc);
 }
 
 // One whole output tile: main loop and store.
