@@ -255,6 +255,43 @@ def to_markdown(rec, title):
     return "\n".join(lines) + "\n"
 
 
+def update_traffic(path, key, workload, rec, command=""):
+    """Writes the calibrated HBM bytes of record `rec` into the traffic table
+    at `path` (profiles/traffic.json, which bench.py reads for its roofline
+    line) under `key` (the bench variant), for the workload string bench.py
+    matches (e.g. "N=8192 NITER=10"), together with the copy calibration the
+    bytes were scaled by.  Other entries are kept."""
+    t = rec.get("traffic")
+    if not t:
+        raise ValueError("the record has no traffic (FETCH_SIZE / WRITE_SIZE runs and the copy "
+                         "calibration are needed)")
+    try:
+        with open(path) as f:
+            table = json.load(f)
+    except (OSError, ValueError):
+        table = {}
+    table["calibration"] = {
+        "kernel": COPY_KERNEL + " (1 GiB device-to-device copy)",
+        "fetch_kib": (1 << 30) / 1024.0 / t["scale_fetch"],
+        "write_kib": (1 << 30) / 1024.0 / t["scale_write"],
+        "bytes_each_way": 1 << 30,
+        "scale_fetch": round(t["scale_fetch"], 4),
+        "scale_write": round(t["scale_write"], 4),
+    }
+    table[key] = {
+        "workload": workload,
+        "kernels": rec["step_kernels"],
+        "fetch_kib_raw": t["fetch_kib"],
+        "write_kib_raw": t["write_kib"],
+        "hbm_bytes_per_launch": int(round(t["hbm_bytes"])),
+        "command": command,
+    }
+    with open(path, "w") as f:
+        json.dump(table, f, indent=1)
+        f.write("\n")
+    return table[key]
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--bench", required=True, help="file holding the bench.py JSON line")
@@ -272,6 +309,9 @@ def main(argv=None):
     ap.add_argument("--title", default="profiling record")
     ap.add_argument("--command", action="append", default=[],
                     help="command that produced an input (repeatable)")
+    ap.add_argument("--traffic-json", help="traffic table to update (profiles/traffic.json)")
+    ap.add_argument("--traffic-key", help="its entry: the bench variant")
+    ap.add_argument("--traffic-workload", help="the workload string bench.py matches")
     a = ap.parse_args(argv)
     with open(a.bench) as f:
         bench = json.loads([ln for ln in f if ln.strip().startswith("{")][-1])
@@ -289,7 +329,13 @@ def main(argv=None):
     if a.markdown:
         with open(a.markdown, "w") as f:
             f.write(to_markdown(rec, a.title))
-    if not a.json and not a.markdown:
+    if a.traffic_json:
+        if not (a.traffic_key and a.traffic_workload):
+            ap.error("--traffic-json needs --traffic-key and --traffic-workload")
+        cmd = next((c for c in a.command if "FETCH_SIZE" in c), "")
+        update_traffic(a.traffic_json, a.traffic_key, a.traffic_workload, rec,
+                       cmd + "   (and --pmc WRITE_SIZE)" if cmd else "")
+    if not a.json and not a.markdown and not a.traffic_json:
         json.dump(rec, sys.stdout, indent=1)
         print()
     return 0

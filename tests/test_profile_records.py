@@ -147,3 +147,30 @@ def test_warmup_dispatches_left_out_of_the_averages(tmp_path):
     assert dom["min_ns"] == 4.0e6 and dom["max_ns"] == 6.0e6
     fill = [r for r in stats if r["name"] == pr.short_name(FILL)][0]
     assert fill["calls"] == 2 and fill["avg_ns"] == 1.0e6
+
+
+def test_traffic_table_update_keeps_other_entries(run_dir, tmp_path):
+    """--traffic-json writes the calibrated bytes of the record under the
+    bench variant and the workload string bench.py looks up, with the copy
+    calibration, and keeps the other entries."""
+    t, _ = run_dir
+    table = str(tmp_path / "traffic.json")
+    with open(table, "w") as f:
+        json.dump({"fp8res": {"workload": "M=1 N=1 K=1", "hbm_bytes_per_launch": 7}}, f)
+    rc = pr.main(["--bench", os.path.join(t, "bench.json"), "--stats", os.path.join(t, "stats"),
+                  "--pmc", os.path.join(t, "fetch"), os.path.join(t, "write"),
+                  "--copy-fetch", os.path.join(t, "cfetch"), "--copy-write", os.path.join(t, "cwrite"),
+                  "--traffic-json", table, "--traffic-key", "fp32",
+                  "--traffic-workload", "N=8192 NITER=10",
+                  "--command", "rocprofv3 --pmc FETCH_SIZE ...", "--command", "rocprofv3 --pmc WRITE_SIZE ..."])
+    assert rc == 0
+    got = json.load(open(table))
+    assert got["fp8res"]["hbm_bytes_per_launch"] == 7
+    e = got["fp32"]
+    assert e["workload"] == "N=8192 NITER=10" and e["kernels"] == ["sgemm_f32_kernel<256, 256, 32, 4, 2, 0>"]
+    assert e["hbm_bytes_per_launch"] == (1000 * 2 + 500) * 1024
+    assert e["command"].startswith("rocprofv3 --pmc FETCH_SIZE")
+    assert got["calibration"]["scale_fetch"] == 2.0 and got["calibration"]["scale_write"] == 1.0
+    with pytest.raises(SystemExit):
+        pr.main(["--bench", os.path.join(t, "bench.json"), "--stats", os.path.join(t, "stats"),
+                 "--traffic-json", table])
