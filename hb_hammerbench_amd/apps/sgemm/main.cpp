@@ -87,7 +87,7 @@ static void usage(const char *prog) {
             "usage: %s [--N n] [--NITER k] [--precision fp32|bf16x3]\n"
             "          [--input pattern|uniform|loguniform] [--seed s]\n"
             "          [--golden host|f64|auto] [--reps r] [--max-print m]\n"
-            "          [--mode square|grouped] [--tile auto|0..4] [--split s] [--tuning file]\n"
+            "          [--mode square|grouped] [--tile auto|0..9] [--split s] [--tuning file]\n"
             "          [--perturb-golden index:delta]\n",
             prog);
 }
