@@ -119,7 +119,10 @@ static bool parse(int argc, char **argv, Options &o) {
         else return false;
     }
     if (o.mode != "square" && o.mode != "grouped") return false;
-    if (o.tile < -1 || o.tile > HBS_TILE_256x256 || o.split < 1 || o.split > 64) return false;
+    if (o.tile < -1 || o.tile > HBS_TILE_128x256 || o.split < 1 || o.split > 64) return false;
+    // the 4 x 1 and 32-deep 128 x 128 tiles exist for FP32 only
+    if (o.precision != "fp32" && (o.tile == HBS_TILE_128x128_4x1 || o.tile == HBS_TILE_128x128_BK32))
+        return false;
     if (o.mode == "square" && (o.tile != -1 || o.split != 1 || !o.tuning.empty())) return false;
     if (o.tile != -1 && !o.tuning.empty()) return false;
     if (o.N < 1 || o.NITER < 1 || o.reps < 0) return false;

@@ -31,6 +31,11 @@ pytestmark = pytest.mark.gpu
      "--split", "2"],
     ["--N", "300", "--NITER", "2", "--mode", "grouped", "--tile", "1", "--split", "3",
      "--input", "uniform", "--golden", "f64", "--reps", "2"],
+    # the other tile configurations: 8-wave 256 x 128 large tiles, and the
+    # FP32-only 4 x 1-wave 128 x 128 tile with split-K
+    ["--N", "640", "--NITER", "2", "--mode", "grouped", "--precision", "bf16x3", "--tile", "8"],
+    ["--N", "300", "--NITER", "2", "--mode", "grouped", "--tile", "5", "--split", "2",
+     "--input", "uniform", "--golden", "f64"],
     # N = 301: slices start off 16-byte boundaries and N is odd, so the
     # grouped plan runs them on its predicated large-tile launch
     ["--N", "301", "--NITER", "3", "--mode", "grouped", "--input", "uniform", "--golden", "f64"],

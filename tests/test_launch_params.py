@@ -68,7 +68,8 @@ LAUNCHER = os.path.join(APP, "sgemm")
 @pytest.mark.parametrize("args", [["--N", "0"], ["--NITER", "-1"], ["--precision", "fp16"],
                                   ["--input", "zeros"], ["--golden", "x"], ["--bogus"],
                                   ["--mode", "batched"], ["--tile", "1"],
-                                  ["--mode", "grouped", "--tile", "5"],
+                                  ["--mode", "grouped", "--tile", "10"],
+                                  ["--mode", "grouped", "--precision", "bf16x3", "--tile", "5"],
                                   ["--mode", "grouped", "--split", "0"],
                                   ["--mode", "grouped", "--tile", "1", "--tuning", "t.txt"]])
 def test_launcher_rejects_bad_arguments(args):
