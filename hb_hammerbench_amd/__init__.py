@@ -18,7 +18,7 @@ __all__ = [
     "fill_pattern", "fill_random", "verify_f64",
     "GEN_UNIFORM", "GEN_LOGUNIFORM", "GEN_NORMAL_OUTLIER", "PREC_FP32", "PREC_BF16X3",
     "GroupedPlan", "Graph", "gemm_fp8res", "fp8res_workspace_size", "gemm_bf16x3",
-    "gemm_fp8res_noalloc", "FP8RES_NOALLOC_SCRATCH_BYTES",
+    "gemm_fp8res_noalloc", "FP8RES_NOALLOC_SCRATCH_BYTES", "gemm_streamk", "streamk_workspace_size",
     "describe_plan",
 ]
 
@@ -94,6 +94,8 @@ SIGNATURES = {
     "hbs_gemm_fp8res_workspace_size": (_sz, [_i, _i, _i]),
     "hbs_gemm_fp8res": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _vp, _sz, _vp]),
     "hbs_gemm_fp8res_noalloc": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _vp]),
+    "hbs_gemm_streamk_workspace_size": (_sz, [_i, _i, _i, _i]),
+    "hbs_gemm_streamk": (_i, [_i, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _vp, _sz, _vp]),
     "hbs_fill_pattern": (_i, [_vp, _vp, _i64, _i64, _vp]),
     "hbs_fill_random": (_i, [_vp, _i64, _i, _u64, _u64, _vp]),
     "hbs_verify_f64": (_i, [_vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _i,
@@ -219,6 +221,28 @@ def gemm_fp8res_noalloc(A, B, C, M, N, K, lda=None, ldb=None, ldc=None, stream=N
                                          K if lda is None else lda, N if ldb is None else ldb,
                                          N if ldc is None else ldc, _stream(stream)),
            "hbs_gemm_fp8res_noalloc")
+
+
+def streamk_workspace_size(M, N, K, precision=PREC_FP32):
+    return int(lib().hbs_gemm_streamk_workspace_size(precision, M, N, K))
+
+
+def gemm_streamk(A, B, C, M, N, K, precision=PREC_FP32, lda=None, ldb=None, ldc=None,
+                 workspace=None, stream=None):
+    """Stream-K FP32 or BF16x3 GEMM.  Without `workspace` a zero-filled one is
+    allocated; a workspace passed in must have been zero-filled once (every
+    call leaves its counters zero).  Returns the workspace for reuse."""
+    ws_bytes = streamk_workspace_size(M, N, K, precision)
+    if workspace is None:
+        import torch
+        workspace = torch.zeros(max(ws_bytes, 1), dtype=torch.uint8, device="cuda")
+    nbytes = workspace.numel() * workspace.element_size()
+    _check(lib().hbs_gemm_streamk(precision, _ptr(A), _ptr(B), _ptr(C), M, N, K,
+                                  K if lda is None else lda, N if ldb is None else ldb,
+                                  N if ldc is None else ldc, _ptr(workspace), nbytes,
+                                  _stream(stream)),
+           "hbs_gemm_streamk")
+    return workspace
 
 
 def fill_pattern(mat1, mat2, n, first=0, stream=None):

@@ -240,6 +240,38 @@ int hbs_gemm_fp8res_noalloc(const float *A, const float *B, float *C, int M, int
                             int lda, int ldb, int ldc, void *stream);
 
 /* ------------------------------------------------------------------------ */
+/* Stream-K GEMM (no reference counterpart): the rectangular FP32 or BF16x3   */
+/* product for grids of output tiles that leave CUs idle.                     */
+/* ------------------------------------------------------------------------ */
+
+/* Workspace bytes of hbs_gemm_streamk for an M x N x K product in `precision`
+ * (HBS_PREC_FP32 or HBS_PREC_BF16X3) on the current device: a counter region
+ * of 256 KiB, then two 256 KiB partial tiles per workgroup (at most one
+ * workgroup per CU: at most 128.25 MiB on a 256-CU MI355X).  0 for invalid
+ * arguments or for products of more than 65536 output tiles of 256 x 256
+ * (those fill the GPU many times over; hbs_sgemm_strided and hbs_gemm_bf16x3
+ * serve them). */
+size_t hbs_gemm_streamk_workspace_size(int precision, int M, int N, int K);
+/* C[M x N] = A[M x K] . B[K x N] (row-major, leading dimensions lda >= K,
+ * ldb >= N, ldc >= N) with FP32-MFMA (HBS_PREC_FP32) or BF16x3
+ * (HBS_PREC_BF16X3) arithmetic, the tiles and main loops of hbs_sgemm_strided
+ * and hbs_gemm_bf16x3, scheduled stream-K: G workgroups (G <= CUs) each take
+ * an equal run of (256 x 256 tile, 32-deep K stage) units, and a tile shared
+ * by several workgroups is summed from their partials in K order by the last
+ * to finish.  G minimises the longest run plus the measured cost of summing
+ * the most-split tile.  Pays where those entry points launch fewer 256 x 256
+ * tiles than CUs over a deep K (M = N = 2048, K = 16384; M = N = 3072, K >=
+ * 3072); the sums of split tiles differ from theirs in rounding order only,
+ * are deterministic, and meet the same error bounds plus one addition per
+ * extra split.  The workspace (hbs_gemm_streamk_workspace_size bytes, 16-byte
+ * aligned) must be all zero before its first use; every call leaves its
+ * counter region zero again, so it can be reused by later calls of any shape
+ * and precision whose size fits, but calls sharing it must be ordered. */
+int hbs_gemm_streamk(int precision, const float *A, const float *B, float *C, int M, int N, int K,
+                     int lda, int ldb, int ldc, void *workspace, size_t workspace_bytes,
+                     void *stream);
+
+/* ------------------------------------------------------------------------ */
 /* Input generation on the device.                                            */
 /* ------------------------------------------------------------------------ */
 
