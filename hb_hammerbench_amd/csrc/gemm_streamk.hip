@@ -67,6 +67,17 @@ __device__ __forceinline__ int unit_owner(int64_t x, int64_t units, int grid) {
     return (int)(((x + 1) * grid + units - 1) / units - 1);
 }
 
+// The calling wave's part of a 256 x 256 tile, as the square kernels' stores
+// place it (edge tiles spread their valid part over the SIMDs).
+template <int PREC, int MODE>
+__device__ __forceinline__ hbs::WaveTile wave_tile_of(int M, int N, int m0, int n0) {
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if constexpr (PREC == HBS_PREC_FP32)
+        return hbs::f32_wave_tile<kBM, kBN, kBK, kWM, kWN, MODE>(M, N, m0, n0, wave);
+    else
+        return hbs::bf16x3_wave_tile<GB, MODE>(M, N, m0, n0, wave);
+}
+
 template <int PREC, int MODE>
 __global__ __launch_bounds__(kThreads) void streamk_kernel(Args p) {
     constexpr bool kF32 = PREC == HBS_PREC_FP32;
@@ -133,9 +144,15 @@ __global__ __launch_bounds__(kThreads) void streamk_kernel(Args p) {
                                                 __HIP_MEMORY_SCOPE_AGENT) == nseg - 1;
             __syncthreads();
             if (s_last) {
-                // one 32 x 32 block at a time (16 values per partial beside the
-                // accumulators), the partials of kMaxSplit segments in flight
-                // together, then added in K order
+                // one 32 x 32 block at a time, the partials of kMaxSplit
+                // segments in flight together, added in K order and stored
+                // straight to C: the accumulators are dead here, which keeps
+                // the register pressure (and the spills around this code) low
+                const hbs::WaveTile wt = wave_tile_of<PREC, MODE>(p.M, p.N, m0, n0);
+                const int ncols = wt.kind == hbs::kWaveHalf ? kFN / 2 : kFN;
+                const int rb = m0 + wt.om + 4 * (lane >> 5), cb = n0 + wt.on + (lane & 31);
+                float *lanep = p.C + (int64_t)rb * p.ldc + cb;
+                const int vm = p.M - rb, vn = p.N - cb;  // valid rows / columns from the lane's own
 #pragma unroll
                 for (int r = 0; r < kFM; ++r) {
 #pragma unroll
@@ -159,10 +176,15 @@ __global__ __launch_bounds__(kThreads) void streamk_kernel(Args p) {
                             for (int j = 0; j < kMaxSplit; ++j)
                                 if (j0 + j < nseg) v += part[j];
                         }
-                        acc[r][c] = v;
+                        if (wt.kind == hbs::kWaveNone || c >= ncols) continue;
+#pragma unroll
+                        for (int q = 0; q < 16; ++q) {
+                            const int dr = r * 32 + 8 * (q >> 2) + (q & 3);
+                            if (MODE == hbs::kTileInterior || (dr < vm && c * 32 < vn))
+                                hbs::stg(lanep + ((int64_t)dr * p.ldc + c * 32), v[q]);
+                        }
                     }
                 }
-                store();
                 if (tid == 0)
                     __hip_atomic_store(&p.counters[t], 0, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
@@ -178,11 +200,11 @@ struct Plan {
 };
 
 // Fix-up cost in units (one 256 x 256 x 32 stage on one CU), measured on
-// MI355X: the last workgroup of a split tile spends about 40 us per
+// MI355X: the last workgroup of a split tile spends about 25 us per
 // kMaxSplit partials it reads (the reads come right after every CU's
 // partial stores and are latency-bound), against a stage of about 7.7 us in
 // FP32 and 2.7 us in BF16x3.
-constexpr int kFixupUnits[2] = {5, 15};
+constexpr int kFixupUnits[2] = {3, 9};
 
 // G minimises  ceil(U / G) + fix-up  over 1 <= G <= min(CUs, U): the longest
 // run of units plus, when any tile is split, the fix-up of the most-split

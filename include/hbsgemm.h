@@ -260,10 +260,11 @@ size_t hbs_gemm_streamk_workspace_size(int precision, int M, int N, int K);
  * by several workgroups is summed from their partials in K order by the last
  * to finish.  G minimises the longest run plus the measured cost of summing
  * the most-split tile.  Pays where those entry points launch fewer 256 x 256
- * tiles than CUs over a deep K (M = N = 2048, K = 16384; M = N = 3072, K >=
- * 3072); the sums of split tiles differ from theirs in rounding order only,
- * are deterministic, and meet the same error bounds plus one addition per
- * extra split.  The workspace (hbs_gemm_streamk_workspace_size bytes, 16-byte
+ * tiles than CUs, most over a deep K (M = N = 2048, K = 16384: 1.1x FP32,
+ * 1.5x BF16x3; M = N = 2560 or 3072, K >= M); loses up to 12% on small
+ * cubes (M = N = K <= 2048).  The sums of split tiles differ from theirs in
+ * rounding order only, are deterministic, and meet the same error bounds
+ * plus one addition per extra split.  The workspace (hbs_gemm_streamk_workspace_size bytes, 16-byte
  * aligned) must be all zero before its first use; every call leaves its
  * counter region zero again, so it can be reused by later calls of any shape
  * and precision whose size fits, but calls sharing it must be ordered. */

@@ -63,8 +63,8 @@ def views(a, b, c, M, N, K, lda, ldb, ldc, offs=(0, 0, 0)):
     return A, B, C, Ch
 
 
-# (M, N, K, pads, pointer offsets): 36 tiles over 144 workgroups (each tile
-# split four ways); ragged M and N (clamped loads); K not a multiple of 32
+# (M, N, K, pads, pointer offsets): 36 tiles over 216 (FP32) or 144 (BF16x3)
+# workgroups (each tile split six or four ways); ragged M and N (clamped loads); K not a multiple of 32
 # (zero fill past K in the last segment); unaligned operands (predicated
 # loads); runs that do not divide a tile's stages; a deep K over few tiles
 # (every CU, tiles split 16 ways or more); fewer units than CUs.
@@ -102,7 +102,7 @@ def test_streamk_within_bound_and_padding_untouched(prec, case):
 
 @pytest.mark.parametrize("prec", [hb.PREC_FP32, hb.PREC_BF16X3])
 def test_streamk_reference_pattern_bit_exact(prec):
-    """N = 1536 on the reference's i%7 / i%3 inputs, tiles split four ways:
+    """N = 1536 on the reference's i%7 / i%3 inputs, tiles split six or four ways:
     exact, as in the one-tile-per-workgroup kernels."""
     N = 1536
     a = torch.empty(N * N, device="cuda"); b = torch.empty(N * N, device="cuda")
@@ -142,8 +142,11 @@ def test_streamk_deterministic_counters_zero_and_workspace_reuse():
 def test_streamk_argument_checks():
     L = hb.lib()
     slot = 256 * 256 * 4
+    # 36 tiles: FP32 splits each over 6 workgroups, BF16x3 (whose stages are
+    # shorter against the same fix-up) over 4
     need = hb.streamk_workspace_size(1536, 1536, 1536)
-    assert need == 256 * 1024 + 2 * 144 * slot  # 36 tiles x 4 workgroups
+    assert need == 256 * 1024 + 2 * 216 * slot
+    assert hb.streamk_workspace_size(1536, 1536, 1536, hb.PREC_BF16X3) == 256 * 1024 + 2 * 144 * slot
     for prec in (hb.PREC_FP32, hb.PREC_BF16X3):
         assert hb.streamk_workspace_size(4096, 4096, 4096, prec) == 256 * 1024 + 2 * 256 * slot
         assert hb.streamk_workspace_size(300, 200, 96, prec) == 256 * 1024 + 2 * 2 * slot
