@@ -34,7 +34,8 @@ def parse():
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--variant", choices=["fp32", "bf16x3", "grouped-fp32", "grouped-bf16x3",
-                                         "fp8res"], default="fp32")
+                                         "fp8res", "streamk-fp32", "streamk-bf16x3"],
+                   default="fp32")
     p.add_argument("--problems", type=int, default=512, help="grouped: number of problems")
     p.add_argument("--dump-shapes", default=None,
                    help="grouped: write the seeded problem shapes (M N K per line) and exit")
@@ -53,7 +54,8 @@ def parse():
     p.add_argument("--no-vendor", action="store_true",
                    help="skip the same-box vendor-library comparison (torch / hipBLASLt)")
     p.add_argument("--no-compare", action="store_true",
-                   help="fp32: skip the BF16x3 comparison on the same workload")
+                   help="fp32: skip the BF16x3 comparison on the same workload; streamk: skip "
+                        "the same product through the strided entry point")
     p.add_argument("--mode", choices=["gemm", "copy"], default="gemm",
                    help="copy: a plain device-to-device copy of --copy-mib MiB, the "
                         "reference kernel that HBM byte counters are calibrated on")
@@ -466,6 +468,64 @@ def setup_fp8res(args, hb, torch):
         "traffic": measured_traffic("fp8res", f"M={M} N={N} K={K}")}
 
 
+def setup_streamk(args, hb, torch):
+    """Stream-K, M = N = 2048, K = 16384: 64 output tiles of 256 x 256 over a
+    deep K, a product the one-tile-per-workgroup kernels run on a quarter of
+    the CUs.  FP32 on uniform[-1,1) inputs, BF16x3 on log-uniform ones, as in
+    the square variants; the workspace is allocated and zeroed once, outside
+    the step."""
+    M = N = 2048
+    K = 16384
+    fp32 = args.variant == "streamk-fp32"
+    prec = hb.PREC_FP32 if fp32 else hb.PREC_BF16X3
+    kind = hb.GEN_UNIFORM if fp32 else hb.GEN_LOGUNIFORM
+    a = torch.empty(M * K, dtype=torch.float32, device="cuda")
+    b = torch.empty(K * N, dtype=torch.float32, device="cuda")
+    c = torch.empty(M * N, dtype=torch.float32, device="cuda")
+    hb.fill_random(a, M * K, kind, args.seed)
+    hb.fill_random(b, K * N, kind, args.seed + 1)
+    ws = torch.zeros(hb.streamk_workspace_size(M, N, K, prec), dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+
+    def step():
+        hb.gemm_streamk(a, b, c, M, N, K, prec, workspace=ws)
+
+    def verify():
+        st = hb.verify_f64(a, b, c, M, N, K)
+        return {"rows": "all", "e_cw": st["e_cw"], "e_nw": st["e_nw"], "e_F": st["e_F"],
+                "oracle": "FP64 on device (hbs_verify_f64)"}
+
+    def strided():
+        """The same product through the one-tile-per-workgroup entry point."""
+        f = (lambda: hb.sgemm_strided(a, b, c, M, N, K, K, N, N)) if fp32 else \
+            (lambda: hb.gemm_bf16x3(a, b, c, M, N, K))
+        wall, _ = time_steps(torch, 10, 2, f)
+        return {"entry": "hbs_sgemm_strided" if fp32 else "hbs_gemm_bf16x3",
+                "value": round(2.0 * M * N * K / wall / 1e12, 3), "unit": "TFLOP/s"}
+
+    def vendor(ours):
+        A2, B2 = a.view(M, K), b.view(K, N)
+        if fp32:
+            C2 = torch.empty(M, N, dtype=torch.float32, device="cuda")  # c keeps our result
+            return vendor_leg(torch, 2.0 * M * N * K, [(lambda: torch.mm(A2, B2, out=C2), 1)],
+                              "torch.mm in float32", ours)
+        ah, bh = A2.bfloat16(), B2.bfloat16()
+        ch = torch.empty(M, N, dtype=torch.bfloat16, device="cuda")
+        return vendor_leg(torch, 2.0 * M * N * K, [(lambda: torch.mm(ah, bh, out=ch), 3)],
+                          "three torch.mm products in bfloat16", ours)
+
+    G = (ws.numel() - 256 * 1024) // (2 * 256 * 256 * 4)
+    workload = (f"stream-K {'FP32' if fp32 else 'BF16x3'} GEMM, M=N=2048, K=16384 (64 tiles, "
+                f"{G} workgroups), {'uniform[-1,1)' if fp32 else 'log-uniform'} seed {args.seed}")
+    return step, 2.0 * M * N * K, workload, {
+        "vendor": vendor,
+        "verify": verify,
+        "strided": strided,
+        "config": {"M": M, "N": N, "K": K, "workgroups": G,
+                   "bytes_algorithmic": 4.0 * (M * K + K * N + M * N)},
+        "traffic": measured_traffic(args.variant, f"M={M} N={N} K={K}")}
+
+
 def main():
     args = parse()
     if args.dump_shapes:
@@ -489,6 +549,8 @@ def main():
         setup = setup_grouped(args, hb, torch)
     elif args.variant == "fp8res":
         setup = setup_fp8res(args, hb, torch)
+    elif args.variant.startswith("streamk"):
+        setup = setup_streamk(args, hb, torch)
     else:
         setup = setup_square(args, hb, torch)
     step, flops, workload, extra = setup
@@ -556,6 +618,8 @@ def main():
         torch.cuda.synchronize()
     if not args.no_verify:
         line["accuracy"] = extra["verify"]()
+    if "strided" in extra and not args.no_compare:
+        line["strided_same_box"] = extra["strided"]()
     if "compare" in extra:
         # after the FP32 result is verified: the comparison overwrites the inputs
         line["comparison"] = {"bf16x3": extra["compare"](value)}
