@@ -1,8 +1,9 @@
 """Host threads calling the C-ABI at once, each on its own stream.
 
-Four threads each run, three times over, the FP32 and BF16x3 SGEMMs, the FP8 +
-residual GEMM (with its own workspace) and a grouped BF16x3 plan (created,
-captured into a graph, launched and destroyed in the thread).  ctypes drops
+Four threads each run, eight times over, the FP32 and BF16x3 SGEMMs, the FP8 +
+residual GEMM and the stream-K GEMM (each with its own workspace) and a grouped
+BF16x3 plan (created, captured into a graph, launched and destroyed in the
+thread).  ctypes drops
 the GIL during the calls, so the library's entry points, its per-device
 caches and the grouped graph capture run concurrently.  Every kernel is
 deterministic, so each thread's outputs must equal, bit for bit, the same
@@ -23,6 +24,7 @@ pytestmark = pytest.mark.gpu
 THREADS, ROUNDS = 4, 8
 N, NITER = 512, 2
 FM, FN, FK = 384, 512, 320  # FP8 shape: the 256 x 256 kernel with ragged rows
+SM, SN, SK = 300, 520, 1000  # stream-K shape: 6 tiles split over several workgroups
 SHAPES = [(256, 320, 192), (512, 512, 256), (100, 260, 68), (64, 576, 512)]
 
 
@@ -40,18 +42,24 @@ def make_outputs():
         "b3": torch.empty(N * N * NITER, device="cuda"),
         "fp8": torch.empty(FM * FN, device="cuda"),
         "grouped": torch.empty(sum(M * Nn for M, Nn, _ in SHAPES), device="cuda"),
+        "sk": torch.empty(SM * SN, device="cuda"),
     }
     ws = torch.empty(hb.fp8res_workspace_size(FM, FN, FK), dtype=torch.uint8, device="cuda")
-    return out, ws
+    skws = torch.zeros(hb.streamk_workspace_size(SM, SN, SK, hb.PREC_BF16X3), dtype=torch.uint8,
+                       device="cuda")
+    return out, (ws, skws)
 
 
-def run_all(a, b, out, ws, stream):
+def run_all(a, b, out, wss, stream):
     """Every entry point once on `stream`; returns the objects that must
     outlive the launches."""
+    ws, skws = wss
     hb.sgemm(a, b, out["f32"], N, NITER, stream=stream)
     hb.sgemm_bf16x3(a, b, out["b3"], N, NITER, stream=stream)
     hb.gemm_fp8res(a[:FM * FK], b[:FK * FN], out["fp8"], FM, FN, FK, workspace=ws,
                    stream=stream)
+    hb.gemm_streamk(a[:SM * SK], b[:SK * SN], out["sk"], SM, SN, SK, hb.PREC_BF16X3,
+                    workspace=skws, stream=stream)
     probs, oa, ob, oc = [], 0, 0, 0
     for M, Nn, K in SHAPES:
         A, B, C = a[oa:oa + M * K], b[ob:ob + K * Nn], out["grouped"][oc:oc + M * Nn]

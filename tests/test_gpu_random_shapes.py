@@ -10,6 +10,9 @@ launches are all reached.  Bars, per element:
  * FP32 (hbs_sgemm_strided): |C - R| <= gamma_{K+2} S;
  * BF16x3 (hbs_gemm_bf16x3): |C - R| <= (3.1 * 2^-16 + gamma_{3K+2}) S;
  * grouped plans of both precisions: the same per problem;
+ * stream-K (hbs_gemm_streamk) of both precisions: the same with s more
+   additions for a tile split s ways (s <= G + 1, G its workgroups), through
+   one workspace reused by every case;
 with R and S = |A||B| from the oracle's FP64 product, and every element of C
 outside the M x N result (padding columns and the gaps around it) unchanged.
 """
@@ -30,10 +33,10 @@ def gamma(n):
     return n * U / (1 - n * U)
 
 
-def bound(prec, K, S):
+def bound(prec, K, S, extra=0):
     if prec == hb.PREC_FP32:
-        return gamma(K + 2) * S
-    return (3.1 * 2.0 ** -16 + gamma(3 * K + 2)) * S
+        return gamma(K + 2 + extra) * S
+    return (3.1 * 2.0 ** -16 + gamma(3 * K + 2 + extra)) * S
 
 
 def draw_case(rng, big=False):
@@ -59,7 +62,7 @@ def operands(case, seed):
     return a, b, c, lda, ldb, ldc
 
 
-def check(prec, case, a, b, c, lda, ldb, ldc):
+def check(prec, case, a, b, c, lda, ldb, ldc, extra=0):
     M, N, K, _, (oa, ob, oc) = case
     Ah, Bh, Ch = a.cpu().numpy(), b.cpu().numpy(), c.cpu().numpy()
     A = np.lib.stride_tricks.as_strided(Ah[oa:], (M, K), (lda * 4, 4)).copy()
@@ -67,7 +70,7 @@ def check(prec, case, a, b, c, lda, ldb, ldc):
     C = np.lib.stride_tricks.as_strided(Ch[oc:], (M, N), (ldc * 4, 4)).copy()
     R, S = oracle.gemm_f64(A, B)
     err = np.abs(C.astype(np.float64) - R)
-    assert np.all(err <= bound(prec, K, S)), (case, float(err.max()))
+    assert np.all(err <= bound(prec, K, S, extra)), (case, float(err.max()))
     written = np.zeros(Ch.size, bool)
     for r in range(M):
         written[oc + r * ldc: oc + r * ldc + N] = True
@@ -126,3 +129,20 @@ def test_random_grouped_plans(prec, seed):
         check(prec, case, a, b, c, lda, ldb, ldc)
     graph.close()
     plan.close()
+
+
+@pytest.mark.parametrize("prec", [hb.PREC_FP32, hb.PREC_BF16X3])
+def test_random_shapes_streamk(prec):
+    """24 random cases through one stream-K workspace: each call must leave
+    the tile counters zero for the next, whatever its shape."""
+    ws = torch.zeros(hb.streamk_workspace_size(8192, 8192, 8192, prec), dtype=torch.uint8,
+                     device="cuda")
+    for i, case in enumerate(cases(4100 + prec, 24, big_every=3)):
+        print(f"streamk prec {prec} case {i}: {case}", flush=True)
+        a, b, c, lda, ldb, ldc = operands(case, 400 + i)
+        M, N, K, _, (oa, ob, oc) = case
+        hb.gemm_streamk(a[oa:], b[ob:], c[oc:], M, N, K, prec, lda, ldb, ldc, workspace=ws)
+        torch.cuda.synchronize()
+        G = (hb.streamk_workspace_size(M, N, K, prec) - 256 * 1024) // (2 * 256 * 256 * 4)
+        check(prec, case, a, b, c, lda, ldb, ldc, extra=G + 1)
+        assert not ws[:256 * 1024].any().item(), case

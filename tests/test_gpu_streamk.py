@@ -165,3 +165,30 @@ def test_streamk_argument_checks():
                               need - 4, None) == -3  # HBS_ERR_ALIGN
     assert L.hbs_gemm_streamk(0, ptr, ptr, ptr, 64, 64, 64, 32, 64, 64, ws.data_ptr(), need,
                               None) == -1  # lda < K
+
+
+@pytest.mark.parametrize("prec", [hb.PREC_FP32, hb.PREC_BF16X3])
+def test_streamk_graph_replays_match_eager(prec):
+    """Captured into a graph (the counters are reset by the kernel itself, so
+    the graph holds the one launch and no memset), each of four replays, with
+    the program's own work and synchronisations between them, gives the eager
+    call's bits and leaves the counters zero."""
+    M, N, K = 1000, 1300, 1024
+    a, b, c = operands(M, N, K, K, N, N, 31)
+    ws = torch.zeros(hb.streamk_workspace_size(M, N, K, prec), dtype=torch.uint8, device="cuda")
+    hb.gemm_streamk(a, b, c, M, N, K, prec, workspace=ws)
+    torch.cuda.synchronize()
+    eager = c.clone()
+    side = torch.cuda.Stream()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=side):
+        hb.gemm_streamk(a, b, c, M, N, K, prec, workspace=ws)
+    for rep in range(4):
+        c.fill_(SENTINEL)
+        junk = torch.randn(1 << 20, device="cuda").sum()  # other work between replays
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(c.view(torch.int32), eager.view(torch.int32)), rep
+        assert not ws[:256 * 1024].any().item(), rep
+        del junk
