@@ -58,6 +58,7 @@ struct Args {
     int tiles_m, tiles_n, iters;  // iters: K stages per tile
     int64_t units;                // tiles * iters
     int grid;                     // G
+    int remap;                    // give each XCD a contiguous range of workgroups
     float *partials;              // 2 G slots of kTileFloats
     int *counters;                // one per tile, zero at rest
 };
@@ -84,7 +85,15 @@ __global__ __launch_bounds__(kThreads) void streamk_kernel(Args p) {
     constexpr int LDS = kF32 ? GF::kLdsFloats * 4 : 2 * GB::kStage * 2;
     __shared__ __attribute__((aligned(16))) char smem[LDS];
     __shared__ int s_last;
-    const int w = hbs::xcd_remap(blockIdx.x, gridDim.x);
+    // Workgroups hold runs in block order.  With few segments per tile each
+    // XCD takes a contiguous range of them (xcd_remap), so a split tile's
+    // partials are summed from this XCD's L2.  With 8 or more per tile the
+    // blocks are left in the hardware's round-robin over the XCDs, which puts
+    // on one XCD the workgroups that cover the same K range of neighbouring
+    // tiles: they share A and B panels in its L2 (measured +4% FP32, +8-13%
+    // BF16x3 at 1024 x 1024 x 32768; the remap is better by 4-15% at
+    // M = N = K <= 2048).
+    const int w = p.remap ? hbs::xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
     const int64_t u1 = (int64_t)(w + 1) * p.units / p.grid;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int64_t u = (int64_t)w * p.units / p.grid; u < u1;) {
@@ -290,8 +299,9 @@ extern "C" int hbs_gemm_streamk(int precision, const float *A, const float *B, f
     else if (vec && K % kBK == 0) mode = hbs::kTileClamp;
     else if (vec && K % 4 == 0 && (!fp32 || N % 4 == 0)) mode = hbs::kTileClampK;
     char *ws = (char *)workspace;
+    const int remap = pl.grid < 8 * pl.tiles_m * pl.tiles_n;
     const Args a{A, B, C, M, N, K, lda, ldb, ldc, pl.tiles_m, pl.tiles_n, pl.iters, pl.units,
-                 pl.grid, (float *)(ws + kCounterBytes), (int *)ws};
+                 pl.grid, remap, (float *)(ws + kCounterBytes), (int *)ws};
     return fp32 ? launch<HBS_PREC_FP32>(a, mode, (hipStream_t)stream)
                 : launch<HBS_PREC_BF16X3>(a, mode, (hipStream_t)stream);
 }
