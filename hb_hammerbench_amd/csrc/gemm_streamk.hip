@@ -280,12 +280,12 @@ extern "C" int hbs_gemm_streamk(int precision, const float *A, const float *B, f
     if (M < 1 || N < 1 || K < 1 || lda < K || ldb < N || ldc < N) return HBS_ERR_ARG;
     if (!hbs::dims_ok(M, N, K, lda, ldb, ldc)) return HBS_ERR_ARG;
     if (!A || !B || !C) return HBS_ERR_NULL;
-    const int st = hbs::check_device();
-    if (st) return st;
     if ((int64_t)hbs::ceil_div(M, kBM) * hbs::ceil_div(N, kBN) > kMaxTiles) return HBS_ERR_ARG;
     const Plan pl = make_plan(precision, M, N, K, hbs::cu_count());
     if (!workspace || workspace_bytes_given < workspace_bytes(pl)) return HBS_ERR_WORKSPACE;
     if (!hbs::aligned16(workspace)) return HBS_ERR_ALIGN;
+    const int st = hbs::check_device();
+    if (st) return st;
     // tile modes as in the square kernels: interior when every tile is whole
     // and the operands allow 16-byte loads, clamped loads for ragged M or N,
     // zero fill past K when K is not a multiple of the stage, else

@@ -6,6 +6,8 @@ import ctypes
 import os
 import re
 
+import numpy as np
+
 import pytest
 
 import hb_hammerbench_amd as hb
@@ -182,3 +184,40 @@ def test_square_and_rectangular_entries_reject_sizes_beyond_the_int_limit():
     assert L.hbs_gemm_fp8res_workspace_size(8, 8, big) == 0
     out = hb.ErrStats()
     assert L.hbs_verify_f64(P, P, P, big, 8, 8, 8, 8, 8, 0, 8, ctypes.byref(out), None) == -1
+
+
+def test_streamk_plan_and_argument_checks():
+    """The stream-K workspace follows its plan (256 KiB of counters, then two
+    256 KiB partial slots for each of G workgroups, 1 <= G <= min(CUs, units));
+    argument, size and workspace errors come before any device call."""
+    L = hb.lib()
+    slot, counters = 256 * 256 * 4, 256 * 1024
+    cus = 256  # MI355X, and the planner's default when no device answers
+    rng = np.random.default_rng(5)
+    for prec in (hb.PREC_FP32, hb.PREC_BF16X3):
+        for _ in range(40):
+            M, N, K = (int(x) for x in rng.integers(1, 9000, 3))
+            ws = L.hbs_gemm_streamk_workspace_size(prec, M, N, K)
+            units = -(-M // 256) * -(-N // 256) * -(-K // 32)
+            G, rem = divmod(ws - counters, 2 * slot)
+            assert rem == 0 and 1 <= G <= min(cus, units), (M, N, K, ws)
+        # products that fill the GPU take one workgroup per CU
+        assert L.hbs_gemm_streamk_workspace_size(prec, 4096, 4096, 4096) == counters + 2 * cus * slot
+        # one stage of one tile: one workgroup
+        assert L.hbs_gemm_streamk_workspace_size(prec, 5, 7, 3) == counters + 2 * slot
+        assert L.hbs_gemm_streamk_workspace_size(prec, 0, 8, 8) == 0
+        assert L.hbs_gemm_streamk_workspace_size(prec, 8, 8, 0x7fffffff) == 0
+        assert L.hbs_gemm_streamk_workspace_size(prec, 300 * 256, 300 * 256, 32) == 0  # > 65536 tiles
+    assert L.hbs_gemm_streamk_workspace_size(2, 64, 64, 64) == 0
+    need = L.hbs_gemm_streamk_workspace_size(0, 64, 64, 64)
+    W = ctypes.c_void_p(1 << 20)  # never dereferenced: every call below fails its checks
+    assert L.hbs_gemm_streamk(2, P, P, P, 64, 64, 64, 64, 64, 64, W, need, None) == -1
+    assert L.hbs_gemm_streamk(0, P, P, P, 0, 64, 64, 64, 64, 64, W, need, None) == -1
+    assert L.hbs_gemm_streamk(0, P, P, P, 64, 64, 64, 32, 64, 64, W, need, None) == -1
+    assert L.hbs_gemm_streamk(0, None, P, P, 64, 64, 64, 64, 64, 64, W, need, None) == -2
+    assert L.hbs_gemm_streamk(0, P, P, P, 64, 64, 64, 64, 64, 64, W, need - 1, None) == -4
+    assert L.hbs_gemm_streamk(0, P, P, P, 64, 64, 64, 64, 64, 64, None, need, None) == -4
+    assert L.hbs_gemm_streamk(0, P, P, P, 64, 64, 64, 64, 64, 64, ctypes.c_void_p((1 << 20) + 4),
+                              need, None) == -3
+    big = 0x7fffffff - 4096 + 1
+    assert L.hbs_gemm_streamk(0, P, P, P, 8, 8, big, big, 8, 8, W, need, None) == -1
