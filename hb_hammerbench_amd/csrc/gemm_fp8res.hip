@@ -15,8 +15,8 @@
 //   integers exact, so the reference pattern (0..6 x 0..2) stays bit-exact.
 //
 // Launches: quantise A, quantise B (transposed, K-contiguous), GEMM.  The
-// workspace holds, per operand X (rows R = M or N, padded to Rp; Kp = K padded
-// to 128):  X8 [Rp][Kp] e4m3 bytes, Xr [Rp][Kp] bf16 residuals, and the tile
+// workspace holds, per operand X (rows R = M or N, padded to Rp, a multiple of
+// 256; Kp = K padded to 128):  X8 [Rp][Kp] e4m3 bytes, Xr [Rp][Kp] bf16 residuals, and the tile
 // exponents eX [Rp/128][Kp/128] int32 (for B indexed [n tile][k tile]) -- three
 // bytes per element.  The bf16 copy of q that the residual products need is
 // rebuilt from the e4m3 bytes in registers (v_cvt_scalef32_pk_bf16_fp8, exact:
@@ -357,7 +357,7 @@ __global__ __launch_bounds__(kThreads) void gemm_fp8res_kernel(GemmArgs p) {
 }
 
 // Wide GEMM: 256 x 256 output tile, 8 waves as 4 x 2 of 64 x 128, BK = 64,
-// for padded widths Np that are multiples of 256.  A third fewer bytes are
+// chosen over the 256 x 128 form by use_wide below.  A third fewer bytes are
 // loaded per flop than with the 256 x 128 tile.  Its stages do not fit twice
 // in the 160 KB LDS (96 KB each), so the e4m3 planes have one LDS buffer and
 // the residual planes two:
@@ -546,11 +546,13 @@ struct Layout {
     size_t a8, ar, ea, b8, br, eb, total;  // byte offsets
 };
 
-// M, N, K <= hbs::kMaxDim, so the padded sizes fit in int
+// M, N, K <= hbs::kMaxDim, so the padded sizes fit in int.  B is padded to
+// a multiple of the wide tile's 256 columns whichever form runs: the 256 x 128
+// form reads only its first round_up(N, 128) rows of the B planes.
 Layout layout(int M, int N, int K) {
     Layout L;
     L.Mp = (int)round_up(M, kBM);
-    L.Np = (int)round_up(N, kBN);
+    L.Np = (int)round_up(N, kWBN);
     L.Kp = (int)round_up(K, kQT);
     auto al = [](size_t x) { return (size_t)round_up((int64_t)x, 256); };
     size_t o = 0;
@@ -576,11 +578,28 @@ int check_args(const float *A, const float *B, const float *C, int M, int N, int
 // holds L.total bytes, 16-byte aligned; every byte the GEMM reads is written
 // by the quantisers first (padding rows and columns as zeros), so nothing in
 // it needs clearing beforehand.
+// Whether the 256 x 256 form runs: always when round_up(N, 128) is already a
+// multiple of 256; otherwise when its rounds of one workgroup per CU, each
+// costing 1.75 rounds of the 256 x 128 form (its tile does twice the work at
+// 0.52 against 0.46 of the roofline; 1.72 measured at 16384 x 16512 x 4096),
+// take less time than the 256 x 128 form's rounds.  Measured at M = 16384,
+// K = 4096 (wide against 256 x 128 time): N = 384 0.79x and 1152 0.92x
+// (256 x 128 kept), 2176 1.00x, 640 1.12x, 4224 1.06x, 16512 1.13x (wide).
+bool use_wide(const Layout &L, int N) {
+    const int64_t narrow_w = round_up(N, kBN);
+    if (narrow_w % kWBN == 0) return true;
+    const int64_t cus = hbs::cu_count(), tm = L.Mp / kBM;
+    const int64_t narrow_rounds = (tm * (narrow_w / kBN) + cus - 1) / cus;
+    const int64_t wide_rounds = (tm * (L.Np / kWBN) + cus - 1) / cus;
+    return 7 * wide_rounds < 4 * narrow_rounds;
+}
+
 int run_fp8res(const float *A, const float *B, float *C, int M, int N, int K, int lda, int ldb,
                int ldc, const Layout &L, char *ws, hipStream_t s) {
     const int64_t qa_blocks = (int64_t)(L.Mp / kQT) * (L.Kp / kQT);
     const int64_t qb_blocks = (int64_t)(L.Np / kQT) * (L.Kp / kQT);
-    const int64_t blocks = (int64_t)(L.Mp / kBM) * (L.Np / kBN);
+    const int narrow_tn = (int)(round_up(N, kBN) / kBN);
+    const int64_t blocks = (int64_t)(L.Mp / kBM) * narrow_tn;
     if (qa_blocks > 0x7fffffffLL || qb_blocks > 0x7fffffffLL || blocks > 0x7fffffffLL)
         return HBS_ERR_ARG;
     QuantArgs qa{A, M, K, lda, hbs::aligned16(A) && lda % 4 == 0, L.Mp, L.Kp,
@@ -597,8 +616,8 @@ int run_fp8res(const float *A, const float *B, float *C, int M, int N, int K, in
     GemmArgs g{(const uint8_t *)(ws + L.a8), (const uint8_t *)(ws + L.b8),
                (const uint16_t *)(ws + L.ar), (const uint16_t *)(ws + L.br),
                (const int *)(ws + L.ea), (const int *)(ws + L.eb), C, M, N, L.Kp, ldc,
-               L.Mp / kBM, L.Np / kBN};
-    if (L.Np % kWBN == 0) {
+               L.Mp / kBM, narrow_tn};
+    if (use_wide(L, N)) {
         g.tiles_n = L.Np / kWBN;
         const unsigned wblocks = (unsigned)(g.tiles_m * g.tiles_n);
         if (M % kBM == 0 && N % kWBN == 0)

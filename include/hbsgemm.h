@@ -212,7 +212,7 @@ int hbs_graph_destroy(void *graph_exec);
 /* ------------------------------------------------------------------------ */
 
 /* Workspace bytes for an M x N x K problem.  Layout (byte offsets aligned to
- * 256; Mp = M rounded up to 256, Np = N rounded up to 128, Kp = K rounded up
+ * 256; Mp = M rounded up to 256, Np = N rounded up to 256, Kp = K rounded up
  * to 128): A8 [Mp][Kp] e4m3 | Ar [Mp][Kp] bf16 | eA [Mp/128][Kp/128] int32 |
  * B8 [Np][Kp] | Br [Np][Kp] | eB [Np/128][Kp/128] (B stored transposed: row n
  * holds column n of B).  q = decode(X8) * 2^e is rebuilt on the fly. */

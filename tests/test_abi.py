@@ -155,6 +155,24 @@ def test_fp8res_argument_checks():
     assert L.hbs_gemm_fp8res(P, P, P, 8, 8, 8, 8, 8, 8, ctypes.c_void_p(8), 1 << 30, None) == -3
 
 
+def test_fp8res_workspace_follows_documented_layout():
+    """The workspace size is the layout include/hbsgemm.h documents: Mp and Np
+    rounded up to 256, Kp to 128, every plane starting on a 256-byte boundary
+    (B is padded to the wide kernel's 256 columns whichever form runs)."""
+    L = hb.lib()
+
+    def rup(x, m):
+        return (x + m - 1) // m * m
+    for M, N, K in [(1, 1, 1), (300, 200, 100), (512, 384, 1024), (16384, 640, 4096),
+                    (16384, 16384, 4096), (257, 129, 129)]:
+        Mp, Np, Kp = rup(M, 256), rup(N, 256), rup(K, 128)
+        end = 0
+        for nbytes in (Mp * Kp, Mp * Kp * 2, (Mp // 128) * (Kp // 128) * 4,
+                       Np * Kp, Np * Kp * 2, (Np // 128) * (Kp // 128) * 4):
+            end = rup(end + nbytes, 256)
+        assert L.hbs_gemm_fp8res_workspace_size(M, N, K) == end, (M, N, K)
+
+
 def test_fp8res_noalloc_argument_checks():
     """Argument and size errors of the allocation-free entry point come before
     any device call: problems whose workspace exceeds the fixed scratch get

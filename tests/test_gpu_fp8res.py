@@ -40,7 +40,7 @@ def rup(x, m):
 
 def planes(ws, M, N, K):
     """Splits the documented workspace layout into its planes (numpy)."""
-    Mp, Np, Kp = rup(M, 256), rup(N, 128), rup(K, 128)
+    Mp, Np, Kp = rup(M, 256), rup(N, 256), rup(K, 128)
     w = ws.cpu().numpy()
     o = 0
 
@@ -115,15 +115,17 @@ def test_fp8res_quantiser_matches_oracle(M, N, K):
     assert np.array_equal(oracle.bf16_bits_to_f32(Br_)[:N, :K], Br)
     assert np.array_equal(decode_plane(A8, eA, M, K), Aq)
     assert np.array_equal(decode_plane(B8, eB, N, K), Bq)
-    # padding is zero
+    # padding is zero (the wide kernel reads B's rows up to round_up(N, 256))
     assert not A8[M:].any() and not A8[:, K:].any()
+    assert not B8[N:].any() and not B8[:, K:].any() and not Br_[N:].any()
 
 
-# Padded widths Np (N rounded up to 128) that are multiples of 256 run the
-# 256 x 256 kernel, the others the 256 x 128 one: (256, 512, 384) and
-# (384, 768, 512) take the wide kernel, (300, 200, 100) and (300, 700, 200)
-# its predicated-store form; (256, 512, 100) and (512, 256, 128) give it the
-# shortest pipeline (Kp = 128: two 64-deep stages).
+# Widths whose round_up(N, 128) is a multiple of 256 run the 256 x 256
+# kernel; at these sizes (few tiles) the others run the 256 x 128 one:
+# (256, 512, 384) and (384, 768, 512) take the wide kernel, (300, 200, 100)
+# and (300, 700, 200) its predicated-store form; (256, 512, 100) and
+# (512, 256, 128) give it the shortest pipeline (Kp = 128: two 64-deep
+# stages); (256, 128, 256) and (512, 384, 1024) take the 256 x 128 kernel.
 @pytest.mark.parametrize("M,N,K", [(256, 128, 256), (300, 200, 100), (512, 384, 1024),
                                    (256, 512, 384), (384, 768, 512), (300, 700, 200),
                                    (256, 512, 100), (512, 256, 128)])
@@ -167,6 +169,45 @@ def test_fp8res_wide_tile_equals_narrow_tile(M, K):
     wide = c512.view(M, 512)[:, :384].cpu().numpy()
     narrow = c384.view(M, 384).cpu().numpy()
     assert np.array_equal(wide.view(np.uint32), narrow.view(np.uint32))
+
+
+def test_fp8res_padded_wide_form_equals_narrow_form():
+    """N = 640 at M = 16384: round_up(N, 128) = 640 is not a multiple of 256,
+    and on 256 CUs the 256 x 256 form (192 tiles, one round) beats the
+    256 x 128 form (320 tiles, two rounds), so the wide kernel runs over B's
+    planes padded to 768 columns, storing C's columns past N under
+    predicates.  N = 384 runs the 256 x 128 form.  B's first 384 columns are
+    whole scaling tiles in both, so the shared columns agree bit for bit, and
+    C's columns past N (ldc = 704) keep their sentinel."""
+    M, K, N, ldc = 16384, 256, 640, 704
+    a = torch.empty(M * K, dtype=torch.float32, device="cuda")
+    b = torch.empty(K * N, dtype=torch.float32, device="cuda")
+    hb.fill_random(a, M * K, hb.GEN_NORMAL_OUTLIER, 21)
+    hb.fill_random(b, K * N, hb.GEN_NORMAL_OUTLIER, 22)
+    b384 = b.view(K, N)[:, :384].contiguous().view(-1)
+    c640 = torch.full((M * ldc,), 7.5, dtype=torch.float32, device="cuda")
+    c384 = torch.empty(M * 384, dtype=torch.float32, device="cuda")
+    hb.gemm_fp8res(a, b, c640, M, N, K, ldc=ldc)
+    hb.gemm_fp8res(a, b384, c384, M, 384, K)
+    torch.cuda.synchronize()
+    wide = c640.view(M, ldc).cpu().numpy()
+    narrow = c384.view(M, 384).cpu().numpy()
+    assert np.array_equal(wide[:, :384].view(np.uint32), narrow.view(np.uint32))
+    assert np.all(wide[:, N:] == 7.5)
+    # every column of the wide run within the algorithm's bound on a row band;
+    # the band is one row of 128 x 128 scaling tiles, so quantising it alone
+    # gives the scales the whole of A gets
+    rows = slice(4096, 4224)
+    A = a.view(M, K)[rows].cpu().numpy()
+    B = b.view(K, N).cpu().numpy()
+    Aq, Ar, _ = oracle.fp8_quantize(np.ascontiguousarray(A))
+    BqT, BrT, _ = oracle.fp8_quantize(np.ascontiguousarray(B.T))
+    Bq, Br = BqT.T.astype(np.float64), BrT.T.astype(np.float64)
+    Aq, Ar = Aq.astype(np.float64), Ar.astype(np.float64)
+    ideal = Aq @ Bq + Aq @ Br + Ar @ Bq
+    absum = np.abs(Aq) @ np.abs(Bq) + np.abs(Aq) @ np.abs(Br) + np.abs(Ar) @ np.abs(Bq)
+    err = np.abs(wide[rows, :N] - ideal)
+    assert np.all(err <= gamma(3 * K + 2) * absum + fp8_block_term(Aq, Bq))
 
 
 def test_fp8res_identity_b_reproduces_quantised_a():
