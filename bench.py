@@ -18,6 +18,7 @@ the other workloads.
 import argparse
 import json
 import os
+import statistics
 import sys
 import time
 
@@ -606,6 +607,7 @@ def main():
             "frac": round(achieved / peak, 4),
             "traffic": extra.get("traffic"),
             "kernel_ms_avg": round(avg_ms, 4),
+            "kernel_ms_median": round(statistics.median(per_launch_ms), 4),
         },
     }
     if not args.no_vendor and "vendor" in extra:
