@@ -78,15 +78,16 @@ constexpr int kBigThreads = 512;
 // Epilogue for a tile of WM x WN waves, each with FM x FN 32x32 accumulator
 // blocks (wave w at wm = w % WM, wn = w / WM, as the tile configs place it).
 // Direct tiles store C; split tiles store a partial, and the last split to
-// arrive sums all partials in split order and stores C.
-template <int BM, int BN, int WM, int WN>
+// arrive sums all partials in split order and stores C.  Q16: the
+// accumulator layout of store_acc (the tile configuration's kQ16 for BF16x3,
+// false for FP32).
+template <int BM, int BN, int WM, int WN, bool Q16>
 __device__ __forceinline__ void finish_tile(floatx16 (&acc)[BM / WM / 32][BN / WN / 32],
                                             const Prob &p, const Item &it, float *partials,
                                             int *counters, int *s_flag) {
     constexpr int FM = BM / WM / 32, FN = BN / WN / 32;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave % WM, wn = wave / WM;
-    const int l32 = lane & 31, h = lane >> 5;
     const int m0 = it.tm * BM, n0 = it.tn * BN;
 
     if (it.nsplit > 1) {
@@ -112,8 +113,8 @@ __device__ __forceinline__ void finish_tile(floatx16 (&acc)[BM / WM / 32][BN / W
 #pragma unroll 1
         for (int b = 0; b < FM * FN; ++b) {
             const int r = b / FN, c = b % FN;
-            const int row0 = m0 + wm * (BM / WM) + r * 32 + 4 * h;
-            const int col = n0 + wn * (BN / WN) + c * 32 + l32;
+            const int row0 = m0 + wm * (BM / WM) + r * 32 + hbs::acc_lane_row<Q16>(lane);
+            const int col = n0 + wn * (BN / WN) + c * 32 + hbs::acc_lane_col<Q16>(lane);
             float *cp = p.C + (int64_t)row0 * p.ldc + col;
             const float *src0 = partials + it.poff + wave * (FM * FN * 16 * 64) + b * 16 * 64 + lane;
             float v[16];
@@ -127,8 +128,8 @@ __device__ __forceinline__ void finish_tile(floatx16 (&acc)[BM / WM / 32][BN / W
             }
 #pragma unroll
             for (int q = 0; q < 16; ++q) {
-                const int dr = 8 * (q >> 2) + (q & 3);
-                if (col < p.N && row0 + dr < p.M) hbs::stg(cp + (int64_t)dr * p.ldc, v[q]);
+                const int dr = hbs::acc_row<Q16>(q), dc = hbs::acc_col<Q16>(q);
+                if (col + dc < p.N && row0 + dr < p.M) hbs::stg(cp + ((int64_t)dr * p.ldc + dc), v[q]);
             }
         }
         if (tid == 0)
@@ -137,9 +138,9 @@ __device__ __forceinline__ void finish_tile(floatx16 (&acc)[BM / WM / 32][BN / W
     }
     const int rb = m0 + wm * (BM / WM), cb = n0 + wn * (BN / WN);
     if (m0 + BM <= p.M && n0 + BN <= p.N)
-        hbs::store_acc<FM, FN, false>(p.C, p.ldc, rb, cb, p.M, p.N, FN, acc);
+        hbs::store_acc<FM, FN, false, Q16>(p.C, p.ldc, rb, cb, p.M, p.N, FN, acc);
     else
-        hbs::store_acc<FM, FN, true>(p.C, p.ldc, rb, cb, p.M, p.N, FN, acc);
+        hbs::store_acc<FM, FN, true, Q16>(p.C, p.ldc, rb, cb, p.M, p.N, FN, acc);
 }
 
 // ---------------------------------------------------------------------------
@@ -161,7 +162,7 @@ __device__ void f32_tile(float *smem, const Prob &p, const Item &it, float *part
     typename G::Regs regs;
     hbs::NoChainGlobal ch;  // operands from the problem table: global loads
     hbs::f32_block_mma<BM, BN, BK, WM, WN, hbs::kTileClampK>(smem, t, acc, regs, ch);
-    finish_tile<BM, BN, WM, WN>(acc, p, it, partials, counters, s_flag);
+    finish_tile<BM, BN, WM, WN, false>(acc, p, it, partials, counters, s_flag);
 }
 
 template <int BM, int BN, int WM, int WN>
@@ -176,7 +177,7 @@ __device__ void b3_tile(uint16_t *smem, const Prob &p, const Item &it, float *pa
     typename G::Regs regs;
     hbs::NoChainGlobal ch;  // operands from the problem table: global loads
     hbs::bf16x3_block_mma<G, hbs::kTileClampK>(smem, t, acc, regs, ch);
-    finish_tile<BM, BN, WM, WN>(acc, p, it, partials, counters, s_flag);
+    finish_tile<BM, BN, WM, WN, G::kQ16>(acc, p, it, partials, counters, s_flag);
 }
 
 // LDS of the largest configuration of each precision (FP32: 128 x 128 with
@@ -264,7 +265,8 @@ __global__ __launch_bounds__(kBigThreads) void grouped_big_kernel(const Prob *__
     static_assert(MODE == hbs::kTileClamp || MODE == hbs::kTileClampK || MODE == hbs::kTileEdge,
                   "mode");
     using GF = hbs::F32Cfg<BM, BN, 32, WM, WN>;
-    using GB = hbs::B3Cfg<BM, BN, 32, WM, WN>;
+    // BF16x3 on 16 x 16 MFMAs (B3Cfg: more flops per joule at the power limit)
+    using GB = hbs::B3Cfg<BM, BN, 32, WM, WN, true>;
     static_assert(GF::kThreads == kBigThreads && GB::kThreads == kBigThreads, "8 waves");
     constexpr int LDS = kF32 ? GF::kLdsFloats * 4 : 2 * GB::kStage * 2;
     using Regs = std::conditional_t<kF32, typename GF::Regs, typename GB::Regs>;

@@ -35,7 +35,7 @@ namespace {
 
 constexpr int kBM = 256, kBN = 256, kBK = 32, kWM = 4, kWN = 2, kThreads = 512;
 using GF = hbs::F32Cfg<kBM, kBN, kBK, kWM, kWN>;
-using GB = hbs::B3Main;
+using GB = hbs::B3Main16;  // BF16x3 on 16 x 16 MFMAs (more flops per joule at the power limit)
 static_assert(GF::kThreads == kThreads && GB::kThreads == kThreads, "8 waves");
 static_assert(GF::kFM == GB::kFM && GF::kFN == GB::kFN, "accumulators");
 constexpr int kFM = GF::kFM, kFN = GF::kFN;
@@ -82,6 +82,7 @@ __device__ __forceinline__ hbs::WaveTile wave_tile_of(int M, int N, int m0, int 
 template <int PREC, int MODE>
 __global__ __launch_bounds__(kThreads) void streamk_kernel(Args p) {
     constexpr bool kF32 = PREC == HBS_PREC_FP32;
+    constexpr bool kQ16 = !kF32 && GB::kQ16;  // accumulator layout (store_acc)
     constexpr int LDS = kF32 ? GF::kLdsFloats * 4 : 2 * GB::kStage * 2;
     __shared__ __attribute__((aligned(16))) char smem[LDS];
     __shared__ int s_last;
@@ -159,7 +160,8 @@ __global__ __launch_bounds__(kThreads) void streamk_kernel(Args p) {
                 // the register pressure (and the spills around this code) low
                 const hbs::WaveTile wt = wave_tile_of<PREC, MODE>(p.M, p.N, m0, n0);
                 const int ncols = wt.kind == hbs::kWaveHalf ? kFN / 2 : kFN;
-                const int rb = m0 + wt.om + 4 * (lane >> 5), cb = n0 + wt.on + (lane & 31);
+                const int rb = m0 + wt.om + hbs::acc_lane_row<kQ16>(lane);
+                const int cb = n0 + wt.on + hbs::acc_lane_col<kQ16>(lane);
                 float *lanep = p.C + (int64_t)rb * p.ldc + cb;
                 const int vm = p.M - rb, vn = p.N - cb;  // valid rows / columns from the lane's own
 #pragma unroll
@@ -188,9 +190,10 @@ __global__ __launch_bounds__(kThreads) void streamk_kernel(Args p) {
                         if (wt.kind == hbs::kWaveNone || c >= ncols) continue;
 #pragma unroll
                         for (int q = 0; q < 16; ++q) {
-                            const int dr = r * 32 + 8 * (q >> 2) + (q & 3);
-                            if (MODE == hbs::kTileInterior || (dr < vm && c * 32 < vn))
-                                hbs::stg(lanep + ((int64_t)dr * p.ldc + c * 32), v[q]);
+                            const int dr = r * 32 + hbs::acc_row<kQ16>(q);
+                            const int dc = c * 32 + hbs::acc_col<kQ16>(q);
+                            if (MODE == hbs::kTileInterior || (dr < vm && dc < vn))
+                                hbs::stg(lanep + ((int64_t)dr * p.ldc + dc), v[q]);
                         }
                     }
                 }

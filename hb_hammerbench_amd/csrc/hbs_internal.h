@@ -50,24 +50,49 @@ __device__ __forceinline__ void stg(T *p, T v) {
     *(__attribute__((address_space(1))) T *)p = v;
 }
 
-// Stores a wave's FM x FN accumulator blocks of 32 x 32 (v_mfma_f32_32x32x*
-// results: in block (r, c) lane l holds column 32 c + (l & 31) and, in
-// register q, row 32 r + 8 (q >> 2) + 4 (l >> 5) + (q & 3)) to C at the
+// Where a lane's accumulator registers sit in C.  A wave's sub-tile is held
+// as FM x FN regions of 32 x 32, one floatx16 per region, in one of two
+// layouts:
+//   Q16 = false (v_mfma_f32_32x32x*): lane l holds column (l & 31) and, in
+//     register q, row 8 (q >> 2) + 4 (l >> 5) + (q & 3);
+//   Q16 = true (four v_mfma_f32_16x16x* blocks, block s = q >> 2 at rows
+//     16 (s & 1), columns 16 (s >> 1) of the region): lane l holds column
+//     16 (q >> 3) + (l & 15) and row 16 ((q >> 2) & 1) + 4 (l >> 4) + (q & 3).
+// acc_lane_row / acc_lane_col: the lane's own offset; acc_row / acc_col: the
+// wave-uniform offset of register q.
+template <bool Q16>
+__device__ __forceinline__ int acc_lane_row(int lane) {
+    return Q16 ? 4 * (lane >> 4) : 4 * (lane >> 5);
+}
+template <bool Q16>
+__device__ __forceinline__ int acc_lane_col(int lane) {
+    return Q16 ? (lane & 15) : (lane & 31);
+}
+template <bool Q16>
+__host__ __device__ constexpr int acc_row(int q) {
+    return Q16 ? 16 * ((q >> 2) & 1) + (q & 3) : 8 * (q >> 2) + (q & 3);
+}
+template <bool Q16>
+__host__ __device__ constexpr int acc_col(int q) {
+    return Q16 ? 16 * (q >> 3) : 0;
+}
+
+// Stores a wave's FM x FN accumulator regions (layout Q16, above) to C at the
 // wave's sub-tile origin (rb, cb), row stride ldc.  The lane's own address is
 // formed once; each store adds a wave-uniform offset to it, so a store costs
 // one 64-bit add and no 64-bit multiply.  PRED: only rows < M and columns < N
-// are written.  Column blocks from ncols on are skipped (a half-width wave).
+// are written.  Column regions from ncols on are skipped (a half-width wave).
 // (A 16-byte-store form, each group of four registers transposed across its
 // lane quad by DPP, was bit-identical but 2.5% slower on the grouped BF16x3
 // group: its 16 extra VALU instructions per four values cost more than the
 // three quarters of the store instructions they save.)
-template <int FM, int FN, bool PRED>
+template <int FM, int FN, bool PRED, bool Q16 = false>
 __device__ __forceinline__ void store_acc(float *__restrict__ C, int ldc, int rb, int cb, int M,
                                           int N, int ncols, const floatx16 (&acc)[FM][FN]) {
     const int lane = threadIdx.x & 63;
-    const int l32 = lane & 31, h = lane >> 5;
-    float *lanep = C + ((int64_t)rb + 4 * h) * ldc + cb + l32;
-    const int vm = M - rb - 4 * h, vn = N - cb - l32;  // valid rows / columns from the lane's own
+    const int lr = acc_lane_row<Q16>(lane), lc = acc_lane_col<Q16>(lane);
+    float *lanep = C + ((int64_t)rb + lr) * ldc + cb + lc;
+    const int vm = M - rb - lr, vn = N - cb - lc;  // valid rows / columns from the lane's own
 #pragma unroll
     for (int r = 0; r < FM; ++r) {
 #pragma unroll
@@ -75,9 +100,9 @@ __device__ __forceinline__ void store_acc(float *__restrict__ C, int ldc, int rb
             if (c >= ncols) continue;
 #pragma unroll
             for (int q = 0; q < 16; ++q) {
-                const int dr = r * 32 + 8 * (q >> 2) + (q & 3);
-                if (!PRED || (dr < vm && c * 32 < vn))
-                    stg(lanep + ((int64_t)dr * ldc + c * 32), acc[r][c][q]);
+                const int dr = r * 32 + acc_row<Q16>(q), dc = c * 32 + acc_col<Q16>(q);
+                if (!PRED || (dr < vm && dc < vn))
+                    stg(lanep + ((int64_t)dr * ldc + dc), acc[r][c][q]);
             }
         }
     }

@@ -3,8 +3,11 @@
 //
 // Each float is split x = hi + lo, hi = bf16_rne(x), lo = bf16_rne(x - hi)
 // (x - hi is exact), and C = Alo.Bhi + Ahi.Blo + Ahi.Bhi is accumulated in
-// FP32 by v_mfma_f32_32x32x16_bf16; the lo.lo term (relative size 2^-16) is
-// dropped.  The split happens in registers on the way to LDS.
+// FP32 by v_mfma_f32_32x32x16_bf16 or, in configurations with Q16 set, by
+// v_mfma_f32_16x16x32_bf16; the lo.lo term (relative size 2^-16) is dropped.
+// The split happens in registers on the way to LDS.  Accumulators are handed
+// out as one floatx16 per 32 x 32 region, in the layout store_acc names by
+// the same Q16 flag.
 #pragma once
 
 #include <type_traits>
@@ -15,9 +18,16 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 namespace hbs {
 
-template <int BM_, int BN_, int BK_, int WM_, int WN_>
+// Q16_: the MFMA shape.  On random data a loop of v_mfma_f32_16x16x32_bf16
+// sustains 16% more BF16 flops than one of v_mfma_f32_32x32x16_bf16 at the
+// board's power limit (1960 against 1685 TFLOP/s; both about 2490 on zeros),
+// so the 16 x 16 form pays where these kernels run at the power limit with
+// short tiles (the grouped large-tile launch +3.6%, stream-K +4%); the square
+// kernel measured 3% slower with it and keeps 32 x 32.
+template <int BM_, int BN_, int BK_, int WM_, int WN_, bool Q16_ = false>
 struct B3Cfg {
     static constexpr int BM = BM_, BN = BN_, BK = BK_, WM = WM_, WN = WN_;
+    static constexpr bool kQ16 = Q16_;
     static constexpr int kThreads = WM * WN * 64;
     static constexpr int kWTM = BM / WM, kWTN = BN / WN;
     static constexpr int kFM = kWTM / 32, kFN = kWTN / 32;
@@ -57,6 +67,8 @@ struct B3Cfg {
 // Production configuration: 256 x 256 tile, BK = 32, 8 waves of 64 x 128 (the
 // fastest of the shapes measured at N = 8192, NITER = 10).
 using B3Main = B3Cfg<256, 256, 32, 4, 2>;
+// The same tile on 16 x 16 MFMAs: the grouped large-tile launch and stream-K.
+using B3Main16 = B3Cfg<256, 256, 32, 4, 2, true>;
 
 // hi and lo planes of two floats, each as one dword holding two bf16 (x0 low).
 __device__ __forceinline__ void split_pk(float x0, float x1, uint32_t &hi, uint32_t &lo) {
@@ -102,14 +114,26 @@ __device__ __forceinline__ void bf16x3_block_mma_impl(uint16_t *smem, const floa
     constexpr int kLdsRow = G::kLdsRow, kStage = G::kStage;
     constexpr int kGroups = G::kGroups, kItemsA = G::kItemsA, kItemsB = G::kItemsB;
 
+    constexpr bool kQ16 = G::kQ16;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const int l32 = lane & 31, h = lane >> 5;
+    const int l32 = lane & 31, h = lane >> 5;  // 32 x 32 fragments: row / column, k half
+    const int l16 = lane & 15, g = lane >> 4;  // 16 x 16 fragments: row / column, 8-k group
 
+    // Q16: 16 x 16 blocks of v_mfma_f32_16x16x32_bf16, block (rr, cc) at rows
+    // 16 rr, columns 16 cc of the wave's sub-tile, packed into acc at the end
+    floatx4 a4[kQ16 ? 2 * kFM : 1][kQ16 ? 2 * kFN : 1];
+    if constexpr (kQ16) {
 #pragma unroll
-    for (int r = 0; r < kFM; ++r)
+        for (int r = 0; r < 2 * kFM; ++r)
 #pragma unroll
-        for (int c = 0; c < kFN; ++c) acc[r][c] = floatx16{0};
+            for (int c = 0; c < 2 * kFN; ++c) a4[r][c] = floatx4{0, 0, 0, 0};
+    } else {
+#pragma unroll
+        for (int r = 0; r < kFM; ++r)
+#pragma unroll
+            for (int c = 0; c < kFN; ++c) acc[r][c] = floatx16{0};
+    }
 
     // the stage in flight: the caller's registers when chained (they carry
     // the next tile's stage 1 across the call), else two local arrays
@@ -211,43 +235,92 @@ __device__ __forceinline__ void bf16x3_block_mma_impl(uint16_t *smem, const floa
             *(uint4 *)(d + 8) = uint4{lo[0], lo[1], lo[2], lo[3]};
         }
     };
-    // one 16-deep substep of the stage in buffer `buf`
-    auto compute = [&](int buf, int st) {
-        if (CK == kWaveNone) return;
-        const uint16_t *as = smem + buf * kStage + (om + l32) * kLdsRow;
-        const uint16_t *bsm = smem + buf * kStage + kBM * kLdsRow + (on + l32) * kLdsRow;
-        const int off = (2 * st + h) * 16;  // this lane half's 8-k group
-        bf16x8 ahi[kFM], alo[kFM], bhi[kFNc], blo[kFNc];
+    // Half st of the stage in buffer `buf`.
+    //   32 x 32 MFMAs: the 16-deep substep st; lane half h reads 8-k group
+    //   2 st + h of its fragment row (or column) l & 31.
+    //   Q16: the wave's 16-column blocks st kFNc .. st kFNc + kFNc - 1 of the
+    //   stage's one 32-deep step against all its 16-row blocks; lane l reads
+    //   8-k group l >> 4 of row (or column) l & 15 of a block.  The A
+    //   fragments are read once per stage (load_a) and serve both halves:
+    //   re-reading them in each half (a third more LDS reads) gave the
+    //   grouped launch no gain over 32 x 32 MFMAs.
+    // Each fragment is one 16-byte hi and one lo read.
+    bf16x8 ahi16[kQ16 ? 2 * kFM : 1], alo16[kQ16 ? 2 * kFM : 1];
+    auto load_a = [&](int buf) {
+        if constexpr (kQ16) {
+            if (CK == kWaveNone) return;
+            const uint16_t *as = smem + buf * kStage + (om + l16) * kLdsRow + g * 16;
 #pragma unroll
-        for (int r = 0; r < kFM; ++r) {
-            ahi[r] = *(const bf16x8 *)(as + r * 32 * kLdsRow + off);
-            alo[r] = *(const bf16x8 *)(as + r * 32 * kLdsRow + off + 8);
-        }
-#pragma unroll
-        for (int c = 0; c < kFNc; ++c) {
-            bhi[c] = *(const bf16x8 *)(bsm + c * 32 * kLdsRow + off);
-            blo[c] = *(const bf16x8 *)(bsm + c * 32 * kLdsRow + off + 8);
-        }
-        // Consecutive MFMAs share their second operand (the B fragment): per
-        // column block, each product term runs over the row blocks.  On the
-        // same inputs this order measured 8% faster than issuing the three
-        // terms of one block back to back (17% with all-zero data), while
-        // sharing the first operand instead measured no gain.  Each
-        // accumulator still receives lo.hi, hi.lo, hi.hi in that order.
-#pragma unroll
-        for (int c = 0; c < kFNc; ++c) {
-#pragma unroll
-            for (int r = 0; r < kFM; ++r)
-                acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[r], bhi[c], acc[r][c], 0, 0, 0);
-#pragma unroll
-            for (int r = 0; r < kFM; ++r)
-                acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[r], blo[c], acc[r][c], 0, 0, 0);
-#pragma unroll
-            for (int r = 0; r < kFM; ++r)
-                acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[r], bhi[c], acc[r][c], 0, 0, 0);
+            for (int r = 0; r < 2 * kFM; ++r) {
+                ahi16[r] = *(const bf16x8 *)(as + r * 16 * kLdsRow);
+                alo16[r] = *(const bf16x8 *)(as + r * 16 * kLdsRow + 8);
+            }
         }
     };
-    static_assert(kBK == 32, "two substeps per stage");
+    // Consecutive MFMAs share their second operand (the B fragment): per
+    // column block, each product term runs over the row blocks.  On the same
+    // inputs this order measured 8% faster than issuing the three terms of
+    // one block back to back (17% with all-zero data), while sharing the
+    // first operand instead measured no gain.  Each accumulator still
+    // receives lo.hi, hi.lo, hi.hi in that order.
+    auto compute = [&](int buf, int st) {
+        if (CK == kWaveNone) return;
+        if constexpr (kQ16) {
+            const uint16_t *bsm =
+                smem + buf * kStage + kBM * kLdsRow + (on + l16) * kLdsRow + g * 16;
+            bf16x8 bhi[kFNc], blo[kFNc];
+#pragma unroll
+            for (int j = 0; j < kFNc; ++j) {
+                const int c = st * kFNc + j;
+                bhi[j] = *(const bf16x8 *)(bsm + c * 16 * kLdsRow);
+                blo[j] = *(const bf16x8 *)(bsm + c * 16 * kLdsRow + 8);
+            }
+#pragma unroll
+            for (int j = 0; j < kFNc; ++j) {
+                const int c = st * kFNc + j;
+#pragma unroll
+                for (int r = 0; r < 2 * kFM; ++r)
+                    a4[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo16[r], bhi[j], a4[r][c],
+                                                                       0, 0, 0);
+#pragma unroll
+                for (int r = 0; r < 2 * kFM; ++r)
+                    a4[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi16[r], blo[j], a4[r][c],
+                                                                       0, 0, 0);
+#pragma unroll
+                for (int r = 0; r < 2 * kFM; ++r)
+                    a4[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi16[r], bhi[j], a4[r][c],
+                                                                       0, 0, 0);
+            }
+        } else {
+            const uint16_t *as = smem + buf * kStage + (om + l32) * kLdsRow;
+            const uint16_t *bsm = smem + buf * kStage + kBM * kLdsRow + (on + l32) * kLdsRow;
+            const int off = (2 * st + h) * 16;  // this lane half's 8-k group
+            bf16x8 ahi[kFM], alo[kFM], bhi[kFNc], blo[kFNc];
+#pragma unroll
+            for (int r = 0; r < kFM; ++r) {
+                ahi[r] = *(const bf16x8 *)(as + r * 32 * kLdsRow + off);
+                alo[r] = *(const bf16x8 *)(as + r * 32 * kLdsRow + off + 8);
+            }
+#pragma unroll
+            for (int c = 0; c < kFNc; ++c) {
+                bhi[c] = *(const bf16x8 *)(bsm + c * 32 * kLdsRow + off);
+                blo[c] = *(const bf16x8 *)(bsm + c * 32 * kLdsRow + off + 8);
+            }
+#pragma unroll
+            for (int c = 0; c < kFNc; ++c) {
+#pragma unroll
+                for (int r = 0; r < kFM; ++r)
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[r], bhi[c], acc[r][c], 0, 0, 0);
+#pragma unroll
+                for (int r = 0; r < kFM; ++r)
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[r], blo[c], acc[r][c], 0, 0, 0);
+#pragma unroll
+                for (int r = 0; r < kFM; ++r)
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[r], bhi[c], acc[r][c], 0, 0, 0);
+            }
+        }
+    };
+    static_assert(kBK == 32, "each stage computed in two halves");
 
     // Pipeline: the global loads of stage kt + 2 are issued in the middle of
     // step kt, right after stage kt + 1 has been split into the other LDS
@@ -270,6 +343,7 @@ __device__ __forceinline__ void bf16x3_block_mma_impl(uint16_t *smem, const floa
     int kt = 0;
     for (; kt + 2 < KT; ++kt) {
         const int cur = (kt + b0) & 1;
+        load_a(cur);
         compute(cur, 0);
         sstore(cur ^ 1);
         gload((kt + 2) * kBK);
@@ -288,6 +362,7 @@ __device__ __forceinline__ void bf16x3_block_mma_impl(uint16_t *smem, const floa
     }
     if (kt + 1 < KT) {
         const int cur = (kt + b0) & 1;
+        load_a(cur);
         compute(cur, 0);
         sstore(cur ^ 1);
         if constexpr (Chain::kOn) {
@@ -301,6 +376,7 @@ __device__ __forceinline__ void bf16x3_block_mma_impl(uint16_t *smem, const floa
         ++kt;
     }
     const int cur = (kt + b0) & 1;
+    load_a(cur);
     compute(cur, 0);
     if constexpr (Chain::kOn) {
         if (chain) {
@@ -319,6 +395,15 @@ __device__ __forceinline__ void bf16x3_block_mma_impl(uint16_t *smem, const floa
         }
         ch.pre = chain;
         ch.buf0 = cur ^ 1;
+    }
+    if constexpr (kQ16) {
+#pragma unroll
+        for (int r = 0; r < kFM; ++r)
+#pragma unroll
+            for (int c = 0; c < kFN; ++c)
+#pragma unroll
+                for (int q = 0; q < 16; ++q)
+                    acc[r][c][q] = a4[2 * r + ((q >> 2) & 1)][2 * c + (q >> 3)][q & 3];
     }
 }
 
@@ -382,9 +467,11 @@ __device__ __forceinline__ void bf16x3_block_store(float *__restrict__ Cg, int M
     if (wt.kind == kWaveNone) return;
     const int ncols = wt.kind == kWaveHalf ? G::kFN / 2 : G::kFN;
     if (MODE == kTileInterior || (m0 + G::BM <= M && n0 + G::BN <= N))
-        store_acc<G::kFM, G::kFN, false>(Cg, ldc, m0 + wt.om, n0 + wt.on, M, N, ncols, acc);
+        store_acc<G::kFM, G::kFN, false, G::kQ16>(Cg, ldc, m0 + wt.om, n0 + wt.on, M, N, ncols,
+                                                   acc);
     else
-        store_acc<G::kFM, G::kFN, true>(Cg, ldc, m0 + wt.om, n0 + wt.on, M, N, ncols, acc);
+        store_acc<G::kFM, G::kFN, true, G::kQ16>(Cg, ldc, m0 + wt.om, n0 + wt.on, M, N, ncols,
+                                                  acc);
 }
 
 // One whole output tile: main loop and store.
