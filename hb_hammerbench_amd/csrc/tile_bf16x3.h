@@ -22,8 +22,8 @@ namespace hbs {
 // sustains 16% more BF16 flops than one of v_mfma_f32_32x32x16_bf16 at the
 // board's power limit (1960 against 1685 TFLOP/s; both about 2490 on zeros),
 // so the 16 x 16 form pays where these kernels run at the power limit with
-// short tiles (the grouped large-tile launch +3.6%, stream-K +4%); the square
-// kernel measured 3% slower with it and keeps 32 x 32.
+// short tiles (the grouped large-tile launch +3.7%, stream-K +3.6%); the
+// square kernel measured 3-4% slower with it and keeps 32 x 32.
 template <int BM_, int BN_, int BK_, int WM_, int WN_, bool Q16_ = false>
 struct B3Cfg {
     static constexpr int BM = BM_, BN = BN_, BK = BK_, WM = WM_, WN = WN_;
