@@ -58,7 +58,7 @@ extern "C" const char *hbs_strerror(int code) {
     return "unknown status code";
 }
 
-extern "C" int hbs_version(void) { return 400; }
+extern "C" int hbs_version(void) { return 500; }
 
 // ---------------------------------------------------------------------------
 // apps/sgemm/main.cpp:46-52 -- mat1[i] = i % 7, mat2[i] = i % 3.

@@ -183,8 +183,10 @@ int hbs_grouped_plan_describe(const hbs_gemm_problem *problems, int count, int p
  * the plan has 256 x 256 tiles), then the 256-thread persistent launch (when
  * it has smaller tiles).  Workgroups take their items from queues in plan
  * memory, which reset themselves at the end of each launch.  Both precisions
- * read the operands in place.  Deterministic; runs of one plan must not
- * overlap (they share the queues, split-K counters and partial buffers). */
+ * read the operands in place; BF16x3 tiles of the large-tile launch run on
+ * 16 x 16 x 32 MFMAs, so they round differently from hbs_gemm_bf16x3 (same
+ * error bounds).  Deterministic; runs of one plan must not overlap (they
+ * share the queues, split-K counters and partial buffers). */
 int hbs_grouped_run(const hbs_grouped_plan *plan, void *stream);
 /* Frees the plan's device buffers (synchronous).  Returns HBS_ERR_IN_USE and
  * frees nothing while a graph made from the plan by hbs_grouped_graph_create
@@ -255,16 +257,18 @@ size_t hbs_gemm_streamk_workspace_size(int precision, int M, int N, int K);
 /* C[M x N] = A[M x K] . B[K x N] (row-major, leading dimensions lda >= K,
  * ldb >= N, ldc >= N) with FP32-MFMA (HBS_PREC_FP32) or BF16x3
  * (HBS_PREC_BF16X3) arithmetic, the tiles and main loops of hbs_sgemm_strided
- * and hbs_gemm_bf16x3, scheduled stream-K: G workgroups (G <= CUs) each take
+ * and hbs_gemm_bf16x3 (BF16x3 here on 16 x 16 x 32 MFMAs instead of
+ * 32 x 32 x 16), scheduled stream-K: G workgroups (G <= CUs) each take
  * an equal run of (256 x 256 tile, 32-deep K stage) units, and a tile shared
  * by several workgroups is summed from their partials in K order by the last
  * to finish.  G minimises the longest run plus the measured cost of summing
  * the most-split tile.  Pays where those entry points launch fewer 256 x 256
  * tiles than CUs, most over a deep K (M = N = 2048, K = 16384: 1.1x FP32,
  * 1.5x BF16x3; M = N = 2560 or 3072, K >= M); loses up to 12% on small
- * cubes (M = N = K <= 2048).  The sums of split tiles differ from theirs in
- * rounding order only, are deterministic, and meet the same error bounds
- * plus one addition per extra split.  The workspace (hbs_gemm_streamk_workspace_size bytes, 16-byte
+ * cubes (M = N = K <= 2048).  Results differ from theirs in rounding order
+ * only (the split tiles' sums, and in BF16x3 the MFMA shape), are
+ * deterministic, and meet the same error bounds plus one addition per extra
+ * split.  The workspace (hbs_gemm_streamk_workspace_size bytes, 16-byte
  * aligned) must be all zero before its first use; every call leaves its
  * counter region zero again, so it can be reused by later calls of any shape
  * and precision whose size fits, but calls sharing it must be ordered. */

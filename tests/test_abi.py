@@ -41,7 +41,7 @@ def test_python_binding_covers_every_declared_symbol():
 
 def test_strerror_and_version():
     L = hb.lib()
-    assert L.hbs_version() == 400  # 0.4.0: hbs_gemm_fp8res_noalloc
+    assert L.hbs_version() == 500  # 0.5.0: FP8 workspace pads B to 256 columns; grouped and stream-K BF16x3 on 16x16 MFMAs
     for code in [0, -1, -2, -3, -4, -5, -6, -7, -8]:
         assert L.hbs_strerror(code)
     assert L.hbs_strerror(-99) == b"unknown status code"
