@@ -70,7 +70,7 @@ int run_bf16x3(const float *A, const float *B, float *C, int M, int N, int K, in
     const int64_t big_tiles = (int64_t)((M + 255) / 256) * ((N + 255) / 256) * batch;
     const int64_t mid_tiles = (int64_t)((M + 127) / 128) * ((N + 127) / 128) * batch;
     if (hbs::prefer_big_tiles(big_tiles, mid_tiles, hbs::cu_count(), 0.66))
-        return launch_gemm<hbs::B3Main>(A, B, C, M, N, K, lda, ldb, ldc, sA, sB, sC, batch, 8,
+        return launch_gemm<hbs::B3Main16>(A, B, C, M, N, K, lda, ldb, ldc, sA, sB, sC, batch, 8,
                                         stream);
     if (mid_tiles >= 128)
         return launch_gemm<hbs::B3Cfg<128, 128, 32, 2, 2>>(A, B, C, M, N, K, lda, ldb, ldc, sA,

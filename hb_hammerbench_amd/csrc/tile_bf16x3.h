@@ -21,9 +21,9 @@ namespace hbs {
 // Q16_: the MFMA shape.  On random data a loop of v_mfma_f32_16x16x32_bf16
 // sustains 16% more BF16 flops than one of v_mfma_f32_32x32x16_bf16 at the
 // board's power limit (1960 against 1685 TFLOP/s; both about 2490 on zeros),
-// so the 16 x 16 form pays where these kernels run at the power limit with
-// short tiles (the grouped large-tile launch +3.7%, stream-K +3.6%); the
-// square kernel measured 3-4% slower with it and keeps 32 x 32.
+// so the 16 x 16 form pays in these power-limited kernels: the grouped
+// large-tile launch +3.7%, and with the pinned schedule of the main loop
+// (q16_schedule) the square kernel +4.9% and stream-K +9% over 32 x 32.
 template <int BM_, int BN_, int BK_, int WM_, int WN_, bool Q16_ = false>
 struct B3Cfg {
     static constexpr int BM = BM_, BN = BN_, BK = BK_, WM = WM_, WN = WN_;
@@ -64,10 +64,11 @@ struct B3Cfg {
         }
     }
 };
-// Production configuration: 256 x 256 tile, BK = 32, 8 waves of 64 x 128 (the
-// fastest of the shapes measured at N = 8192, NITER = 10).
+// 256 x 256 tile, BK = 32, 8 waves of 64 x 128 (the fastest of the shapes
+// measured at N = 8192, NITER = 10), on 32 x 32 MFMAs ...
 using B3Main = B3Cfg<256, 256, 32, 4, 2>;
-// The same tile on 16 x 16 MFMAs: the grouped large-tile launch and stream-K.
+// ... and the production form on 16 x 16 MFMAs: the square kernel, the
+// grouped large-tile launch and stream-K.
 using B3Main16 = B3Cfg<256, 256, 32, 4, 2, true>;
 
 // hi and lo planes of two floats, each as one dword holding two bf16 (x0 low).
@@ -340,6 +341,38 @@ __device__ __forceinline__ void bf16x3_block_mma_impl(uint16_t *smem, const floa
         if (KT > 1) gload(kBK);
         __syncthreads();
     }
+    // Q16 steady-state schedule of a full-width 64 x 128 wave (48 MFMAs per
+    // half stage).  Left alone, the compiler runs a dozen MFMAs, then the
+    // whole split (VALU) and LDS store of the next stage, then the other
+    // MFMAs in one run, and the matrix core idles through the split (MFMA
+    // busy 55.7% against 67.6% on 32 x 32).  Pinned here: half 0's MFMAs each
+    // followed by three VALU instructions of the split, then by one LDS
+    // write; then half 1's B reads (they may not pass the LDS writes); then
+    // half 1's MFMAs with the next stage's global loads between them.
+    auto q16_schedule = [&]() {
+        if constexpr (kQ16 && CK == kWaveFull && kFM == 2 && kFNc == 4) {
+#pragma unroll
+            for (int i = 0; i < 40; ++i) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+                __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);  // VALU
+            }
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x002, 5, 0);
+#pragma unroll
+            for (int i = 0; i < 7; ++i) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+                __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);  // LDS write
+            }
+            __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);  // LDS read: half 1's B
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+#pragma unroll
+            for (int i = 0; i < 20; ++i) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+                __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // global load
+            }
+        }
+    };
     int kt = 0;
     for (; kt + 2 < KT; ++kt) {
         const int cur = (kt + b0) & 1;
@@ -351,6 +384,7 @@ __device__ __forceinline__ void bf16x3_block_mma_impl(uint16_t *smem, const floa
         if constexpr (Chain::kOn) {
             if (kt == 0) ch.publish();
         }
+        q16_schedule();
         __syncthreads();
     }
     bool chain = false;
