@@ -348,9 +348,12 @@ __device__ __forceinline__ void bf16x3_block_mma_impl(uint16_t *smem, const floa
     // busy 55.7% against 67.6% on 32 x 32).  Pinned here: half 0's MFMAs each
     // followed by three VALU instructions of the split, then by one LDS
     // write; then half 1's B reads (they may not pass the LDS writes); then
-    // half 1's MFMAs with the next stage's global loads between them.
+    // half 1's MFMAs with the next stage's global loads between them.  Not
+    // for chained loops (the grouped large-tile launch): applied there, in
+    // the same region as the main body, the group ran 7.5% slower (339 ->
+    // 314 TFLOP/s, with 16 bytes of spills).
     auto q16_schedule = [&]() {
-        if constexpr (kQ16 && CK == kWaveFull && kFM == 2 && kFNc == 4) {
+        if constexpr (kQ16 && !Chain::kOn && CK == kWaveFull && kFM == 2 && kFNc == 4) {
 #pragma unroll
             for (int i = 0; i < 40; ++i) {
                 __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
